@@ -1,0 +1,1101 @@
+// refcpu — CPU restatement of Polyquant's configuration-interaction hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle that the product's HIP path is
+// checked against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg may load it; the product library (polyquant_amd/) never links or calls it.
+//
+// It restates, function by function, the reference's CPU algorithm (Polyquant v0.7.0):
+//   * strings / addresses / excitation maps ........ src/ci/detset/determinant_creation.cpp
+//   * bit operations (holes, parts, phase, occ/virt) src/ci/detset/determinant_operations.cpp
+//   * Slater-Condon rules .......................... src/ci/detset/slater_condon*.cpp
+//   * symmetric-scatter ("singleshot") sigma ....... src/ci/detset/direct_{single,mixed}particle.cpp
+//   * slow sigma ................................... src/ci/detset/direct.cpp:43-53
+//   * Davidson driver wiring ....................... src/ci/epci.cpp:589-621 (Spectra fork behaviour)
+//   * AO->MO quarter transform ..................... src/integral/integral.cpp:207-516
+//
+// Documented deviations (arithmetic is unchanged):
+//   1. Address <-> string-tuple lookups use flat arrays (O(1)) instead of the reference's
+//      unordered_map + linear scan det_idx_unfold (determinant_creation.cpp:378-388).
+//   2. Excitation maps are built by generating each string's excitations and looking them
+//      up in a hash map, instead of scanning the whole string list
+//      (determinant_creation.cpp:209-351); both give the ascending index list.
+//   3. Determinant strings are one 64-bit word (n_orb <= 63).  The free bit operations below
+//      stay multi-word so the reference's multi-word known answers can be checked.
+//   4. Errors return codes / set a message instead of APP_ABORT -> exit(1).
+//
+// Parity status: every function here is pinned in tests/ (known-answer vectors from
+// tests/unit_tests/ci/ci_test.cpp, brute-force second quantisation, exact diagonalisation,
+// closed-form transform identities).  The Davidson trajectory itself is parity-unpinned
+// (the Spectra fork is not available); only its converged eigenpairs are checked.
+#include <omp.h>
+
+#include <algorithm>
+#include <array>
+#include <bit>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+// ---------------------------------------------------------------------------------------
+// Free bit operations on multi-word determinants.  Word w holds orbitals
+// (W-1-w)*64 .. (W-1-w)*64+63 (determinant_creation.cpp:63-68).
+// ---------------------------------------------------------------------------------------
+
+// determinant_operations.cpp:5-11
+int single_spin_num_excitation(const uint64_t* Di, const uint64_t* Dj, int W) {
+  int deg = 0;
+  for (int i = 0; i < W; i++) deg += std::popcount(Di[i] ^ Dj[i]);
+  return deg / 2;
+}
+
+// determinant_operations.cpp:17-30
+void get_holes(const uint64_t* Di, const uint64_t* Dj, int W, std::vector<int>& holes) {
+  for (int i = 0; i < W; i++) {
+    uint64_t H = (Di[i] ^ Dj[i]) & Di[i];
+    while (H) {
+      int pos = std::countr_zero(H);
+      holes.push_back((W - i - 1) * 64 + pos);
+      H &= ~(1ull << pos);
+    }
+  }
+  std::sort(holes.begin(), holes.end());
+}
+
+// determinant_operations.cpp:32-45
+void get_parts(const uint64_t* Di, const uint64_t* Dj, int W, std::vector<int>& parts) {
+  for (int i = 0; i < W; i++) {
+    uint64_t P = (Di[i] ^ Dj[i]) & Dj[i];
+    while (P) {
+      int pos = std::countr_zero(P);
+      parts.push_back((W - i - 1) * 64 + pos);
+      P &= ~(1ull << pos);
+    }
+  }
+  std::sort(parts.begin(), parts.end());
+}
+
+// determinant_operations.cpp:47-116
+double get_phase(const uint64_t* Di, int W, const std::vector<int>& holes, const std::vector<int>& parts) {
+  uint64_t nperm = 0;
+  const uint64_t one = 1, zero = 0;
+  for (size_t l = 0; l < holes.size(); l++) {
+    uint64_t high = (uint64_t)std::max(parts[l], holes[l]);
+    uint64_t low = (uint64_t)std::min(parts[l], holes[l]) + one;
+    uint64_t j = low >> 6;
+    uint64_t k = high >> 6;
+    j = W - j - one;
+    k = W - k - one;
+    uint64_t m = high & 63;
+    uint64_t n = low & 63;
+    if (j == k) {
+      nperm += std::popcount(Di[j] & (((one << m) - one) & (~(one << n) + one)));
+    } else {
+      nperm += std::popcount(Di[j] & (~zero & (~(one << n) + one)));
+      nperm += std::popcount(Di[k] & ((one << m) - one));
+      for (int i = (int)k + 1; i < (int)j; i++) nperm += std::popcount(Di[i]);
+    }
+  }
+  if (holes.size() == 2) {
+    int a = std::min(holes[0], parts[0]);
+    int b = std::max(holes[0], parts[0]);
+    int c = std::min(holes[1], parts[1]);
+    int d = std::max(holes[1], parts[1]);
+    if ((a < c) && (c < b) && (b < d)) nperm++;
+  }
+  return (nperm & 1) ? -1.0 : 1.0;
+}
+
+// determinant_operations.cpp:118-137
+void get_occ_virt(const uint64_t* D, int W, int max_orb, std::vector<int>& occ, std::vector<int>& virt) {
+  for (int i = 0; i < W; i++) {
+    for (int j = 0; j < 64; j++) {
+      int orb = (W - i - 1) * 64 + j;
+      if (orb >= max_orb) break;
+      if ((D[i] >> j) & 1ull)
+        occ.push_back(orb);
+      else
+        virt.push_back(orb);
+    }
+  }
+  std::sort(occ.begin(), occ.end());
+  std::sort(virt.begin(), virt.end());
+}
+
+// symmetric_matrix_triangular_idx (src/integral/integral.hpp:14-20)
+inline int idx2(int i, int j) { return i > j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+// Lexicographic k-combinations of positions 0..n-1, the order of iter::combinations
+// (used at determinant_creation.cpp:58-59).
+template <class F>
+void for_each_combination(int n, int k, F&& f) {
+  if (k > n) return;
+  std::vector<int> c(k);
+  std::iota(c.begin(), c.end(), 0);
+  if (k == 0) { f(c); return; }
+  while (true) {
+    f(c);
+    int i = k - 1;
+    while (i >= 0 && c[i] == n - k + i) i--;
+    if (i < 0) break;
+    c[i]++;
+    for (int j = i + 1; j < k; j++) c[j] = c[j - 1] + 1;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// Determinant set
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+typedef struct {
+  int nspecies;
+  int n_orb[2];
+  int n_alpha[2];
+  int n_beta[2];
+  double charge[2];
+  int nspin_blocks[2];    // 1: one spatial-orbital set for both spins; 2: separate
+  int ex_level[2][3];     // per species: {max alpha level, max beta level, max total}
+  int max_collective;     // max summed excitation degree over species
+  const double* h[2][2];                // [species][spin]  n x n, column-major
+  const double* eri[2][2][2][2];        // [a][sa][b][sb]   npair_a x npair_b, column-major, b >= a
+} rc_desc;
+
+}  // extern "C"
+
+namespace {
+
+struct DetSet {
+  int nsp = 1;
+  int n_orb[2]{}, nocc[2][2]{}, nspin[2]{1, 1};
+  double charge[2]{-1.0, 1.0};
+  int ex[2][3]{};
+  int max_coll = 0;
+  std::vector<uint64_t> strings[2][2];
+  std::vector<std::vector<int>> singles[2][2], doubles[2][2];
+  int64_t nstr[4]{1, 1, 1, 1};
+  std::vector<int32_t> addr_of;                 // product index -> address or -1
+  std::vector<std::array<int32_t, 4>> unfold;   // address -> string tuple
+  int64_t N = 0;
+  std::vector<double> hdiag;
+  std::vector<double> h[2][2];
+  std::vector<double> eri[2][2][2][2];
+  int npair[2]{};
+
+  // integral accessors, spin index already reduced by (1 % nspin)
+  double H1(int p, int s, int i, int j) const { return h[p][s][i + (size_t)j * n_orb[p]]; }
+  double V(int a, int sa, int b, int sb, int r, int c) const {
+    return eri[a][sa][b][sb][(size_t)r + (size_t)c * npair[a]];
+  }
+  int bspin(int p) const { return 1 % nspin[p]; }
+
+  const uint64_t* det(int p, int s, int i) const { return &strings[p][s][i]; }
+  int64_t product_index(const int32_t* t) const {
+    return ((t[0] * nstr[1] + t[1]) * nstr[2] + t[2]) * nstr[3] + t[3];
+  }
+  int32_t address(const int32_t* t) const { return addr_of[product_index(t)]; }
+};
+
+// create_det (determinant_creation.cpp:6-43) and create_unique_excitation
+// (:139-146, driven by epci.cpp:366-379): HF string, then every level-k excitation in
+// iter::combinations order, k = 1..max level.
+bool build_strings(DetSet& d, int p, int s, int nocc, int max_level) {
+  const int n = d.n_orb[p];
+  uint64_t hf = (nocc == 0) ? 0ull : ((nocc >= 64) ? ~0ull : ((1ull << nocc) - 1));
+  auto& out = d.strings[p][s];
+  out.clear();
+  out.push_back(hf);
+  std::vector<int> occ, virt;
+  get_occ_virt(&hf, 1, n, occ, virt);
+  for (int k = 1; k <= max_level; k++) {
+    if (k > (int)virt.size()) {  // determinant_creation.cpp:54-56
+      g_err = "Excitation level exceeds virtual size!";
+      return false;
+    }
+    for_each_combination((int)occ.size(), k, [&](const std::vector<int>& oc) {
+      for_each_combination((int)virt.size(), k, [&](const std::vector<int>& vc) {
+        uint64_t t = hf;
+        for (int o : oc) t &= ~(1ull << occ[o]);
+        for (int v : vc) t |= 1ull << virt[v];
+        out.push_back(t);
+      });
+    });
+  }
+  return true;
+}
+
+// get_unique_excitation_set + create_unique_excitation_map_{singles,doubles}
+// (determinant_creation.cpp:74-102, 209-351): ascending indices of strings at exactly
+// `level` excitations.  Doubles are skipped when the HF string's word is < 2 (:306).
+void build_maps(DetSet& d, int p, int s) {
+  const auto& str = d.strings[p][s];
+  std::unordered_map<uint64_t, int> index;
+  index.reserve(str.size() * 2);
+  for (size_t i = 0; i < str.size(); i++) index[str[i]] = (int)i;
+  const int n = d.n_orb[p];
+  auto& S = d.singles[p][s];
+  auto& D = d.doubles[p][s];
+  S.assign(str.size(), {});
+  D.assign(str.size(), {});
+  const bool do_doubles = !(str[0] < 2);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < (int64_t)str.size(); i++) {
+    std::vector<int> occ, virt;
+    uint64_t I = str[i];
+    get_occ_virt(&I, 1, n, occ, virt);
+    std::vector<int> sl, dl;
+    for (int h : occ)
+      for (int q : virt) {
+        auto it = index.find((I & ~(1ull << h)) | (1ull << q));
+        if (it != index.end()) sl.push_back(it->second);
+      }
+    if (do_doubles && occ.size() >= 2 && virt.size() >= 2) {
+      for (size_t a = 0; a < occ.size(); a++)
+        for (size_t b = a + 1; b < occ.size(); b++)
+          for (size_t c = 0; c < virt.size(); c++)
+            for (size_t e = c + 1; e < virt.size(); e++) {
+              uint64_t J = (I & ~(1ull << occ[a]) & ~(1ull << occ[b])) | (1ull << virt[c]) | (1ull << virt[e]);
+              auto it = index.find(J);
+              if (it != index.end()) dl.push_back(it->second);
+            }
+    }
+    std::sort(sl.begin(), sl.end());
+    std::sort(dl.begin(), dl.end());
+    S[i] = std::move(sl);
+    D[i] = std::move(dl);
+  }
+}
+
+// create_excitation (determinant_creation.cpp:148-207): addresses in loop order
+// i(e alpha) -> j(e beta) -> k(p alpha) -> l(p beta), filtered by excitation degree.
+void build_addresses(DetSet& d) {
+  for (int q = 0; q < 4; q++) d.nstr[q] = 1;
+  d.nstr[0] = d.strings[0][0].size();
+  d.nstr[1] = d.strings[0][1].size();
+  if (d.nsp == 2) {
+    d.nstr[2] = d.strings[1][0].size();
+    d.nstr[3] = d.strings[1][1].size();
+  }
+  const int64_t P = d.nstr[0] * d.nstr[1] * d.nstr[2] * d.nstr[3];
+  d.addr_of.assign(P, -1);
+  d.unfold.clear();
+  auto deg = [&](int p, int i, int j) {
+    return single_spin_num_excitation(d.det(p, 0, 0), d.det(p, 0, i), 1) +
+           single_spin_num_excitation(d.det(p, 1, 0), d.det(p, 1, j), 1);
+  };
+  int64_t N = 0;
+  for (int32_t i = 0; i < d.nstr[0]; i++)
+    for (int32_t j = 0; j < d.nstr[1]; j++) {
+      int deg0 = deg(0, i, j);
+      if (deg0 > d.ex[0][2]) continue;
+      for (int32_t k = 0; k < d.nstr[2]; k++)
+        for (int32_t l = 0; l < d.nstr[3]; l++) {
+          int deg1 = 0;
+          if (d.nsp == 2) {
+            deg1 = deg(1, k, l);
+            if (deg1 > d.ex[1][2]) continue;
+          }
+          if (deg0 + deg1 > d.max_coll) continue;
+          std::array<int32_t, 4> t{i, j, k, l};
+          d.addr_of[d.product_index(t.data())] = (int32_t)N++;
+          d.unfold.push_back(t);
+        }
+    }
+  d.N = N;
+}
+
+// same_part_ham_diag (slater_condon_singleparticle.cpp:4-51)
+double same_part_ham_diag(const DetSet& d, int p, const int32_t* I) {
+  const int a = 0, b = d.bspin(p), n = d.n_orb[p];
+  std::vector<int> ao, av, bo, bv;
+  get_occ_virt(d.det(p, 0, I[2 * p]), 1, n, ao, av);
+  get_occ_virt(d.det(p, 1, I[2 * p + 1]), 1, n, bo, bv);
+  double e = 0.0;
+  for (int i : ao) e += d.H1(p, a, i, i);
+  for (int i : bo) e += d.H1(p, b, i, i);
+  for (int i : ao) {
+    for (int j : ao) {
+      e += 0.5 * d.V(p, a, p, a, idx2(i, i), idx2(j, j));
+      e -= 0.5 * d.V(p, a, p, a, idx2(i, j), idx2(j, i));
+    }
+    for (int j : bo) e += 0.5 * d.V(p, a, p, b, idx2(i, i), idx2(j, j));
+  }
+  for (int i : bo) {
+    for (int j : bo) {
+      e += 0.5 * d.V(p, b, p, b, idx2(i, i), idx2(j, j));
+      e -= 0.5 * d.V(p, b, p, b, idx2(i, j), idx2(j, i));
+    }
+    for (int j : ao) e += 0.5 * d.V(p, b, p, a, idx2(i, i), idx2(j, j));
+  }
+  return e;
+}
+
+// same_part_ham_single (slater_condon_singleparticle.cpp:53-105)
+double same_part_ham_single(const DetSet& d, int p, const int32_t* I, const int32_t* J) {
+  const int a = 0, b = d.bspin(p), n = d.n_orb[p];
+  const uint64_t* Ia = d.det(p, 0, I[2 * p]);
+  const uint64_t* Ib = d.det(p, 1, I[2 * p + 1]);
+  const uint64_t* Ja = d.det(p, 0, J[2 * p]);
+  const uint64_t* Jb = d.det(p, 1, J[2 * p + 1]);
+  int spin = (*Ia == *Ja) ? 1 : 0;
+  std::vector<int> ao, av, bo, bv, holes, parts;
+  get_occ_virt(Ia, 1, n, ao, av);
+  get_occ_virt(Ib, 1, n, bo, bv);
+  double e = 0.0;
+  if (spin == 0) {
+    get_holes(Ia, Ja, 1, holes);
+    get_parts(Ia, Ja, 1, parts);
+    double ph = get_phase(Ia, 1, holes, parts);
+    int h = holes[0], q = parts[0];
+    e += d.H1(p, a, h, q);
+    for (int i : ao) {
+      e += d.V(p, a, p, a, idx2(h, q), idx2(i, i));
+      e -= d.V(p, a, p, a, idx2(h, i), idx2(i, q));
+    }
+    for (int i : bo) e += d.V(p, a, p, b, idx2(h, q), idx2(i, i));
+    e *= ph;
+  } else {
+    get_holes(Ib, Jb, 1, holes);
+    get_parts(Ib, Jb, 1, parts);
+    double ph = get_phase(Ib, 1, holes, parts);
+    int h = holes[0], q = parts[0];
+    e += d.H1(p, b, h, q);
+    for (int i : bo) {
+      e += d.V(p, b, p, b, idx2(h, q), idx2(i, i));
+      e -= d.V(p, b, p, b, idx2(h, i), idx2(i, q));
+    }
+    for (int i : ao) e += d.V(p, b, p, a, idx2(h, q), idx2(i, i));
+    e *= ph;
+  }
+  return e;
+}
+
+// same_part_ham_double (slater_condon_singleparticle.cpp:107-175)
+double same_part_ham_double(const DetSet& d, int p, const int32_t* I, const int32_t* J) {
+  const int a = 0, b = d.bspin(p);
+  const uint64_t* Ia = d.det(p, 0, I[2 * p]);
+  const uint64_t* Ib = d.det(p, 1, I[2 * p + 1]);
+  const uint64_t* Ja = d.det(p, 0, J[2 * p]);
+  const uint64_t* Jb = d.det(p, 1, J[2 * p + 1]);
+  std::vector<int> holes, parts;
+  double e = 0.0;
+  if (*Ia == *Ja) {
+    get_holes(Ib, Jb, 1, holes);
+    get_parts(Ib, Jb, 1, parts);
+    double ph = get_phase(Ib, 1, holes, parts);
+    e += d.V(p, b, p, b, idx2(holes[0], parts[0]), idx2(holes[1], parts[1]));
+    e -= d.V(p, b, p, b, idx2(holes[0], parts[1]), idx2(holes[1], parts[0]));
+    e *= ph;
+  } else if (*Ib == *Jb) {
+    get_holes(Ia, Ja, 1, holes);
+    get_parts(Ia, Ja, 1, parts);
+    double ph = get_phase(Ia, 1, holes, parts);
+    e += d.V(p, a, p, a, idx2(holes[0], parts[0]), idx2(holes[1], parts[1]));
+    e -= d.V(p, a, p, a, idx2(holes[0], parts[1]), idx2(holes[1], parts[0]));
+    e *= ph;
+  } else {
+    std::vector<int> ah, ap, bh, bp;
+    get_holes(Ia, Ja, 1, ah);
+    get_parts(Ia, Ja, 1, ap);
+    get_holes(Ib, Jb, 1, bh);
+    get_parts(Ib, Jb, 1, bp);
+    double ph = get_phase(Ia, 1, ah, ap) * get_phase(Ib, 1, bh, bp);
+    e += d.V(p, a, p, b, idx2(ah[0], ap[0]), idx2(bh[0], bp[0]));
+    e *= ph;
+  }
+  return e;
+}
+
+// mixed_part_ham_diag (slater_condon_mixedparticle.cpp:4-52); no charge factor inside.
+double mixed_part_ham_diag(const DetSet& d, int p, int o, const int32_t* I) {
+  if (o < p) std::swap(p, o);
+  const int pa = 0, pb = d.bspin(p), oa = 0, ob = d.bspin(o);
+  std::vector<int> poa, pva, pob, pvb, ooa, ova, oob, ovb;
+  get_occ_virt(d.det(o, 0, I[2 * o]), 1, d.n_orb[o], ooa, ova);
+  get_occ_virt(d.det(o, 1, I[2 * o + 1]), 1, d.n_orb[o], oob, ovb);
+  get_occ_virt(d.det(p, 0, I[2 * p]), 1, d.n_orb[p], poa, pva);
+  get_occ_virt(d.det(p, 1, I[2 * p + 1]), 1, d.n_orb[p], pob, pvb);
+  double e = 0.0;
+  for (int i : poa) {
+    for (int j : ooa) e += d.V(p, pa, o, oa, idx2(i, i), idx2(j, j));
+    for (int j : oob) e += d.V(p, pa, o, ob, idx2(i, i), idx2(j, j));
+  }
+  for (int i : pob) {
+    for (int j : ooa) e += d.V(p, pb, o, oa, idx2(i, i), idx2(j, j));
+    for (int j : oob) e += d.V(p, pb, o, ob, idx2(i, i), idx2(j, j));
+  }
+  return e;
+}
+
+// mixed_part_ham_single (slater_condon_mixedparticle.cpp:54-145)
+double mixed_part_ham_single(const DetSet& d, int p, int o, const int32_t* I, const int32_t* J) {
+  if (o < p) std::swap(p, o);
+  const int pa = 0, pb = d.bspin(p), oa = 0, ob = d.bspin(o);
+  const uint64_t *pIa = d.det(p, 0, I[2 * p]), *pIb = d.det(p, 1, I[2 * p + 1]);
+  const uint64_t *pJa = d.det(p, 0, J[2 * p]), *pJb = d.det(p, 1, J[2 * p + 1]);
+  const uint64_t *oIa = d.det(o, 0, I[2 * o]), *oIb = d.det(o, 1, I[2 * o + 1]);
+  const uint64_t *oJa = d.det(o, 0, J[2 * o]), *oJb = d.det(o, 1, J[2 * o + 1]);
+  double e = 0.0;
+  std::vector<int> ao, av, bo, bv, holes, parts;
+  if (*oIa == *oJa && *oIb == *oJb) {
+    get_occ_virt(oIa, 1, d.n_orb[o], ao, av);
+    get_occ_virt(oIb, 1, d.n_orb[o], bo, bv);
+    double ph = 1.0;
+    if (*pIa == *pJa) {
+      get_holes(pIb, pJb, 1, holes);
+      get_parts(pIb, pJb, 1, parts);
+      ph *= get_phase(pIb, 1, holes, parts);
+      for (int i : ao) e += d.V(p, pb, o, oa, idx2(holes[0], parts[0]), idx2(i, i));
+      for (int i : bo) e += d.V(p, pb, o, ob, idx2(holes[0], parts[0]), idx2(i, i));
+    } else {
+      get_holes(pIa, pJa, 1, holes);
+      get_parts(pIa, pJa, 1, parts);
+      ph *= get_phase(pIa, 1, holes, parts);
+      for (int i : ao) e += d.V(p, pa, o, oa, idx2(holes[0], parts[0]), idx2(i, i));
+      for (int i : bo) e += d.V(p, pa, o, ob, idx2(holes[0], parts[0]), idx2(i, i));
+    }
+    e *= ph;
+  } else {
+    get_occ_virt(pIa, 1, d.n_orb[p], ao, av);
+    get_occ_virt(pIb, 1, d.n_orb[p], bo, bv);
+    double ph = 1.0;
+    if (*oIb == *oJb) {
+      get_holes(oIa, oJa, 1, holes);
+      get_parts(oIa, oJa, 1, parts);
+      ph *= get_phase(oIa, 1, holes, parts);
+      for (int i : ao) e += d.V(p, pa, o, oa, idx2(i, i), idx2(holes[0], parts[0]));
+      for (int i : bo) e += d.V(p, pb, o, oa, idx2(i, i), idx2(holes[0], parts[0]));
+    } else {
+      get_holes(oIb, oJb, 1, holes);
+      get_parts(oIb, oJb, 1, parts);
+      ph *= get_phase(oIb, 1, holes, parts);
+      for (int i : ao) e += d.V(p, pa, o, ob, idx2(i, i), idx2(holes[0], parts[0]));
+      for (int i : bo) e += d.V(p, pb, o, ob, idx2(i, i), idx2(holes[0], parts[0]));
+    }
+    e *= ph;
+  }
+  return e;
+}
+
+// mixed_part_ham_double (slater_condon_mixedparticle.cpp:147-236)
+double mixed_part_ham_double(const DetSet& d, int p, int o, const int32_t* I, const int32_t* J) {
+  if (o < p) std::swap(p, o);
+  const int pa = 0, pb = d.bspin(p), oa = 0, ob = d.bspin(o);
+  const uint64_t *pIa = d.det(p, 0, I[2 * p]), *pIb = d.det(p, 1, I[2 * p + 1]);
+  const uint64_t *pJa = d.det(p, 0, J[2 * p]), *pJb = d.det(p, 1, J[2 * p + 1]);
+  const uint64_t *oIa = d.det(o, 0, I[2 * o]), *oIb = d.det(o, 1, I[2 * o + 1]);
+  const uint64_t *oJa = d.det(o, 0, J[2 * o]), *oJb = d.det(o, 1, J[2 * o + 1]);
+  // which spin of each species is excited; both branches share the same arithmetic
+  int ps = -1, os = -1;
+  if (*pIa == *pJa && *oIa == *oJa) {
+    ps = 1; os = 1;
+  } else if (*pIb == *pJb && *oIb == *oJb) {
+    ps = 0; os = 0;
+  } else if (*pIa == *pJa && *oIb == *oJb) {
+    ps = 1; os = 0;
+  } else if (*pIb == *pJb && *oIa == *oJa) {
+    ps = 0; os = 1;
+  } else {
+    return 0.0;
+  }
+  const uint64_t* pI = ps ? pIb : pIa;
+  const uint64_t* pJ = ps ? pJb : pJa;
+  const uint64_t* oI = os ? oIb : oIa;
+  const uint64_t* oJ = os ? oJb : oJa;
+  std::vector<int> ph_, pp_, oh_, op_;
+  get_holes(pI, pJ, 1, ph_);
+  get_parts(pI, pJ, 1, pp_);
+  get_holes(oI, oJ, 1, oh_);
+  get_parts(oI, oJ, 1, op_);
+  double phase = get_phase(pI, 1, ph_, pp_) * get_phase(oI, 1, oh_, op_);
+  const int psb = ps ? pb : pa, osb = os ? ob : oa;
+  return phase * d.V(p, psb, o, osb, idx2(ph_[0], pp_[0]), idx2(oh_[0], op_[0]));
+}
+
+int num_excitation_pair(const DetSet& d, int p, const int32_t* I, const int32_t* J) {
+  return single_spin_num_excitation(d.det(p, 0, I[2 * p]), d.det(p, 0, J[2 * p]), 1) +
+         single_spin_num_excitation(d.det(p, 1, I[2 * p + 1]), d.det(p, 1, J[2 * p + 1]), 1);
+}
+
+bool same_strings(const DetSet& d, int p, const int32_t* I, const int32_t* J) {
+  return I[2 * p] == J[2 * p] && I[2 * p + 1] == J[2 * p + 1];
+}
+
+// Slater_Condon (slater_condon.cpp:4-100).  The mixed-species loop restates the
+// reference's behaviour: its `continue` precedes `other_idx_part++` (:54-56), so the
+// electron-positron term is added exactly once, with idx_part = 1, other_idx_part = 0.
+double slater_condon(const DetSet& d, int64_t i, int64_t j) {
+  if (i == j) return d.hdiag[i];
+  const int32_t* I = d.unfold[i].data();
+  const int32_t* J = d.unfold[j].data();
+  double e = 0.0;
+  for (int p = 0; p < d.nsp; p++) {
+    bool others_equal = true;
+    for (int o = 0; o < d.nsp; o++)
+      if (o != p && !same_strings(d, o, I, J)) others_equal = false;
+    if (others_equal) {
+      int lvl = num_excitation_pair(d, p, I, J);
+      if (lvl == 0)
+        e += same_part_ham_diag(d, p, I);
+      else if (lvl == 1)
+        e += same_part_ham_single(d, p, I, J);
+      else if (lvl == 2)
+        e += same_part_ham_double(d, p, I, J);
+    }
+  }
+  if (d.nsp == 2) {
+    const int p = 1, o = 0;
+    int ei = num_excitation_pair(d, p, I, J);
+    int eo = num_excitation_pair(d, o, I, J);
+    double cf = d.charge[p] * d.charge[o];
+    if (ei + eo < 3) {
+      if (ei == 0 && eo == 0)
+        e += cf * mixed_part_ham_diag(d, p, o, I);
+      else if (ei == 1 && eo == 1)
+        e += cf * mixed_part_ham_double(d, p, o, I, J);
+      else if (ei == 1 || eo == 1)
+        e += cf * mixed_part_ham_single(d, p, o, I, J);
+    }
+  }
+  return e;
+}
+
+// precompute_diagonal_Slater_Condon (slater_condon.cpp:102-153); the mixed diagonal is
+// added once for the same reason as above.
+void build_diag(DetSet& d) {
+  d.hdiag.assign(d.N, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < d.N; i++) {
+    const int32_t* I = d.unfold[i].data();
+    double e = 0.0;
+    for (int p = 0; p < d.nsp; p++) e += same_part_ham_diag(d, p, I);
+    if (d.nsp == 2) e += d.charge[0] * d.charge[1] * mixed_part_ham_diag(d, 1, 0, I);
+    d.hdiag[i] = e;
+  }
+}
+
+// --- singleshot sigmas -----------------------------------------------------------------
+// Both restate the reference's symmetric scatter: each connected pair is visited once
+// (J index > I index on the first differing string), H_IJ is evaluated from I, and the
+// contribution is added to sigma(I) and sigma(J).  OpenMP threads take i_det round-robin
+// and own a private full-size sigma, reduced at the end
+// (direct_singleparticle.cpp:248-256,369-370).
+
+struct Scatter {
+  const double* C;
+  int64_t ldc;
+  int k;
+  double* S;  // thread-private, N x k
+  int64_t N;
+  inline void add(int64_t i, int64_t j, double v) const {
+    if (v == 0.0) return;
+    for (int s = 0; s < k; s++) {
+      S[i + s * N] += v * C[j + s * ldc];
+      S[j + s * N] += v * C[i + s * ldc];
+    }
+  }
+};
+
+// sigma_one_species_class_singleshot (direct_singleparticle.cpp:239-372)
+void sigma_one(const DetSet& d, const double* C, int64_t ldc, int k, double* out, int64_t lds) {
+  const int64_t N = d.N;
+  int nth = omp_get_max_threads();
+  std::vector<std::vector<double>> priv(nth);
+#pragma omp parallel
+  {
+    int tid = omp_get_thread_num();
+    int nt = omp_get_num_threads();
+    priv[tid].assign((size_t)N * k, 0.0);
+    Scatter sc{C, ldc, k, priv[tid].data(), N};
+    for (int64_t i = tid; i < N; i += nt) {
+      const int32_t* I = d.unfold[i].data();
+      const int32_t IA = I[0], IB = I[1];
+      for (int s = 0; s < k; s++) priv[tid][i + s * N] += d.hdiag[i] * C[i + s * ldc];
+      for (int JA : d.singles[0][0][IA]) {
+        if (JA < IA) continue;
+        int32_t J[4] = {JA, IB, 0, 0};
+        int32_t aj = d.address(J);
+        if (aj >= 0) sc.add(i, aj, same_part_ham_single(d, 0, I, J));
+        for (int JB : d.singles[0][1][IB]) {
+          int32_t J2[4] = {JA, JB, 0, 0};
+          int32_t a2 = d.address(J2);
+          if (a2 >= 0) sc.add(i, a2, same_part_ham_double(d, 0, I, J2));
+        }
+      }
+      for (int JB : d.singles[0][1][IB]) {
+        if (JB < IB) continue;
+        int32_t J[4] = {IA, JB, 0, 0};
+        int32_t aj = d.address(J);
+        if (aj >= 0) sc.add(i, aj, same_part_ham_single(d, 0, I, J));
+      }
+      for (int JA : d.doubles[0][0][IA]) {
+        if (JA < IA) continue;
+        int32_t J[4] = {JA, IB, 0, 0};
+        int32_t aj = d.address(J);
+        if (aj >= 0) sc.add(i, aj, same_part_ham_double(d, 0, I, J));
+      }
+      for (int JB : d.doubles[0][1][IB]) {
+        if (JB < IB) continue;
+        int32_t J[4] = {IA, JB, 0, 0};
+        int32_t aj = d.address(J);
+        if (aj >= 0) sc.add(i, aj, same_part_ham_double(d, 0, I, J));
+      }
+    }
+  }
+  for (int s = 0; s < k; s++)
+    for (int64_t i = 0; i < N; i++) {
+      double v = 0.0;
+      for (int t = 0; t < nth; t++)
+        if (!priv[t].empty()) v += priv[t][i + s * N];
+      out[i + s * lds] = v;
+    }
+}
+
+// sigma_two_species_class_singleshot (direct_mixedparticle.cpp:270-611)
+void sigma_two(const DetSet& d, const double* C, int64_t ldc, int k, double* out, int64_t lds) {
+  const int64_t N = d.N;
+  const double cf = d.charge[0] * d.charge[1];
+  int nth = omp_get_max_threads();
+  std::vector<std::vector<double>> priv(nth);
+#pragma omp parallel
+  {
+    int tid = omp_get_thread_num();
+    int nt = omp_get_num_threads();
+    priv[tid].assign((size_t)N * k, 0.0);
+    Scatter sc{C, ldc, k, priv[tid].data(), N};
+    for (int64_t i = tid; i < N; i += nt) {
+      const int32_t* I = d.unfold[i].data();
+      const int32_t A = I[0], B = I[1], Cc = I[2], D = I[3];
+      for (int s = 0; s < k; s++) priv[tid][i + s * N] += d.hdiag[i] * C[i + s * ldc];
+      auto visit = [&](int32_t a, int32_t b, int32_t c, int32_t e, auto&& elem) {
+        int32_t J[4] = {a, b, c, e};
+        int32_t aj = d.address(J);
+        if (aj >= 0) sc.add(i, aj, elem(J));
+      };
+      // part 0 spin 0 singles and the doubles nested in them (:305-384)
+      for (int JA : d.singles[0][0][A]) {
+        if (JA < A) continue;
+        visit(JA, B, Cc, D, [&](const int32_t* J) {
+          return same_part_ham_single(d, 0, I, J) + cf * mixed_part_ham_single(d, 0, 1, I, J);
+        });
+        for (int JB : d.singles[0][1][B])
+          visit(JA, JB, Cc, D, [&](const int32_t* J) {
+            return same_part_ham_double(d, 0, I, J) + cf * mixed_part_ham_double(d, 0, 1, I, J);
+          });
+        for (int JC : d.singles[1][0][Cc])
+          visit(JA, B, JC, D, [&](const int32_t* J) { return cf * mixed_part_ham_double(d, 0, 1, I, J); });
+        for (int JD : d.singles[1][1][D])
+          visit(JA, B, Cc, JD, [&](const int32_t* J) { return cf * mixed_part_ham_double(d, 0, 1, I, J); });
+      }
+      // part 0 spin 1 singles (:386-448)
+      for (int JB : d.singles[0][1][B]) {
+        if (JB < B) continue;
+        visit(A, JB, Cc, D, [&](const int32_t* J) {
+          return same_part_ham_single(d, 0, I, J) + cf * mixed_part_ham_single(d, 0, 1, I, J);
+        });
+        for (int JC : d.singles[1][0][Cc])
+          visit(A, JB, JC, D, [&](const int32_t* J) { return cf * mixed_part_ham_double(d, 0, 1, I, J); });
+        for (int JD : d.singles[1][1][D])
+          visit(A, JB, Cc, JD, [&](const int32_t* J) { return cf * mixed_part_ham_double(d, 0, 1, I, J); });
+      }
+      // part 1 spin 0 singles (:450-493)
+      for (int JC : d.singles[1][0][Cc]) {
+        if (JC < Cc) continue;
+        visit(A, B, JC, D, [&](const int32_t* J) {
+          return same_part_ham_single(d, 1, I, J) + cf * mixed_part_ham_single(d, 0, 1, I, J);
+        });
+        for (int JD : d.singles[1][1][D])
+          visit(A, B, JC, JD, [&](const int32_t* J) {
+            return same_part_ham_double(d, 1, I, J) + cf * mixed_part_ham_double(d, 0, 1, I, J);
+          });
+      }
+      // part 1 spin 1 singles (:495-517)
+      for (int JD : d.singles[1][1][D]) {
+        if (JD < D) continue;
+        visit(A, B, Cc, JD, [&](const int32_t* J) {
+          return same_part_ham_single(d, 1, I, J) + cf * mixed_part_ham_single(d, 0, 1, I, J);
+        });
+      }
+      // same-string doubles for each of the four strings (:519-605)
+      for (int JA : d.doubles[0][0][A])
+        if (JA >= A) visit(JA, B, Cc, D, [&](const int32_t* J) { return same_part_ham_double(d, 0, I, J); });
+      for (int JB : d.doubles[0][1][B])
+        if (JB >= B) visit(A, JB, Cc, D, [&](const int32_t* J) { return same_part_ham_double(d, 0, I, J); });
+      for (int JC : d.doubles[1][0][Cc])
+        if (JC >= Cc) visit(A, B, JC, D, [&](const int32_t* J) { return same_part_ham_double(d, 1, I, J); });
+      for (int JD : d.doubles[1][1][D])
+        if (JD >= D) visit(A, B, Cc, JD, [&](const int32_t* J) { return same_part_ham_double(d, 1, I, J); });
+    }
+  }
+  for (int s = 0; s < k; s++)
+    for (int64_t i = 0; i < N; i++) {
+      double v = 0.0;
+      for (int t = 0; t < nth; t++)
+        if (!priv[t].empty()) v += priv[t][i + s * N];
+      out[i + s * lds] = v;
+    }
+}
+
+// --- small dense symmetric eigensolver (cyclic Jacobi), used by the Davidson below ------
+void jacobi_eigh(int m, std::vector<double>& A, std::vector<double>& evals, std::vector<double>& evecs) {
+  evecs.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; i++) evecs[i + i * m] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < m; p++)
+      for (int q = p + 1; q < m; q++) off += A[p + q * m] * A[p + q * m];
+    if (off < 1e-30) break;
+    for (int p = 0; p < m; p++)
+      for (int q = p + 1; q < m; q++) {
+        double apq = A[p + q * m];
+        if (std::fabs(apq) < 1e-300) continue;
+        double app = A[p + p * m], aqq = A[q + q * m];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int r = 0; r < m; r++) {
+          double arp = A[r + p * m], arq = A[r + q * m];
+          A[r + p * m] = c * arp - s * arq;
+          A[r + q * m] = s * arp + c * arq;
+        }
+        for (int r = 0; r < m; r++) {
+          double apr = A[p + r * m], aqr = A[q + r * m];
+          A[p + r * m] = c * apr - s * aqr;
+          A[q + r * m] = s * apr + c * aqr;
+        }
+        for (int r = 0; r < m; r++) {
+          double vp = evecs[r + p * m], vq = evecs[r + q * m];
+          evecs[r + p * m] = c * vp - s * vq;
+          evecs[r + q * m] = s * vp + c * vq;
+        }
+      }
+  }
+  std::vector<int> order(m);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return A[x + x * m] < A[y + y * m]; });
+  std::vector<double> V2((size_t)m * m);
+  evals.resize(m);
+  for (int c = 0; c < m; c++) {
+    evals[c] = A[order[c] + order[c] * m];
+    for (int r = 0; r < m; r++) V2[r + c * m] = evecs[r + order[c] * m];
+  }
+  evecs.swap(V2);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C API (ctypes)
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char* rc_last_error(void) { return g_err.c_str(); }
+
+void* rc_create(const rc_desc* desc) {
+  auto* d = new DetSet();
+  d->nsp = desc->nspecies;
+  if (d->nsp < 1 || d->nsp > 2) {
+    g_err = "The CI code can currently handle up to a maximum of 2 unique quantum particle types.";
+    delete d;
+    return nullptr;
+  }
+  d->max_coll = desc->max_collective;
+  for (int p = 0; p < d->nsp; p++) {
+    d->n_orb[p] = desc->n_orb[p];
+    if (d->n_orb[p] < 1 || d->n_orb[p] > 63) {
+      g_err = "n_orb must be in 1..63 (one 64-bit word per string)";
+      delete d;
+      return nullptr;
+    }
+    d->nocc[p][0] = desc->n_alpha[p];
+    d->nocc[p][1] = desc->n_beta[p];
+    d->charge[p] = desc->charge[p];
+    d->nspin[p] = desc->nspin_blocks[p];
+    for (int q = 0; q < 3; q++) d->ex[p][q] = desc->ex_level[p][q];
+    d->npair[p] = d->n_orb[p] * (d->n_orb[p] + 1) / 2;
+  }
+  for (int p = 0; p < d->nsp; p++)
+    for (int s = 0; s < d->nspin[p]; s++)
+      d->h[p][s].assign(desc->h[p][s], desc->h[p][s] + (size_t)d->n_orb[p] * d->n_orb[p]);
+  for (int a = 0; a < d->nsp; a++)
+    for (int sa = 0; sa < d->nspin[a]; sa++)
+      for (int b = a; b < d->nsp; b++)
+        for (int sb = 0; sb < d->nspin[b]; sb++) {
+          const double* src = desc->eri[a][sa][b][sb];
+          if (!src) continue;
+          d->eri[a][sa][b][sb].assign(src, src + (size_t)d->npair[a] * d->npair[b]);
+        }
+  for (int p = 0; p < d->nsp; p++)
+    for (int s = 0; s < 2; s++)
+      if (!build_strings(*d, p, s, d->nocc[p][s], d->ex[p][s])) {
+        delete d;
+        return nullptr;
+      }
+  build_addresses(*d);
+  for (int p = 0; p < d->nsp; p++)
+    for (int s = 0; s < 2; s++) build_maps(*d, p, s);
+  build_diag(*d);
+  return d;
+}
+
+void rc_destroy(void* h) { delete static_cast<DetSet*>(h); }
+int64_t rc_ndets(void* h) { return static_cast<DetSet*>(h)->N; }
+int64_t rc_nstrings(void* h, int p, int s) { return (int64_t) static_cast<DetSet*>(h)->strings[p][s].size(); }
+void rc_strings(void* h, int p, int s, uint64_t* out) {
+  auto& v = static_cast<DetSet*>(h)->strings[p][s];
+  std::memcpy(out, v.data(), v.size() * sizeof(uint64_t));
+}
+void rc_unfold(void* h, int64_t addr, int32_t* out4) {
+  auto& t = static_cast<DetSet*>(h)->unfold[addr];
+  for (int q = 0; q < 4; q++) out4[q] = t[q];
+}
+// Writes all addresses' string tuples (N x 4, row-major).
+void rc_unfold_all(void* h, int32_t* out) {
+  auto* d = static_cast<DetSet*>(h);
+  std::memcpy(out, d->unfold.data(), d->unfold.size() * sizeof(int32_t) * 4);
+}
+int64_t rc_address(void* h, const int32_t* t4) { return static_cast<DetSet*>(h)->address(t4); }
+// which: 1 singles, 2 doubles; returns the list length, copies up to cap entries.
+int rc_excitations(void* h, int which, int p, int s, int idx, int32_t* out, int cap) {
+  auto* d = static_cast<DetSet*>(h);
+  const auto& L = (which == 1 ? d->singles : d->doubles)[p][s][idx];
+  for (int i = 0; i < (int)L.size() && i < cap; i++) out[i] = L[i];
+  return (int)L.size();
+}
+void rc_diag(void* h, double* out) {
+  auto* d = static_cast<DetSet*>(h);
+  std::memcpy(out, d->hdiag.data(), d->N * sizeof(double));
+}
+double rc_slater_condon(void* h, int64_t i, int64_t j) { return slater_condon(*static_cast<DetSet*>(h), i, j); }
+
+// operator* / create_sigma (direct.cpp:5-34)
+int rc_sigma(void* h, const double* C, int64_t ldc, int k, double* S, int64_t lds) {
+  auto* d = static_cast<DetSet*>(h);
+  if (d->nsp == 1)
+    sigma_one(*d, C, ldc, k, S, lds);
+  else
+    sigma_two(*d, C, ldc, k, S, lds);
+  return 0;
+}
+
+// create_sigma_slow (direct.cpp:43-53): sigma(i) = sum_k SC(i,k) C(k)
+int rc_sigma_slow(void* h, const double* C, int64_t ldc, int k, double* S, int64_t lds) {
+  auto* d = static_cast<DetSet*>(h);
+  for (int64_t i = 0; i < d->N; i++)
+    for (int s = 0; s < k; s++) {
+      double v = 0.0;
+#pragma omp parallel for reduction(+ : v)
+      for (int64_t q = 0; q < d->N; q++) {
+        double e = slater_condon(*d, i, q);
+        if (e != 0.0) v += e * C[q + s * ldc];
+      }
+      S[i + s * lds] = v;
+    }
+  return 0;
+}
+
+// Davidson, restating the eigensolver the reference drives at epci.cpp:589-608
+// (Spectra DavidsonSymEigsSolver, SmallestAlge): unit-vector start at the nvec_init
+// smallest diagonal entries, Rayleigh-Ritz, residual test on the first nev Ritz pairs,
+// diagonal-preconditioned corrections, restart to nvec_init Ritz vectors past nvec_max.
+// Returns the iteration count (negative if not converged) and the Ritz values/vectors.
+int rc_davidson(void* h, int nev, int nvec_init, int nvec_max, int maxit, double tol, double* evals_out,
+                double* evecs_out, double* resid_out) {
+  auto* d = static_cast<DetSet*>(h);
+  const int64_t N = d->N;
+  if (nvec_init > N) nvec_init = (int)N;
+  if (nvec_max > N) nvec_max = (int)N;
+  if (nev > nvec_init) nvec_init = nev;
+  std::vector<double> V, W;  // column-major, N x m
+  int m = 0;
+  auto dot = [&](const double* x, const double* y) {
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s)
+    for (int64_t i = 0; i < N; i++) s += x[i] * y[i];
+    return s;
+  };
+  // returns false if the vector vanished after orthogonalisation
+  auto add_vector = [&](std::vector<double>& t) {
+    for (int pass = 0; pass < 2; pass++)
+      for (int c = 0; c < m; c++) {
+        double a = dot(&V[(size_t)c * N], t.data());
+#pragma omp parallel for
+        for (int64_t i = 0; i < N; i++) t[i] -= a * V[(size_t)c * N + i];
+      }
+    double nrm = std::sqrt(dot(t.data(), t.data()));
+    if (nrm < 1e-10) return false;
+    for (auto& x : t) x /= nrm;
+    V.insert(V.end(), t.begin(), t.end());
+    m++;
+    return true;
+  };
+  {
+    std::vector<int64_t> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::partial_sort(order.begin(), order.begin() + nvec_init, order.end(),
+                      [&](int64_t a, int64_t b) { return d->hdiag[a] < d->hdiag[b]; });
+    for (int c = 0; c < nvec_init; c++) {
+      std::vector<double> t(N, 0.0);
+      t[order[c]] = 1.0;
+      add_vector(t);
+    }
+  }
+  int nW = 0;
+  std::vector<double> theta, Y, X((size_t)N * nev), AX((size_t)N * nev), R((size_t)N * nev);
+  std::vector<double> rn(nev);
+  for (int iter = 1; iter <= maxit; iter++) {
+    if (nW < m) {
+      W.resize((size_t)N * m);
+      rc_sigma(h, &V[(size_t)nW * N], N, m - nW, &W[(size_t)nW * N], N);
+      nW = m;
+    }
+    std::vector<double> G((size_t)m * m);
+    for (int a = 0; a < m; a++)
+      for (int b = a; b < m; b++) {
+        double g = 0.5 * (dot(&V[(size_t)a * N], &W[(size_t)b * N]) + dot(&V[(size_t)b * N], &W[(size_t)a * N]));
+        G[a + b * m] = G[b + a * m] = g;
+      }
+    jacobi_eigh(m, G, theta, Y);
+#pragma omp parallel for
+    for (int64_t i = 0; i < N; i++)
+      for (int k = 0; k < nev; k++) {
+        double x = 0.0, ax = 0.0;
+        for (int c = 0; c < m; c++) {
+          x += V[(size_t)c * N + i] * Y[c + k * m];
+          ax += W[(size_t)c * N + i] * Y[c + k * m];
+        }
+        X[i + k * N] = x;
+        AX[i + k * N] = ax;
+        R[i + k * N] = ax - theta[k] * x;
+      }
+    bool conv = true;
+    for (int k = 0; k < nev; k++) {
+      rn[k] = std::sqrt(dot(&R[(size_t)k * N], &R[(size_t)k * N]));
+      if (!(rn[k] < tol)) conv = false;
+    }
+    if (conv || iter == maxit) {
+      for (int k = 0; k < nev; k++) {
+        evals_out[k] = theta[k];
+        resid_out[k] = rn[k];
+      }
+      std::memcpy(evecs_out, X.data(), X.size() * sizeof(double));
+      return conv ? iter : -iter;
+    }
+    if (m + nev > nvec_max) {  // restart on the current Ritz vectors
+      const int keep = std::max(nvec_init, nev);
+      std::vector<double> V2((size_t)N * keep), W2((size_t)N * keep);
+#pragma omp parallel for
+      for (int64_t i = 0; i < N; i++)
+        for (int k = 0; k < keep; k++) {
+          double x = 0.0, ax = 0.0;
+          for (int c = 0; c < m; c++) {
+            x += V[(size_t)c * N + i] * Y[c + k * m];
+            ax += W[(size_t)c * N + i] * Y[c + k * m];
+          }
+          V2[i + (size_t)k * N] = x;
+          W2[i + (size_t)k * N] = ax;
+        }
+      V.swap(V2);
+      W.swap(W2);
+      m = nW = keep;
+    }
+    for (int k = 0; k < nev; k++) {
+      std::vector<double> t(N);
+      for (int64_t i = 0; i < N; i++) {
+        double den = theta[k] - d->hdiag[i];
+        if (std::fabs(den) < 1e-8) den = (den < 0 ? -1e-8 : 1e-8);
+        t[i] = R[i + (size_t)k * N] / den;
+      }
+      add_vector(t);
+    }
+  }
+  return -maxit;
+}
+
+// Free bit operations for the known-answer tests (multi-word).
+int rc_get_holes(const uint64_t* Di, const uint64_t* Dj, int W, int* out) {
+  std::vector<int> v;
+  get_holes(Di, Dj, W, v);
+  std::copy(v.begin(), v.end(), out);
+  return (int)v.size();
+}
+int rc_get_parts(const uint64_t* Di, const uint64_t* Dj, int W, int* out) {
+  std::vector<int> v;
+  get_parts(Di, Dj, W, v);
+  std::copy(v.begin(), v.end(), out);
+  return (int)v.size();
+}
+double rc_get_phase(const uint64_t* Di, int W, const int* holes, const int* parts, int n) {
+  std::vector<int> hv(holes, holes + n), pv(parts, parts + n);
+  return get_phase(Di, W, hv, pv);
+}
+void rc_get_occ_virt(const uint64_t* D, int W, int max_orb, int* occ, int* nocc, int* virt, int* nvirt) {
+  std::vector<int> o, v;
+  get_occ_virt(D, W, max_orb, o, v);
+  std::copy(o.begin(), o.end(), occ);
+  std::copy(v.begin(), v.end(), virt);
+  *nocc = (int)o.size();
+  *nvirt = (int)v.size();
+}
+int rc_num_excitation(const uint64_t* Di, const uint64_t* Dj, int W) { return single_spin_num_excitation(Di, Dj, W); }
+
+// AO->MO transform restated from transform_mo_2_body_integrals (integral.cpp:207-477):
+// four quarter transforms over a full AO tensor (the reference recomputes AO integrals with
+// libint in step 1; here they are an input).  ao: full n_ao^4 tensor (pq|rs) row-major over
+// p,q,r,s.  Output: npair_a x npair_b column-major, eri(idx2(i,j), idx2(k,l)).
+void rc_ao2mo(int nao, int nmo_a, int nmo_b, const double* ao, const double* Ca, const double* Cb, double* out) {
+  const int64_t n = nao;
+  std::vector<double> t1((size_t)nmo_a * n * n * n, 0.0);
+#pragma omp parallel for collapse(2)
+  for (int i = 0; i < nmo_a; i++)
+    for (int64_t q = 0; q < n; q++)
+      for (int64_t r = 0; r < n; r++)
+        for (int64_t s = 0; s < n; s++) {
+          double v = 0.0;
+          for (int64_t p = 0; p < n; p++) v += Ca[p + i * n] * ao[((p * n + q) * n + r) * n + s];
+          t1[((i * n + q) * n + r) * n + s] = v;
+        }
+  std::vector<double> t2((size_t)nmo_a * nmo_a * n * n, 0.0);
+#pragma omp parallel for collapse(2)
+  for (int i = 0; i < nmo_a; i++)
+    for (int j = 0; j < nmo_a; j++)
+      for (int64_t r = 0; r < n; r++)
+        for (int64_t s = 0; s < n; s++) {
+          double v = 0.0;
+          for (int64_t q = 0; q < n; q++) v += Ca[q + j * n] * t1[((i * n + q) * n + r) * n + s];
+          t2[(((int64_t)i * nmo_a + j) * n + r) * n + s] = v;
+        }
+  std::vector<double> t3((size_t)nmo_a * nmo_a * nmo_b * n, 0.0);
+#pragma omp parallel for collapse(2)
+  for (int i = 0; i < nmo_a; i++)
+    for (int j = 0; j < nmo_a; j++)
+      for (int k = 0; k < nmo_b; k++)
+        for (int64_t s = 0; s < n; s++) {
+          double v = 0.0;
+          for (int64_t r = 0; r < n; r++) v += Cb[r + k * n] * t2[(((int64_t)i * nmo_a + j) * n + r) * n + s];
+          t3[(((int64_t)i * nmo_a + j) * nmo_b + k) * n + s] = v;
+        }
+  const int64_t npa = (int64_t)nmo_a * (nmo_a + 1) / 2;
+#pragma omp parallel for collapse(2)
+  for (int i = 0; i < nmo_a; i++)
+    for (int j = 0; j < nmo_a; j++) {
+      if (j < i) continue;
+      for (int k = 0; k < nmo_b; k++)
+        for (int l = k; l < nmo_b; l++) {
+          double v = 0.0;
+          for (int64_t s = 0; s < n; s++) v += Cb[s + l * n] * t3[(((int64_t)i * nmo_a + j) * nmo_b + k) * n + s];
+          out[idx2(i, j) + (int64_t)idx2(k, l) * npa] = v;
+        }
+    }
+}
+
+}  // extern "C"

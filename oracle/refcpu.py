@@ -1,0 +1,238 @@
+"""ctypes wrapper for the oracle (``librefcpu.so``, a CPU restatement of the reference CI).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librefcpu.so")
+_lib = None
+
+_dp = ct.POINTER(ct.c_double)
+
+
+class RcDesc(ct.Structure):
+    _fields_ = [
+        ("nspecies", ct.c_int),
+        ("n_orb", ct.c_int * 2),
+        ("n_alpha", ct.c_int * 2),
+        ("n_beta", ct.c_int * 2),
+        ("charge", ct.c_double * 2),
+        ("nspin_blocks", ct.c_int * 2),
+        ("ex_level", (ct.c_int * 3) * 2),
+        ("max_collective", ct.c_int),
+        ("h", (_dp * 2) * 2),
+        ("eri", (((_dp * 2) * 2) * 2) * 2),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ct.CDLL(_LIB_PATH)
+        L.rc_create.restype = ct.c_void_p
+        L.rc_create.argtypes = [ct.POINTER(RcDesc)]
+        L.rc_destroy.argtypes = [ct.c_void_p]
+        L.rc_last_error.restype = ct.c_char_p
+        for f in ("rc_ndets", "rc_nstrings", "rc_address"):
+            getattr(L, f).restype = ct.c_int64
+        L.rc_ndets.argtypes = [ct.c_void_p]
+        L.rc_nstrings.argtypes = [ct.c_void_p, ct.c_int, ct.c_int]
+        L.rc_strings.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p]
+        L.rc_unfold_all.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.rc_address.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.rc_excitations.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_int]
+        L.rc_diag.argtypes = [ct.c_void_p, ct.c_void_p]
+        L.rc_slater_condon.restype = ct.c_double
+        L.rc_slater_condon.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int64]
+        L.rc_sigma.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64]
+        L.rc_sigma_slow.argtypes = L.rc_sigma.argtypes
+        L.rc_davidson.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double,
+                                  ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.rc_get_holes.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p]
+        L.rc_get_parts.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p]
+        L.rc_get_phase.restype = ct.c_double
+        L.rc_get_phase.argtypes = [ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int]
+        L.rc_get_occ_virt.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                      ct.c_void_p]
+        L.rc_num_excitation.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int]
+        L.rc_ao2mo.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+# --- free bit operations (multi-word determinants) -------------------------------------
+
+def _words(d):
+    return np.ascontiguousarray(np.asarray(d, dtype=np.uint64))
+
+
+def get_holes(di, dj):
+    di, dj = _words(di), _words(dj)
+    out = np.zeros(64 * len(di), dtype=np.int32)
+    n = lib().rc_get_holes(_ptr(di), _ptr(dj), len(di), _ptr(out))
+    return [int(x) for x in out[:n]]
+
+
+def get_parts(di, dj):
+    di, dj = _words(di), _words(dj)
+    out = np.zeros(64 * len(di), dtype=np.int32)
+    n = lib().rc_get_parts(_ptr(di), _ptr(dj), len(di), _ptr(out))
+    return [int(x) for x in out[:n]]
+
+
+def get_phase(di, holes, parts):
+    di = _words(di)
+    h = np.asarray(holes, dtype=np.int32)
+    p = np.asarray(parts, dtype=np.int32)
+    return lib().rc_get_phase(_ptr(di), len(di), _ptr(h), _ptr(p), len(h))
+
+
+def get_occ_virt(d, max_orb):
+    d = _words(d)
+    occ = np.zeros(64 * len(d), dtype=np.int32)
+    virt = np.zeros(64 * len(d), dtype=np.int32)
+    no, nv = ct.c_int(), ct.c_int()
+    lib().rc_get_occ_virt(_ptr(d), len(d), max_orb, _ptr(occ), ct.byref(no), _ptr(virt), ct.byref(nv))
+    return [int(x) for x in occ[: no.value]], [int(x) for x in virt[: nv.value]]
+
+
+def num_excitation(di, dj):
+    di, dj = _words(di), _words(dj)
+    return lib().rc_num_excitation(_ptr(di), _ptr(dj), len(di))
+
+
+# --- determinant set --------------------------------------------------------------------
+
+class OracleCI:
+    """The reference's CI determinant set and operators, restated on the CPU."""
+
+    def __init__(self, prob):
+        self.prob = prob
+        d = RcDesc()
+        d.nspecies = prob.nspecies
+        d.max_collective = min(int(prob.max_collective), 2**31 - 1)
+        self._keep = []
+        for p, sp in enumerate(prob.species):
+            d.n_orb[p] = sp.n_orb
+            d.n_alpha[p] = sp.n_alpha
+            d.n_beta[p] = sp.n_beta
+            d.charge[p] = sp.charge
+            d.nspin_blocks[p] = sp.nspin_blocks
+            lv = sp.levels()
+            for q in range(3):
+                d.ex_level[p][q] = int(lv[q])
+            for s in range(sp.nspin_blocks):
+                h = np.asfortranarray(prob.h[p][s], dtype=np.float64)
+                self._keep.append(h)
+                d.h[p][s] = h.ctypes.data_as(_dp)
+        for (a, sa, b, sb), v in prob.eri.items():
+            arr = np.asfortranarray(v, dtype=np.float64)
+            self._keep.append(arr)
+            d.eri[a][sa][b][sb] = arr.ctypes.data_as(_dp)
+        self.desc = d
+        self.h = lib().rc_create(ct.byref(d))
+        if not self.h:
+            raise RuntimeError(lib().rc_last_error().decode())
+        self.N = lib().rc_ndets(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().rc_destroy(self.h)
+            self.h = None
+
+    def rows(self):
+        return self.N
+
+    def cols(self):
+        return self.N
+
+    def strings(self, p, s):
+        n = lib().rc_nstrings(self.h, p, s)
+        out = np.zeros(n, dtype=np.uint64)
+        lib().rc_strings(self.h, p, s, _ptr(out))
+        return out
+
+    def unfold_all(self):
+        out = np.zeros((self.N, 4), dtype=np.int32)
+        lib().rc_unfold_all(self.h, _ptr(out))
+        return out
+
+    def address(self, t):
+        t4 = np.zeros(4, dtype=np.int32)
+        t4[: len(t)] = t
+        return lib().rc_address(self.h, _ptr(t4))
+
+    def excitations(self, which, p, s, idx):
+        cap = 1 << 16
+        out = np.zeros(cap, dtype=np.int32)
+        n = lib().rc_excitations(self.h, which, p, s, idx, _ptr(out), cap)
+        return out[:n].copy()
+
+    def diag(self):
+        out = np.zeros(self.N)
+        lib().rc_diag(self.h, _ptr(out))
+        return out
+
+    def __call__(self, i, j):
+        return lib().rc_slater_condon(self.h, int(i), int(j))
+
+    def dense(self):
+        H = np.zeros((self.N, self.N))
+        for i in range(self.N):
+            for j in range(self.N):
+                H[i, j] = self(i, j)
+        return H
+
+    def sigma(self, C):
+        C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)
+        S = np.zeros_like(C, order="F")
+        lib().rc_sigma(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(S), S.shape[0])
+        return S
+
+    def sigma_slow(self, C):
+        C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)
+        S = np.zeros_like(C, order="F")
+        lib().rc_sigma_slow(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(S), S.shape[0])
+        return S
+
+    def davidson(self, nev, nvec_init=5, nvec_max=None, maxit=500, tol=1e-6):
+        """Mirror of epci.cpp:589-608: nvec_max defaults to max(2*nvec_init, 10*nev)."""
+        if nvec_max is None:
+            nvec_max = max(2 * nvec_init, 10 * nev)
+        ev = np.zeros(nev)
+        res = np.zeros(nev)
+        X = np.zeros((self.N, nev), order="F")
+        it = lib().rc_davidson(self.h, nev, nvec_init, nvec_max, maxit, tol, _ptr(ev), _ptr(X), _ptr(res))
+        return ev, X, res, it
+
+
+def ao2mo(ao_full, Ca, Cb=None):
+    """Quarter-transform restatement; ao_full is the (n,n,n,n) AO tensor."""
+    if Cb is None:
+        Cb = Ca
+    n = ao_full.shape[0]
+    nma, nmb = Ca.shape[1], Cb.shape[1]
+    ao = np.ascontiguousarray(ao_full, dtype=np.float64)
+    Ca = np.asfortranarray(Ca, dtype=np.float64)
+    Cb = np.asfortranarray(Cb, dtype=np.float64)
+    out = np.zeros((nma * (nma + 1) // 2, nmb * (nmb + 1) // 2), order="F")
+    lib().rc_ao2mo(n, nma, nmb, _ptr(ao), _ptr(Ca), _ptr(Cb), _ptr(out))
+    return out

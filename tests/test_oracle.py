@@ -1,0 +1,273 @@
+"""Pins the oracle (CPU restatement of the reference CI) before anything is compared to it.
+
+* Known-answer vectors ported from the reference's tests/unit_tests/ci/ci_test.cpp:213-482
+  (holes, parts, phase incl. multi-word and the "Be bug" case, occ/virt) and
+  tests/unit_tests/integral/integral_test.cpp:10-14 (idx2).
+* Determinant-set structure: ci_test.cpp:72-130 (H2O/STO-3G CIS: 7 orbitals, 5a5b,
+  N_dets = 21, HF string 0b0011111) and :555-577 (address order).
+* Slater-Condon rules against brute-force second quantisation (tests/bruteforce.py).
+* Sigma "fast vs slow" (ci_test.cpp:692-853) at the reference's tolerances.
+* Davidson eigenpairs against dense diagonalisation.
+* AO->MO transform against the closed form for Cholesky-built AO integrals.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from oracle import refcpu
+from polyquant_amd import workloads as wl
+from tests import bruteforce
+
+EXTREMELYTIGHT = 1e-14  # src/io/utils.hpp:24
+
+
+def b(s):
+    return int(s, 2)
+
+
+# --- ci_test.cpp:213-257 ---------------------------------------------------------------
+def test_get_holes_known_answers():
+    hf = [b("0011111")]
+    assert refcpu.get_holes(hf, [b("0101111")]) == [4]
+    assert refcpu.get_holes(hf, [b("1101101")]) == [1, 4]
+    big_hf = [b("0001111111"), 0]
+    big_dx = [b("0110111101"), 0]
+    assert refcpu.get_holes(big_hf, big_dx) == [64 + 1, 64 + 6]
+    big_hf[1], big_dx[1] = 1, 32
+    assert refcpu.get_holes(big_hf, big_dx) == [0, 64 + 1, 64 + 6]
+    assert refcpu.get_holes([7, 127], [519, 63]) == [6]
+
+
+# --- ci_test.cpp:259-302 ---------------------------------------------------------------
+def test_get_parts_known_answers():
+    hf = [b("0011111")]
+    assert refcpu.get_parts(hf, [b("0101111")]) == [5]
+    assert refcpu.get_parts(hf, [b("1101101")]) == [5, 6]
+    big_hf = [b("0001111111"), 0]
+    big_dx = [b("0110111101"), 0]
+    assert refcpu.get_parts(big_hf, big_dx) == [64 + 7, 64 + 8]
+    big_hf[1], big_dx[1] = 1, 32
+    assert refcpu.get_parts(big_hf, big_dx) == [5, 64 + 7, 64 + 8]
+    assert refcpu.get_parts([7, 127], [519, 63]) == [73]
+
+
+def _phase(di, dj):
+    return refcpu.get_phase(di, refcpu.get_holes(di, dj), refcpu.get_parts(di, dj))
+
+
+# --- ci_test.cpp:304-370 ---------------------------------------------------------------
+def test_get_phase_known_answers():
+    hf, sx, dx = [b("0011111")], [b("0101111")], [b("1101101")]
+    assert _phase(hf, sx) == 1.0
+    assert _phase(hf, dx) == 1.0
+    assert _phase(dx, sx) == -1.0
+    assert _phase([0, hf[0]], [0, sx[0]]) == 1.0
+    assert _phase([7, 127], [519, 63]) == -1.0
+    # the "Be bug" regression pair
+    assert _phase([0, 1], [0, 16]) == 1.0
+    assert _phase([0, 17179869184], [4, 0]) == 1.0
+
+
+# --- ci_test.cpp:371-482 ---------------------------------------------------------------
+def test_get_occ_virt_known_answers():
+    occ, virt = refcpu.get_occ_virt([b("0011111")], 7)
+    assert occ == list(range(5)) and virt == [5, 6]
+    occ, virt = refcpu.get_occ_virt([b("0001111111"), 0], 64 + 10)
+    assert occ == [64 + i for i in range(7)]
+    assert virt == list(range(64)) + [64 + 7, 64 + 8, 64 + 9]
+    occ, virt = refcpu.get_occ_virt([b("0110111101"), 0], 64 + 10)
+    assert occ == [64, 66, 67, 68, 69, 71, 72]
+    assert virt == list(range(64)) + [65, 70, 73]
+    occ, virt = refcpu.get_occ_virt([7, 127], 100)
+    assert occ == list(range(7)) + [64, 65, 66]
+    assert virt == list(range(7, 64)) + list(range(67, 100))
+    occ, virt = refcpu.get_occ_virt([519, 63], 100)
+    assert occ == list(range(6)) + [64, 65, 66, 73]
+    assert len(virt) == 90 and virt[:58] == list(range(6, 64))
+    assert virt[58:] == [i for i in range(67, 100) if i != 73]
+
+
+def test_idx2_known_answers():
+    # tests/unit_tests/integral/integral_test.cpp:10-14
+    assert wl.idx2(0, 0) == 0
+    assert wl.idx2(1, 0) == 1 and wl.idx2(0, 1) == 1
+    assert wl.idx2(1, 1) == 2
+    assert wl.idx2(3, 2) == wl.idx2(2, 3) == 8
+
+
+def test_phase_matches_permutation_sign_bruteforce():
+    """Single and double excitations on n <= 10: the reference phase equals the sign of
+    a+_p1 a+_p0 a_h0 a_h1 applied in occupation-number form (holes/parts sorted)."""
+    rng = np.random.default_rng(7)
+    for n in (6, 8, 10):
+        for _ in range(60):
+            k = int(rng.integers(2, n - 1))
+            occ = sorted(rng.choice(n, size=k, replace=False).tolist())
+            I = sum(1 << o for o in occ)
+            virt = [v for v in range(n) if v not in occ]
+            for lvl in (1, 2):
+                if lvl > len(virt) or lvl > len(occ):
+                    continue
+                hs = sorted(rng.choice(occ, size=lvl, replace=False).tolist())
+                ps = sorted(rng.choice(virt, size=lvl, replace=False).tolist())
+                J = I
+                for h in hs:
+                    J &= ~(1 << h)
+                for p in ps:
+                    J |= 1 << p
+                ph = _phase([I], [J])
+                # operator form: apply a+_p a_h pairs in order (h0 -> p0) then (h1 -> p1)
+                s, cur = 1.0, I
+                for h, p in zip(hs, ps):
+                    r = bruteforce.apply_E(cur, p, h)
+                    s *= r[0]
+                    cur = r[1]
+                assert cur == J
+                assert ph == s, (n, occ, hs, ps)
+
+
+def _cis_problem():
+    # 7 orbitals, 5 alpha + 5 beta, excitation level [1, 1, 1] (H2O/STO-3G CIS layout)
+    return wl.small_problem(11, 7, 5, 5, ex_level=(1, 1, 1))
+
+
+def test_detset_construction_h2o_cis_layout():
+    oc = refcpu.OracleCI(_cis_problem())
+    assert oc.N == 21  # ci_test.cpp:83-84
+    assert int(oc.strings(0, 0)[0]) == b("0011111")
+    assert int(oc.strings(0, 1)[0]) == b("0011111")
+    assert len(oc.strings(0, 0)) == 11
+    # ci_test.cpp:566-575: first 11 addresses are (0, i), then (i - 10, 0)
+    U = oc.unfold_all()
+    for i in range(oc.N):
+        if i < 11:
+            assert tuple(U[i, :2]) == (0, i)
+        else:
+            assert tuple(U[i, :2]) == (i - 10, 0)
+
+
+def test_string_order_level_major_combinations():
+    """Strings are HF, then level-k excitations with occupied combinations outer and
+    virtual combinations inner, each in itertools.combinations order."""
+    n, a = 7, 5
+    oc = refcpu.OracleCI(wl.small_problem(3, n, a, a, ex_level=(2, 2, 2)))
+    expect = [(1 << a) - 1]
+    occ, virt = list(range(a)), list(range(a, n))
+    for k in (1, 2):
+        for oc_ in itertools.combinations(occ, k):
+            for vc in itertools.combinations(virt, k):
+                s = (1 << a) - 1
+                for o in oc_:
+                    s &= ~(1 << o)
+                for v in vc:
+                    s |= 1 << v
+                expect.append(s)
+    got = [int(x) for x in oc.strings(0, 0)]
+    assert got == expect
+    # ci_test.cpp:528-536 uses beta string 12 as a double excitation of the HF string
+    assert refcpu.num_excitation([got[0]], [got[12]]) == 2
+    assert oc.N == 141  # H2O/STO-3G CISD determinant count (cisd_ham_elements.txt = 141^2 rows)
+
+
+def test_excitation_maps_are_ascending_exact_distance():
+    oc = refcpu.OracleCI(wl.small_problem(5, 8, 3, 3))
+    strs = [int(x) for x in oc.strings(0, 0)]
+    for i in range(len(strs)):
+        s1 = oc.excitations(1, 0, 0, i)
+        s2 = oc.excitations(2, 0, 0, i)
+        exp1 = [j for j in range(len(strs)) if bin(strs[i] ^ strs[j]).count("1") == 2]
+        exp2 = [j for j in range(len(strs)) if bin(strs[i] ^ strs[j]).count("1") == 4]
+        assert list(s1) == exp1
+        assert list(s2) == exp2
+
+
+@pytest.mark.parametrize(
+    "kind",
+    ["restricted", "unrestricted", "truncated", "positron", "positron_two_spin", "open_shell"],
+)
+def test_slater_condon_matches_bruteforce(kind):
+    if kind == "restricted":
+        prob = wl.small_problem(21, 5, 2, 2)
+    elif kind == "unrestricted":
+        prob = wl.small_problem(22, 5, 2, 2, nspin=2)
+    elif kind == "truncated":
+        prob = wl.small_problem(23, 6, 3, 3, ex_level=(2, 2, 2))
+    elif kind == "open_shell":
+        prob = wl.small_problem(26, 5, 3, 1)
+    elif kind == "positron":
+        prob = wl.small_problem(24, 4, 1, 1, positron=(4, 1, 0))
+    else:
+        prob = wl.small_problem(25, 4, 2, 1, positron=(3, 2, 1))
+    oc = refcpu.OracleCI(prob)
+    dets = bruteforce.oracle_dets(oc)
+    Hbf = bruteforce.hamiltonian(prob, dets)
+    Hor = oc.dense()
+    assert np.max(np.abs(Hbf - Hor)) < 1e-12
+    assert np.allclose(Hor, Hor.T, atol=1e-13)
+
+
+@pytest.mark.parametrize("two_species", [False, True])
+def test_sigma_fast_vs_slow(two_species):
+    """ci_test.cpp:692-853: unit vectors e_0, e_5 and the uniform vector."""
+    if two_species:
+        prob = wl.small_problem(31, 5, 2, 1, positron=(4, 1, 0))
+        tol = 10 * EXTREMELYTIGHT
+    else:
+        prob = wl.small_problem(32, 7, 5, 5, ex_level=(2, 2, 2))
+        tol = EXTREMELYTIGHT
+    oc = refcpu.OracleCI(prob)
+    N = oc.N
+    vecs = []
+    for k in (0, 5):
+        c = np.zeros(N)
+        c[k] = 1.0
+        vecs.append(c)
+    vecs.append(np.full(N, 1.0 / N))
+    for c in vecs:
+        fast = oc.sigma(c)[:, 0]
+        slow = oc.sigma_slow(c)[:, 0]
+        assert np.max(np.abs(fast - slow)) < tol
+    # block of random vectors at once
+    C = np.random.default_rng(0).standard_normal((N, 3))
+    assert np.allclose(oc.sigma(C), oc.dense() @ C, atol=1e-12)
+
+
+@pytest.mark.parametrize("two_species", [False, True])
+def test_davidson_matches_dense_eigh(two_species):
+    if two_species:
+        prob = wl.small_problem(41, 6, 2, 2, positron=(5, 1, 0))
+    else:
+        prob = wl.small_problem(42, 8, 3, 3)
+    oc = refcpu.OracleCI(prob)
+    ev, X, res, it = oc.davidson(3, nvec_init=5, tol=1e-8)
+    assert it > 0
+    w = np.linalg.eigvalsh(oc.dense())
+    assert np.allclose(ev, w[:3], atol=1e-10)
+    assert np.all(res < 1e-8)
+
+
+def test_ao2mo_closed_form_cholesky():
+    """For (pq|rs) = sum_k L_k,pq L_k,rs the MO integrals are sum_k (C^T L_k C)_pq (C^T L_k C)_rs."""
+    n, K = 6, 10
+    rng = np.random.default_rng(3)
+    L = rng.standard_normal((K, n, n))
+    L = L + L.transpose(0, 2, 1)
+    ao = np.einsum("kpq,krs->pqrs", L, L)
+    C = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    out = refcpu.ao2mo(ao, C)
+    M = np.einsum("pi,kpq,qj->kij", C, L, C)
+    P = wl.pair_index_matrix(n)
+    iu = np.triu_indices(n)
+    ref = np.einsum("kij,kab->ijab", M, M)
+    exp = np.zeros((wl.npair(n), wl.npair(n)))
+    for (i, j) in zip(*iu):
+        for (a, b_) in zip(*iu):
+            exp[P[i, j], P[a, b_]] = ref[i, j, a, b_]
+    assert np.max(np.abs(out - exp)) < 1e-11
+    # identity coefficients reproduce the AO tensor
+    out_id = refcpu.ao2mo(ao, np.eye(n))
+    for (i, j) in zip(*iu):
+        for (a, b_) in zip(*iu):
+            assert abs(out_id[P[i, j], P[a, b_]] - ao[i, j, a, b_]) < 1e-12
