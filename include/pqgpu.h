@@ -1,0 +1,162 @@
+/* pqgpu.h — C ABI of the MI355X configuration-interaction hot path.
+ *
+ * Drop-in boundary for Polyquant's CI solve (shivupa/polyquant v0.7.0).  Every entry point
+ * takes plain pointers and sizes; no C++ or torch types cross it.  Matrices are column-major
+ * like Eigen's defaults, determinant addresses and string orders are the reference's.
+ *
+ * Errors are status codes (never exit()); pqg_last_error() gives the message of the last
+ * failing call on the calling thread.  A pqg_ci handle is used from one host thread at a
+ * time; a pqg_ctx may be shared.
+ */
+#ifndef PQGPU_H
+#define PQGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int pqg_status;
+#define PQG_OK 0
+#define PQG_ERR_ARG 1          /* invalid argument / unsupported shape               */
+#define PQG_ERR_HIP 2          /* HIP runtime error                                  */
+#define PQG_ERR_OOM 3          /* request does not fit the context's HBM budget      */
+#define PQG_ERR_NOCONV 4       /* Davidson did not converge (epci.cpp:620 aborts)    */
+#define PQG_ERR_UNSUPPORTED 5
+
+#define PQG_FP64 0             /* all arithmetic in FP64                              */
+#define PQG_MIXED 1            /* FP32 sigma bulk, FP64 correction / subspace / checks */
+
+#define PQG_AO2MO_FP64 0       /* FP64 MFMA                                           */
+#define PQG_AO2MO_BF16X3 1     /* BF16x3 split products on MFMA, FP32 accumulation    */
+
+typedef struct pqg_ctx pqg_ctx;
+typedef struct pqg_ci pqg_ci;
+
+/* CI problem description.  Mirrors the data POLYQUANT_DETSET reads
+ * (src/ci/determinant_set.hpp:84-119) and POLYQUANT_INTEGRAL's MO members
+ * (src/integral/integral.hpp:189-205):
+ *   h[p][s]          n x n one-body MO integrals (mo_one_body_ints[p][s]), column-major;
+ *   eri[a][sa][b][sb] n_pair_a x n_pair_b two-body block (mo_two_body_ints[a][sa][b][sb]),
+ *                    column-major, element (idx2(i,j), idx2(k,l)); required for b >= a.
+ * Species index order is the reference's (electron = 0, positron = 1); spin block
+ * index for beta is (1 % nspin_blocks) as in slater_condon_singleparticle.cpp:13.
+ * ex_level is excitation_level (epci.hpp:107) = {alpha, beta, total}; a full CI passes
+ * the largest levels.  Strings are one 64-bit word: n_orb <= 63.                       */
+typedef struct {
+  int nspecies;
+  int n_orb[2];
+  int n_alpha[2];
+  int n_beta[2];
+  double charge[2];
+  int nspin_blocks[2];
+  int ex_level[2][3];
+  int max_collective;              /* max_collective_excitation_level (epci.hpp:108)    */
+  const double* h[2][2];
+  const double* eri[2][2][2][2];
+} pqg_ci_desc;
+
+/* Davidson parameters: the knobs POLYQUANT_EPCI::run passes to the eigensolver
+ * (epci.cpp:589-608): num_states, num_subspace_vec, max(2*nsv, 10*nstates), iteration_max,
+ * convergence_E.  precision selects PQG_FP64 or PQG_MIXED.                              */
+typedef struct {
+  int nev;
+  int nvec_init;
+  int nvec_max;                    /* <= 0: max(2*nvec_init, 10*nev)                     */
+  int maxit;
+  double tol;                      /* residual-norm tolerance                            */
+  int precision;
+} pqg_dav_params;
+
+typedef struct {
+  int iterations;
+  int converged;
+  int sigma_calls;                 /* sigma launches (vectors), all precisions           */
+  int sigma_calls_fp64;
+  int restarts;
+  double residual[64];             /* final FP64 residual norms per root                 */
+  double seconds;                  /* wall time of the solve                             */
+} pqg_dav_report;
+
+/* --- context / memory manager --------------------------------------------------------- */
+/* One HIP stream and a device-memory arena limited to hbm_budget bytes.                  */
+pqg_status pqg_ctx_create(int device, uint64_t hbm_budget, pqg_ctx** out);
+pqg_status pqg_ctx_destroy(pqg_ctx* ctx);
+pqg_status pqg_ctx_memory(pqg_ctx* ctx, uint64_t* used, uint64_t* budget, uint64_t* peak);
+pqg_status pqg_ctx_sync(pqg_ctx* ctx);
+const char* pqg_last_error(void);
+
+/* --- determinant set (replaces POLYQUANT_DETSET<uint64_t>) ----------------------------- */
+/* Builds strings in the reference order (determinant_creation.cpp:6-207, epci.cpp:334-388),
+ * uploads integrals, and precomputes the diagonal (slater_condon.cpp:102-153) on the GPU. */
+pqg_status pqg_ci_create(pqg_ctx* ctx, const pqg_ci_desc* desc, pqg_ci** out);
+pqg_status pqg_ci_destroy(pqg_ci* ci);
+/* rows()/cols() (determinant_set.hpp:127-134) */
+int64_t pqg_ci_ndets(const pqg_ci* ci);
+int64_t pqg_ci_nstrings(const pqg_ci* ci, int part, int spin);
+/* diagonal_Hii (determinant_set.hpp:259), reference address order */
+pqg_status pqg_ci_diag(pqg_ci* ci, double* hdiag);
+/* operator()(i, j) -> Slater_Condon (determinant_set.hpp:139, slater_condon.cpp:4-100) for
+ * npairs (i, j) address pairs at once.                                                    */
+pqg_status pqg_ci_elements(pqg_ci* ci, const int64_t* i, const int64_t* j, int64_t npairs, double* out);
+/* operator*(C) (direct.cpp:5-19): S = H C for k vectors, N x k column-major.
+ * on_device != 0: C and S are device pointers.  precision: PQG_FP64 or PQG_MIXED (FP32).   */
+pqg_status pqg_ci_sigma(pqg_ci* ci, const double* C, int64_t ldc, int k, double* S, int64_t lds,
+                        int on_device, int precision);
+/* Full Davidson solve on the GPU (Spectra DavidsonSymEigsSolver, SmallestAlge,
+ * epci.cpp:589-608).  E: nev eigenvalues (without constant shifts); X: N x nev Ritz vectors
+ * (host, reference order, column-major, may be NULL).                                      */
+pqg_status pqg_ci_davidson(pqg_ci* ci, const pqg_dav_params* params, double* E, double* X, int64_t ldx,
+                           pqg_dav_report* report);
+
+/* --- parity dumps ------------------------------------------------------------------------ */
+/* unique_dets[part][spin] strings in the reference order (determinant_creation.cpp:139-146) */
+pqg_status pqg_ci_dump_strings(pqg_ci* ci, int part, int spin, uint64_t* out);
+/* reference string indices of each determinant address (det_idx_unfold, :378-388), N x 4 */
+pqg_status pqg_ci_dump_unfold(pqg_ci* ci, int32_t* out);
+/* Excitations of every string of (part, spin), enumerated on the GPU.  which = 1 (singles)
+ * or 2 (doubles).  For string r (reference order) the entries [off[r], off[r+1]) hold the
+ * excited string's reference index J, ascending (the order of unique_singles/doubles,
+ * determinant_creation.cpp:209-351), and its phase (get_phase, determinant_operations.cpp:47-116)
+ * as +1/-1.  Call with J == NULL to get off[] only (size nstrings+1).                      */
+pqg_status pqg_ci_dump_excitations(pqg_ci* ci, int which, int part, int spin, int64_t* off, int32_t* J,
+                                   int8_t* phase);
+
+/* --- AO -> MO transform (replaces transform_mo_2_body_integrals, integral.cpp:207-477) -- */
+/* eri8: AO integrals with 8-fold symmetry, packed as idx2(idx2(p,q), idx2(r,s)) (idx8,
+ * integral.hpp:76-88), length npair*(npair+1)/2 with npair = nao*(nao+1)/2.
+ * C: nao x nmo column-major.  out4: npair_mo x npair_mo column-major, element
+ * (idx2(i,j), idx2(k,l)).  on_device != 0: eri8, C and out4 are device pointers.
+ * mode: PQG_AO2MO_FP64 or PQG_AO2MO_BF16X3.  seconds (may be NULL): GPU time.              */
+pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* C, double* out4,
+                     int on_device, int mode, double* seconds);
+
+/* --- serving (continuous batching of CI solves) ------------------------------------------ */
+typedef struct pqg_server pqg_server;
+typedef struct {
+  int64_t request_id;
+  int status;                      /* PQG_OK, PQG_ERR_OOM, PQG_ERR_NOCONV, ...           */
+  int nev;
+  double energies[64];
+  double t_submit, t_admit, t_done; /* seconds on the server clock                       */
+  int iterations;
+} pqg_result;
+
+/* max_slots: solves resident on the device at once; budget from the context.            */
+pqg_status pqg_serve_create(pqg_ctx* ctx, int max_slots, pqg_server** out);
+pqg_status pqg_serve_destroy(pqg_server* srv);
+/* Queue one request (copies desc and params).  Returns its id.                           */
+pqg_status pqg_serve_submit(pqg_server* srv, const pqg_ci_desc* desc, const pqg_dav_params* params,
+                            int64_t* request_id);
+/* Advance the loop by one scheduling step: admit queued requests whose memory fits, run one
+ * Davidson iteration of every resident solve (their sigma builds fused into one launch),
+ * retire finished ones.  Returns the number of results written (<= cap).                 */
+pqg_status pqg_serve_step(pqg_server* srv, pqg_result* results, int cap, int* nresults);
+pqg_status pqg_serve_stats(pqg_server* srv, int* queued, int* resident, uint64_t* bytes_resident);
+double pqg_serve_clock(pqg_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PQGPU_H */

@@ -1,0 +1,234 @@
+"""Host-side mirror of Polyquant's CI interfaces over the HIP library.
+
+``DetSet`` stands in for ``POLYQUANT_DETSET<uint64_t>`` (src/ci/determinant_set.hpp:27-201):
+``rows()``, ``cols()``, ``__call__(i, j)`` (Slater-Condon element), ``@``/``__mul__``
+(``operator*``, the sigma build, src/ci/detset/direct.cpp:5-19) and ``diagonal_Hii``.
+``EPCI`` restates the solve flow of ``POLYQUANT_EPCI::run`` (src/ci/epci.cpp:575-621):
+Davidson with ``num_states``, ``num_subspace_vec``, ``max(2*nsv, 10*nstates)``,
+``iteration_max``, ``convergence_E`` and the constant shift added to the energies; a failed
+solve raises like the reference's ``APP_ABORT("CI Calculation did not converge!")``.
+
+Addresses, string orders and matrices are the reference's (column-major N x k).
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes as ct
+import weakref
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from ._lib import CiDesc, DavParams, DavReport, check, load
+
+_dp = ct.POINTER(ct.c_double)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ct.c_void_p)
+
+
+# Live handles, closed (CI handles first) before the interpreter tears down the HIP runtime.
+_live_ci: "weakref.WeakSet" = weakref.WeakSet()
+_live_ctx: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for obj in list(_live_ci):
+        obj.close()
+    for obj in list(_live_ctx):
+        obj.close()
+
+
+class Context:
+    """One GPU, one HIP stream and a device-memory arena bounded by ``hbm_budget`` bytes."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.h = ct.c_void_p()
+        check(load().pqg_ctx_create(device, hbm_budget, ct.byref(self.h)))
+        _live_ctx.add(self)
+
+    def memory(self):
+        used, budget, peak = ct.c_uint64(), ct.c_uint64(), ct.c_uint64()
+        check(load().pqg_ctx_memory(self.h, ct.byref(used), ct.byref(budget), ct.byref(peak)))
+        return used.value, budget.value, peak.value
+
+    def close(self):
+        if self.h:
+            load().pqg_ctx_destroy(self.h)
+            self.h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context()
+    return _default_ctx
+
+
+def make_desc(prob, keep: list) -> CiDesc:
+    """pqg_ci_desc from a workloads.CIProblem (arrays kept alive in ``keep``)."""
+    d = CiDesc()
+    d.nspecies = prob.nspecies
+    d.max_collective = int(min(prob.max_collective, 2**31 - 1))
+    for p, sp in enumerate(prob.species):
+        d.n_orb[p] = sp.n_orb
+        d.n_alpha[p] = sp.n_alpha
+        d.n_beta[p] = sp.n_beta
+        d.charge[p] = sp.charge
+        d.nspin_blocks[p] = sp.nspin_blocks
+        lv = sp.levels()
+        for q in range(3):
+            d.ex_level[p][q] = int(lv[q])
+        for s in range(sp.nspin_blocks):
+            h = np.asfortranarray(prob.h[p][s], dtype=np.float64)
+            keep.append(h)
+            d.h[p][s] = h.ctypes.data_as(_dp)
+    for (a, sa, b, sb), v in prob.eri.items():
+        arr = np.asfortranarray(v, dtype=np.float64)
+        keep.append(arr)
+        d.eri[a][sa][b][sb] = arr.ctypes.data_as(_dp)
+    return d
+
+
+class DetSet:
+    """GPU determinant set: the drop-in for POLYQUANT_DETSET<uint64_t>."""
+
+    def __init__(self, prob, ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.prob = prob
+        self._keep = []
+        self.desc = make_desc(prob, self._keep)
+        self.h = ct.c_void_p()
+        check(load().pqg_ci_create(self.ctx.h, ct.byref(self.desc), ct.byref(self.h)))
+        _live_ci.add(self)
+        self._keep = []  # the library copied everything it needs
+        self.N_dets = load().pqg_ci_ndets(self.h)
+        self._diag = None
+
+    def close(self):
+        if self.h:
+            load().pqg_ci_destroy(self.h)
+            self.h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- POLYQUANT_DETSET interface ---------------------------------------------------
+    def rows(self) -> int:
+        return self.N_dets
+
+    def cols(self) -> int:
+        return self.N_dets
+
+    @property
+    def diagonal_Hii(self) -> np.ndarray:
+        if self._diag is None:
+            out = np.zeros(self.N_dets)
+            check(load().pqg_ci_diag(self.h, _ptr(out)))
+            self._diag = out
+        return self._diag
+
+    def elements(self, i, j) -> np.ndarray:
+        i = np.ascontiguousarray(np.atleast_1d(i), dtype=np.int64)
+        j = np.ascontiguousarray(np.atleast_1d(j), dtype=np.int64)
+        out = np.zeros(len(i))
+        check(load().pqg_ci_elements(self.h, _ptr(i), _ptr(j), len(i), _ptr(out)))
+        return out
+
+    def __call__(self, i: int, j: int) -> float:
+        """operator()(i, j) -> Slater_Condon(i, j)."""
+        return float(self.elements([i], [j])[0])
+
+    def sigma(self, C: np.ndarray, precision: int = _lib.PQG_FP64) -> np.ndarray:
+        """operator*(C): sigma = H C for the columns of C (host arrays)."""
+        C = np.asfortranarray(np.atleast_2d(np.asarray(C, dtype=np.float64).T).T)
+        S = np.zeros_like(C, order="F")
+        check(load().pqg_ci_sigma(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(S), S.shape[0], 0, precision))
+        return S
+
+    def sigma_device(self, C_ptr: int, ldc: int, k: int, S_ptr: int, lds: int, precision: int = _lib.PQG_FP64):
+        """sigma on device buffers (e.g. torch tensors' data_ptr())."""
+        check(load().pqg_ci_sigma(self.h, ct.c_void_p(C_ptr), ldc, k, ct.c_void_p(S_ptr), lds, 1, precision))
+
+    def __matmul__(self, C):
+        return self.sigma(C)
+
+    __mul__ = __matmul__
+
+    # --- parity dumps -------------------------------------------------------------------
+    def nstrings(self, part: int, spin: int) -> int:
+        return load().pqg_ci_nstrings(self.h, part, spin)
+
+    def strings(self, part: int, spin: int) -> np.ndarray:
+        out = np.zeros(self.nstrings(part, spin), dtype=np.uint64)
+        check(load().pqg_ci_dump_strings(self.h, part, spin, _ptr(out)))
+        return out
+
+    def unfold_all(self) -> np.ndarray:
+        out = np.zeros((self.N_dets, 4), dtype=np.int32)
+        check(load().pqg_ci_dump_unfold(self.h, _ptr(out)))
+        return out
+
+    def excitations(self, which: int, part: int, spin: int):
+        """(offsets, J, phase) of every string's singles (1) or doubles (2), reference order."""
+        n = self.nstrings(part, spin)
+        off = np.zeros(n + 1, dtype=np.int64)
+        check(load().pqg_ci_dump_excitations(self.h, which, part, spin, _ptr(off), None, None))
+        J = np.zeros(max(int(off[-1]), 1), dtype=np.int32)
+        ph = np.zeros(max(int(off[-1]), 1), dtype=np.int8)
+        check(load().pqg_ci_dump_excitations(self.h, which, part, spin, _ptr(off), _ptr(J), _ptr(ph)))
+        return off, J[: off[-1]], ph[: off[-1]]
+
+    # --- solver -------------------------------------------------------------------------
+    def davidson(self, nev: int, nvec_init: int = 5, nvec_max: int = 0, maxit: int = 500, tol: float = 1e-6,
+                 precision: int = _lib.PQG_FP64, want_vectors: bool = True):
+        p = DavParams(nev, nvec_init, nvec_max, maxit, tol, precision)
+        E = np.zeros(nev)
+        X = np.zeros((self.N_dets, nev), order="F") if want_vectors else None
+        rep = DavReport()
+        st = load().pqg_ci_davidson(self.h, ct.byref(p), _ptr(E), _ptr(X) if X is not None else None,
+                                    self.N_dets, ct.byref(rep))
+        check(st)
+        return E, X, rep
+
+
+class EPCI:
+    """Solve flow of POLYQUANT_EPCI::run (src/ci/epci.cpp:575-621) with the matrix-free path."""
+
+    def __init__(self, prob, ctx: Optional[Context] = None, num_states: int = 1, num_subspace_vec: int = 5,
+                 iteration_max: int = 500, convergence_E: float = 1e-6, precision: int = _lib.PQG_FP64):
+        self.prob = prob
+        self.detset = DetSet(prob, ctx)
+        self.num_states = num_states
+        self.num_subspace_vec = num_subspace_vec
+        self.iteration_max = iteration_max
+        self.convergence_E = convergence_E
+        self.precision = precision
+        self.constant_shift = prob.e_shift  # E_nuc + frozen-core energy (epci.cpp:584-594)
+        self.energies = None
+        self.C_ci = None
+        self.report = None
+
+    def run(self):
+        nvec_max = max(2 * self.num_subspace_vec, 10 * self.num_states)
+        E, X, rep = self.detset.davidson(self.num_states, self.num_subspace_vec, nvec_max, self.iteration_max,
+                                         self.convergence_E, self.precision)
+        self.energies = E + self.constant_shift
+        self.C_ci = X
+        self.report = rep
+        return self.energies

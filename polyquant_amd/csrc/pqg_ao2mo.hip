@@ -1,0 +1,323 @@
+// AO -> MO four-index transform on MFMA (gfx950).
+//
+// (ij|kl) = sum_{mu nu la si} C[mu,i] C[nu,j] C[la,k] C[si,l] (mu nu|la si), the contraction
+// of transform_mo_2_body_integrals (src/integral/integral.cpp:207-477), organised as two
+// half-transforms over pair indices:
+//   half 1, for every AO pair Q = (la si):  M_Q = unpack((.. | Q)),  T_Q = C^T M_Q C,
+//            B[Q][(ij)] = T_Q[i][j] (i <= j);
+//   half 2, for every MO pair (ij):         M = unpack(B[.][(ij)]),  T = C^T M C,
+//            out[(kl) + (ij) * npair] = T[k][l].
+// Each C^T M C is two dense n x n x n products on the matrix cores:
+//   U = M C (M symmetric), then T = U^T C.
+// Batches of pairs run as one launch (batched GEMM, 64 x 64 output tile per workgroup, K
+// staged through LDS).
+//   PQG_AO2MO_FP64:   v_mfma_f64_16x16x4f64, FP64 accumulation.
+//   PQG_AO2MO_BF16X3: each FP32 operand split into bf16 hi + lo, three products
+//                     hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_bf16, FP32 accumulation.
+// The final (ij|kl) block is symmetric for a single coefficient matrix, so the output row
+// (ij) is written as column (ij) of the column-major result (coalesced).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "pqg_internal.hpp"
+
+namespace pqg {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ int64_t pidx(int64_t i, int64_t j) { return i > j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+// ---------------------------------------------------------------------------------------
+// unpack / pack
+// ---------------------------------------------------------------------------------------
+// X[b][nu][mu] = eri8[idx2(idx2(mu, nu), q0 + b)] for b < nb (symmetric n x n per batch)
+template <typename T>
+__global__ void k_unpack8(const double* __restrict__ eri8, int n, int64_t q0, int nb, T* __restrict__ X) {
+  const int64_t n2 = (int64_t)n * n;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n2 * nb) return;
+  const int b = (int)(idx / n2);
+  const int64_t r = idx % n2;
+  const int nu = (int)(r / n), mu = (int)(r % n);
+  const int64_t P = pidx(mu, nu), Q = q0 + b;
+  X[idx] = (T)eri8[pidx(P, Q)];
+}
+
+// X[b][la][si] = B[idx2(la, si)][c0 + b]   (B row-major npair x ldb)
+template <typename T>
+__global__ void k_unpack_half(const T* __restrict__ B, int64_t ldb, int n, int64_t c0, int nb, T* __restrict__ X) {
+  const int64_t n2 = (int64_t)n * n;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n2 * nb) return;
+  // consecutive threads take consecutive batch entries for one (la, si): coalesced reads
+  const int b = (int)(idx % nb);
+  const int64_t r = idx / nb;
+  const int la = (int)(r / n), si = (int)(r % n);
+  X[b * n2 + r] = B[pidx(la, si) * ldb + c0 + b];
+}
+
+// out[b][pidx(i, j)] = T[b][i][j] for i <= j, with row stride ldo (batch b -> row b0 + b)
+template <typename T, typename O>
+__global__ void k_pack(const T* __restrict__ Tm, int n, int nb, O* __restrict__ out, int64_t ldo) {
+  const int64_t np = (int64_t)n * (n + 1) / 2;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= np * nb) return;
+  const int b = (int)(idx / np);
+  const int64_t p = idx % np;
+  // invert p -> (i, j) with i >= j : i = floor((sqrt(8p+1)-1)/2)
+  int64_t i = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while (i * (i + 1) / 2 > p) i--;
+  while ((i + 1) * (i + 2) / 2 <= p) i++;
+  const int64_t j = p - i * (i + 1) / 2;
+  out[(int64_t)b * ldo + p] = (O)Tm[(int64_t)b * n * n + j * n + i];
+}
+
+// ---------------------------------------------------------------------------------------
+// batched GEMM: out_b (n x n) = op(A_b) * Cm,  op(A) = A (TA = 0) or A^T (TA = 1),
+// A_b row-major n x n, Cm row-major n x n (shared), out row-major.
+// ---------------------------------------------------------------------------------------
+constexpr int BM = 64, BN = 64, BK = 16;
+
+// FP64: 4 waves, each a 32 x 32 quadrant = 2 x 2 tiles of 16 x 16 (v_mfma_f64_16x16x4f64).
+// Operand layouts: A 16x4 -> lane l holds A[l%16][l/16]; B 4x16 -> lane l holds B[l/16][l%16];
+// D 16x16 -> lane l, register r holds D[4*r + l/16][l%16], r = 0..3 (rows interleaved).
+template <int TA>
+__global__ __launch_bounds__(256) void k_bgemm_f64(const double* __restrict__ A, const double* __restrict__ Cm,
+                                                   double* __restrict__ out, int n) {
+  __shared__ double As[BK][BM + 2];  // As[k][m]
+  __shared__ double Bs[BK][BN + 2];  // Bs[k][nn]
+  const int64_t n2 = (int64_t)n * n;
+  const double* Ab = A + (int64_t)blockIdx.z * n2;
+  double* Ob = out + (int64_t)blockIdx.z * n2;
+  const int m0 = blockIdx.y * BM, c0 = blockIdx.x * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = d4{0, 0, 0, 0};
+  for (int k0 = 0; k0 < n; k0 += BK) {
+    for (int i = tid; i < BK * BM; i += 256) {
+      const int kk = i / BM, mm = i % BM;
+      const int gm = m0 + mm, gk = k0 + kk;
+      double v = 0.0;
+      if (gm < n && gk < n) v = TA ? Ab[(int64_t)gk * n + gm] : Ab[(int64_t)gm * n + gk];
+      As[kk][mm] = v;
+    }
+    for (int i = tid; i < BK * BN; i += 256) {
+      const int kk = i / BN, nn = i % BN;
+      const int gn = c0 + nn, gk = k0 + kk;
+      Bs[kk][nn] = (gn < n && gk < n) ? Cm[(int64_t)gk * n + gn] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      const int kk = ks + (lane >> 4);
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const double av = As[kk][wm + a * 16 + (lane & 15)];
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+          const double bv = Bs[kk][wn + b * 16 + (lane & 15)];
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[a][b], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int gm = m0 + wm + a * 16 + 4 * r + (lane >> 4);
+        const int gn = c0 + wn + b * 16 + (lane & 15);
+        if (gm < n && gn < n) Ob[(int64_t)gm * n + gn] = acc[a][b][r];
+      }
+}
+
+// BF16x3: 4 waves, each a 32 x 32 quadrant = one v_mfma_f32_32x32x16_bf16 tile, K step 16.
+// Operand layouts: A 32x16 -> lane l holds A[l%32][8*(l/32) + e], e = 0..7;
+// B 16x32 -> lane l holds B[8*(l/32) + e][l%32]; D 32x32 -> lane l, register r holds
+// D[8*(r/4) + 4*(l/32) + r%4][l%32].
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+template <int TA>
+__global__ __launch_bounds__(256) void k_bgemm_bf16x3(const float* __restrict__ A, const float* __restrict__ Cm,
+                                                      float* __restrict__ out, int n) {
+  __shared__ __bf16 Ah[BM][BK + 8], Al[BM][BK + 8];  // [m][k]
+  __shared__ __bf16 Bh[BN][BK + 8], Bl[BN][BK + 8];  // [n][k]
+  const int64_t n2 = (int64_t)n * n;
+  const float* Ab = A + (int64_t)blockIdx.z * n2;
+  float* Ob = out + (int64_t)blockIdx.z * n2;
+  const int m0 = blockIdx.y * BM, c0 = blockIdx.x * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  f16v acc = {};
+  for (int k0 = 0; k0 < n; k0 += BK) {
+    for (int i = tid; i < BK * BM; i += 256) {
+      const int mm = i / BK, kk = i % BK;
+      const int gm = m0 + mm, gk = k0 + kk;
+      float v = 0.f;
+      if (gm < n && gk < n) v = TA ? Ab[(int64_t)gk * n + gm] : Ab[(int64_t)gm * n + gk];
+      split_bf16(v, Ah[mm][kk], Al[mm][kk]);
+    }
+    for (int i = tid; i < BK * BN; i += 256) {
+      const int kk = i / BN, nn = i % BN;
+      const int gn = c0 + nn, gk = k0 + kk;
+      const float v = (gn < n && gk < n) ? Cm[(int64_t)gk * n + gn] : 0.f;
+      split_bf16(v, Bh[nn][kk], Bl[nn][kk]);
+    }
+    __syncthreads();
+    bf8 ah, al, bh, bl;
+    const int ar = wm + (lane & 31), bc = wn + (lane & 31), kb = 8 * (lane >> 5);
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      ah[e] = Ah[ar][kb + e];
+      al[e] = Al[ar][kb + e];
+      bh[e] = Bh[bc][kb + e];
+      bl[e] = Bl[bc][kb + e];
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int gm = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int gn = c0 + wn + (lane & 31);
+    if (gm < n && gn < n) Ob[(int64_t)gm * n + gn] = acc[r];
+  }
+}
+
+__global__ void k_cvt_d2f(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
+static int gr(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+template <typename T>
+static void bgemm(const T* A, const T* Cm, T* out, int n, int nb, bool transA, hipStream_t st) {
+  dim3 g(gr(n, BN), gr(n, BM), nb);
+  if constexpr (std::is_same<T, double>::value) {
+    if (transA)
+      k_bgemm_f64<1><<<g, 256, 0, st>>>(A, Cm, out, n);
+    else
+      k_bgemm_f64<0><<<g, 256, 0, st>>>(A, Cm, out, n);
+  } else {
+    if (transA)
+      k_bgemm_bf16x3<1><<<g, 256, 0, st>>>(A, Cm, out, n);
+    else
+      k_bgemm_bf16x3<0><<<g, 256, 0, st>>>(A, Cm, out, n);
+  }
+  PQG_HIP(hipGetLastError());
+}
+
+// C^T M_b C for nb symmetric matrices M_b (row-major), through U = M C, T = U^T C
+template <typename T>
+static void ctmc(const T* M, const T* Cm, T* U, T* Tout, int n, int nb, hipStream_t st) {
+  bgemm<T>(M, Cm, U, n, nb, false, st);
+  bgemm<T>(U, Cm, Tout, n, nb, true, st);
+}
+
+template <typename T>
+static void ao2mo_impl(pqg_ctx* ctx, int n, const double* eri8, const double* Cd, double* out4, hipStream_t st) {
+  const int64_t np = (int64_t)n * (n + 1) / 2;
+  const int64_t n2 = (int64_t)n * n;
+  // batch size: keep the three n x n x nb work arrays within ~1/8 of the budget
+  const uint64_t avail = ctx->arena->budget() > ctx->arena->used() ? ctx->arena->budget() - ctx->arena->used() : 0;
+  const uint64_t interm = (uint64_t)np * np * sizeof(T);  // half-transformed B
+  PQG_REQUIRE(avail > interm + (64ull << 20), PQG_ERR_OOM, "AO->MO intermediate does not fit the HBM budget");
+  int nb = (int)std::min<int64_t>(np, std::max<int64_t>(1, (int64_t)((avail - interm) / 4 / (3 * n2 * sizeof(T)))));
+  nb = std::min(nb, 4096);
+  DevBuf<T> Cm(ctx, n2), X(ctx, n2 * nb), U(ctx, n2 * nb), Tm(ctx, n2 * nb), B(ctx, np * np);
+  // C in row-major [mu][i] (input is column-major nao x nmo)
+  {
+    std::vector<double> h(n2);
+    PQG_HIP(hipMemcpyAsync(h.data(), Cd, n2 * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+    std::vector<T> r(n2);
+    for (int mu = 0; mu < n; mu++)
+      for (int i = 0; i < n; i++) r[(size_t)mu * n + i] = (T)h[mu + (size_t)i * n];
+    PQG_HIP(hipMemcpyAsync(Cm.get(), r.data(), n2 * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  // half 1: rows Q of B
+  for (int64_t q0 = 0; q0 < np; q0 += nb) {
+    const int b = (int)std::min<int64_t>(nb, np - q0);
+    k_unpack8<T><<<gr(n2 * b, 256), 256, 0, st>>>(eri8, n, q0, b, X.get());
+    PQG_HIP(hipGetLastError());
+    ctmc<T>(X.get(), Cm.get(), U.get(), Tm.get(), n, b, st);
+    k_pack<T, T><<<gr(np * b, 256), 256, 0, st>>>(Tm.get(), n, b, B.get() + q0 * np, np);
+    PQG_HIP(hipGetLastError());
+  }
+  // half 2: columns (ij) of B; output column (ij) = row (ij) by symmetry
+  for (int64_t c0 = 0; c0 < np; c0 += nb) {
+    const int b = (int)std::min<int64_t>(nb, np - c0);
+    k_unpack_half<T><<<gr(n2 * b, 256), 256, 0, st>>>(B.get(), np, n, c0, b, X.get());
+    PQG_HIP(hipGetLastError());
+    ctmc<T>(X.get(), Cm.get(), U.get(), Tm.get(), n, b, st);
+    k_pack<T, double><<<gr(np * b, 256), 256, 0, st>>>(Tm.get(), n, b, out4 + c0 * np, np);
+    PQG_HIP(hipGetLastError());
+  }
+}
+
+}  // namespace pqg
+
+using namespace pqg;
+
+extern "C" pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* C, double* out4,
+                                int on_device, int mode, double* seconds) {
+  return guard([&] {
+    PQG_REQUIRE(ctx && eri8 && C && out4, PQG_ERR_ARG, "NULL argument");
+    PQG_REQUIRE(nao >= 1 && nmo == nao, PQG_ERR_UNSUPPORTED, "pqg_ao2mo needs nmo == nao");
+    PQG_REQUIRE(mode == PQG_AO2MO_FP64 || mode == PQG_AO2MO_BF16X3, PQG_ERR_ARG, "unknown mode");
+    PQG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int n = nao;
+    const int64_t np = (int64_t)n * (n + 1) / 2;
+    const int64_t n8 = np * (np + 1) / 2;
+    DevBuf<double> e8, cd, o4;
+    const double *de8 = eri8, *dC = C;
+    double* dout = out4;
+    if (!on_device) {
+      e8.reset(ctx, n8);
+      cd.reset(ctx, (size_t)n * n);
+      o4.reset(ctx, (size_t)np * np);
+      PQG_HIP(hipMemcpyAsync(e8.get(), eri8, n8 * 8, hipMemcpyHostToDevice, st));
+      PQG_HIP(hipMemcpyAsync(cd.get(), C, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
+      de8 = e8.get();
+      dC = cd.get();
+      dout = o4.get();
+    }
+    hipEvent_t t0, t1;
+    PQG_HIP(hipEventCreate(&t0));
+    PQG_HIP(hipEventCreate(&t1));
+    PQG_HIP(hipEventRecord(t0, st));
+    if (mode == PQG_AO2MO_FP64)
+      ao2mo_impl<double>(ctx, n, de8, dC, dout, st);
+    else
+      ao2mo_impl<float>(ctx, n, de8, dC, dout, st);
+    PQG_HIP(hipEventRecord(t1, st));
+    PQG_HIP(hipEventSynchronize(t1));
+    float ms = 0.f;
+    PQG_HIP(hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (seconds) *seconds = ms * 1e-3;
+    if (!on_device) {
+      PQG_HIP(hipMemcpyAsync(out4, dout, (size_t)np * np * 8, hipMemcpyDeviceToHost, st));
+      PQG_HIP(hipStreamSynchronize(st));
+    }
+  });
+}

@@ -1,0 +1,1366 @@
+// CI determinant set and the matrix-free sigma build on gfx950.
+//
+// Layout in HBM
+//   * strings: one uint64 per string (orbital o = bit o), per (species, spin) set, sorted by
+//     integer value.  A string's index is its colex rank (combinatorial number system),
+//     computed from 8-bit chunk tables: rank(S) = sum_c T_c[byte_c][electrons at or below c].
+//   * determinant (internal) address: ((i_ea * n_eb + i_eb) * n_pa + i_pa) * n_pb + i_pb.
+//     The reference's addresses (level-major string order, determinant_creation.cpp:139-207)
+//     are mapped at the boundary; for truncated spaces the internal product space carries a
+//     validity mask.
+//   * CI vectors: N x k column-major (k vectors of one Davidson block).
+//   * integrals: coupling blocks V[t][u] = (P|Q) row-major [npair_t][npair_u] per ordered
+//     pair of string sets, charge factor q_t*q_u folded in for mixed species; per-string tables
+//     A (one-string part of single-excitation elements), D (double-excitation elements with
+//     phase), X (spectator sums sum_{i in occ(I_u)} (P|ii)), E1/DW (diagonal parts).
+//
+// Sigma (pull form, no atomics, deterministic): sigma = H C is split by which strings change.
+//   pass A, one workgroup per "row" string of the first set: stages C(row, :) in LDS and
+//     accumulates, for every column determinant, the diagonal, all excitations of the
+//     column-side strings (gathered from the staged row) and, for every single excitation
+//     row -> J of the row string, the row x column doubles from the staged row C(J, :);
+//   pass B, the same kernel on the transposed vector: excitations of the first string only.
+// Excitations are enumerated on the fly from the bitstrings (ctz loops), phases from
+// popcounts, excited-string indices from the rank tables.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "pqg_ci.hpp"
+
+namespace pqg {
+
+// ---------------------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ int ctz64(uint64_t x) { return __builtin_ctzll(x); }
+__device__ __forceinline__ int pairidx(int i, int j) { return i > j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+__device__ __forceinline__ uint64_t low_mask(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
+// bits strictly between lo and hi
+__device__ __forceinline__ uint64_t between_mask(int lo, int hi) { return low_mask(hi) & ~low_mask(lo + 1); }
+
+__device__ __forceinline__ int str_index(const SlotDev& s, const uint32_t* tab, uint64_t J) {
+  uint32_t r = 0;
+  int e = s.nocc;
+  const int w = s.nocc + 1;
+  for (int c = s.nchunk - 1; c >= 0; --c) {
+    const uint32_t x = (uint32_t)(J >> (8 * c)) & 255u;
+    r += tab[((c << 8) + x) * w + e];
+    e -= __popc(x);
+  }
+  return s.r2i ? s.r2i[r] : (int)r;
+}
+
+// Visit all single excitations I -> J of one string: (e, h, p, Q = idx2(h, p), odd phase, J).
+// e enumerates occupied h (ascending) outer, virtual p (ascending) inner.
+template <class F>
+__device__ __forceinline__ void for_singles(uint64_t I, int n_orb, F&& f) {
+  const uint64_t vm = ~I & low_mask(n_orb);
+  int e = 0;
+  for (uint64_t occ = I; occ; occ &= occ - 1) {
+    const int h = ctz64(occ);
+    for (uint64_t vir = vm; vir; vir &= vir - 1) {
+      const int p = ctz64(vir);
+      const int lo = min(h, p), hi = max(h, p);
+      const int odd = __popcll(I & between_mask(lo, hi)) & 1;
+      f(e, h, p, hi * (hi + 1) / 2 + lo, odd, I ^ ((1ull << h) | (1ull << p)));
+      ++e;
+    }
+  }
+}
+
+// Visit all same-string double excitations: holes h0 < h1, particles p0 < p1.
+template <class F>
+__device__ __forceinline__ void for_doubles(uint64_t I, int n_orb, F&& f) {
+  const uint64_t vm = ~I & low_mask(n_orb);
+  int e = 0;
+  for (uint64_t o0 = I; o0; o0 &= o0 - 1) {
+    const int h0 = ctz64(o0);
+    for (uint64_t o1 = o0 & (o0 - 1); o1; o1 &= o1 - 1) {
+      const int h1 = ctz64(o1);
+      for (uint64_t v0 = vm; v0; v0 &= v0 - 1) {
+        const int p0 = ctz64(v0);
+        for (uint64_t v1 = v0 & (v0 - 1); v1; v1 &= v1 - 1) {
+          const int p1 = ctz64(v1);
+          f(e, h0, h1, p0, p1, I ^ ((1ull << h0) | (1ull << h1) | (1ull << p0) | (1ull << p1)));
+          ++e;
+        }
+      }
+    }
+  }
+}
+
+// get_phase for a double excitation (determinant_operations.cpp:47-116), one word.
+__device__ __forceinline__ int double_phase_odd(uint64_t I, int h0, int h1, int p0, int p1) {
+  int a = min(h0, p0), b = max(h0, p0), c = min(h1, p1), d = max(h1, p1);
+  int n = __popcll(I & between_mask(a, b)) + __popcll(I & between_mask(c, d));
+  if (a < c && c < b && b < d) n++;
+  return n & 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// table builders (once per CI)
+// ---------------------------------------------------------------------------------------
+// A[e][I] = h_hp + sum_{i in occ(I)} ((hp|ii) - (hi|ip))
+__global__ void k_tab_singles(SlotDev s, const double* __restrict__ h, const double* __restrict__ V, int npair,
+                              double* __restrict__ A) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= s.nstr) return;
+  const uint64_t b = s.bits[I];
+  const int n = s.n_orb;
+  for_singles(b, n, [&](int e, int hh, int pp, int Q, int, uint64_t) {
+    double v = h[hh * n + pp];
+    for (uint64_t o = b; o; o &= o - 1) {
+      const int i = ctz64(o);
+      v += V[(size_t)Q * npair + pairidx(i, i)] - V[(size_t)pairidx(hh, i) * npair + pairidx(i, pp)];
+    }
+    A[(size_t)e * s.nstr + I] = v;
+  });
+}
+
+// D[e][I] = phase * ((h0 p0|h1 p1) - (h0 p1|h1 p0))   (slater_condon_singleparticle.cpp:107-175)
+__global__ void k_tab_doubles(SlotDev s, const double* __restrict__ V, int npair, double* __restrict__ D) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= s.nstr) return;
+  const uint64_t b = s.bits[I];
+  for_doubles(b, s.n_orb, [&](int e, int h0, int h1, int p0, int p1, uint64_t) {
+    double v = V[(size_t)pairidx(h0, p0) * npair + pairidx(h1, p1)] - V[(size_t)pairidx(h0, p1) * npair + pairidx(h1, p0)];
+    D[(size_t)e * s.nstr + I] = double_phase_odd(b, h0, h1, p0, p1) ? -v : v;
+  });
+}
+
+// X[I_u][P] = sum_{i in occ(I_u)} V_tu[P][ii]   (V_tu includes the charge factor)
+__global__ void k_tab_spectator(SlotDev su, const double* __restrict__ Vtu, int npair_t, int npair_u,
+                                double* __restrict__ X) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)su.nstr * npair_t) return;
+  const int I = (int)(idx / npair_t), P = (int)(idx % npair_t);
+  const uint64_t b = su.bits[I];
+  double v = 0.0;
+  for (uint64_t o = b; o; o &= o - 1) {
+    const int i = ctz64(o);
+    v += Vtu[(size_t)P * npair_u + pairidx(i, i)];
+  }
+  X[idx] = v;
+}
+
+// E1[I] = sum_{i in occ} h_ii + 1/2 sum_{i,j in occ} ((ii|jj) - (ij|ij))
+__global__ void k_tab_e1(SlotDev s, const double* __restrict__ h, const double* __restrict__ V, int npair,
+                         double* __restrict__ E1) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= s.nstr) return;
+  const uint64_t b = s.bits[I];
+  const int n = s.n_orb;
+  double e1 = 0.0, e2 = 0.0;
+  for (uint64_t o = b; o; o &= o - 1) {
+    const int i = ctz64(o);
+    e1 += h[i * n + i];
+    for (uint64_t q = b; q; q &= q - 1) {
+      const int j = ctz64(q);
+      e2 += V[(size_t)pairidx(i, i) * npair + pairidx(j, j)] - V[(size_t)pairidx(i, j) * npair + pairidx(i, j)];
+    }
+  }
+  E1[I] = e1 + 0.5 * e2;
+}
+
+// DW[I_u][i] = sum_{j in occ(I_u)} w(i, j):  w = 1/2 (V_tu[ii][jj] + V_ut[jj][ii]) (same species)
+// or V_tu[ii][jj] (mixed species, charge factor already in V_tu).
+__global__ void k_tab_diagpair(SlotDev su, int n_t, const double* __restrict__ Vtu, const double* __restrict__ Vut,
+                               int npair_t, int npair_u, double* __restrict__ DW) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)su.nstr * n_t) return;
+  const int I = (int)(idx / n_t), i = (int)(idx % n_t);
+  const uint64_t b = su.bits[I];
+  double v = 0.0;
+  for (uint64_t o = b; o; o &= o - 1) {
+    const int j = ctz64(o);
+    if (Vut)
+      v += 0.5 * (Vtu[(size_t)pairidx(i, i) * npair_u + pairidx(j, j)] + Vut[(size_t)pairidx(j, j) * npair_t + pairidx(i, i)]);
+    else
+      v += Vtu[(size_t)pairidx(i, i) * npair_u + pairidx(j, j)];
+  }
+  DW[idx] = v;
+}
+
+struct DiagArgs {
+  SlotDev sl[kSlots];
+  const double* E1[kSlots];
+  const double* DW[kSlots][kSlots];
+  int64_t stride[kSlots];
+  int nslot;
+  int64_t n;
+  const uint8_t* valid;
+};
+
+// diagonal_Hii (slater_condon.cpp:102-153), internal order
+__global__ void k_hdiag(DiagArgs a, double* __restrict__ Hd) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= a.n) return;
+  if (a.valid && !a.valid[I]) {
+    Hd[I] = 1e30;
+    return;
+  }
+  int idx[kSlots];
+  uint64_t bits[kSlots];
+  int64_t r = I;
+  for (int t = a.nslot - 1; t >= 0; --t) {
+    idx[t] = (int)(r % a.sl[t].nstr);
+    r /= a.sl[t].nstr;
+    bits[t] = a.sl[t].bits[idx[t]];
+  }
+  double e = 0.0;
+  for (int t = 0; t < a.nslot; t++) e += a.E1[t][idx[t]];
+  for (int t = 0; t < a.nslot; t++)
+    for (int u = t + 1; u < a.nslot; u++) {
+      const double* dw = a.DW[t][u];
+      if (!dw) continue;
+      const int nt = a.sl[t].n_orb;
+      for (uint64_t o = bits[t]; o; o &= o - 1) e += dw[(size_t)idx[u] * nt + ctz64(o)];
+    }
+  Hd[I] = e;
+}
+
+// ---------------------------------------------------------------------------------------
+// sigma pass kernel
+// ---------------------------------------------------------------------------------------
+constexpr int kCPT = 16;  // max columns per thread
+
+enum PassFlags { F_DIAG = 1, F_COLS = 2, F_MIXED = 4, F_ACCUM = 8 };
+
+template <typename T>
+struct PassArgs {
+  SlotDev sl[kSlots];
+  Tables<T> tb;
+  int nr, rs[3], rn[3];  // row part: slots and string counts
+  int nc, cs[3], cn[3];  // column part
+  int64_t cstride[3];    // column-index stride of each column slot
+  int64_t M;             // columns per row
+  int tiles, tile_cols, cpt;
+  const T* C;
+  int64_t ldc;
+  T* S;
+  int64_t lds;
+  const double* Hd;
+  int flags;
+  int off_x, off_v, off_rt;  // LDS byte offsets
+  int x_off[3], v_off[3], rt_off[3];
+};
+
+template <int NC>
+constexpr int pass_threads() { return NC == 1 ? 1024 : 256; }
+
+template <typename T, int NC>
+__global__ __launch_bounds__(pass_threads<NC>()) void k_sigma_pass(const PassArgs<T> a) {
+  extern __shared__ __align__(16) char smem[];
+  T* crow = reinterpret_cast<T*>(smem);
+  T* xs = reinterpret_cast<T*>(smem + a.off_x);
+  T* vr = reinterpret_cast<T*>(smem + a.off_v);
+  uint32_t* rt = reinterpret_cast<uint32_t*>(smem + a.off_rt);
+
+  const int root = blockIdx.y;
+  const int64_t row = blockIdx.x / a.tiles;
+  const int tile = blockIdx.x % a.tiles;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const T* __restrict__ Cr = a.C + (int64_t)root * a.ldc;
+  const int64_t M = a.M;
+
+  int ri[3];
+  uint64_t rb[3];
+  {
+    int64_t r = row;
+    for (int q = a.nr - 1; q >= 0; --q) {
+      ri[q] = (int)(r % a.rn[q]);
+      r /= a.rn[q];
+    }
+    for (int q = 0; q < a.nr; q++) rb[q] = a.sl[a.rs[q]].bits[ri[q]];
+  }
+  // rank tables and spectator sums for the column slots
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const SlotDev& s = a.sl[a.cs[c]];
+    const int tsz = s.nchunk * 256 * (s.nocc + 1);
+    for (int i = tid; i < tsz; i += nth) rt[a.rt_off[c] + i] = s.rtab[i];
+    const int t = a.cs[c];
+    for (int P = tid; P < s.npair; P += nth) {
+      T v = 0;
+      for (int q = 0; q < a.nr; q++) {
+        const T* X = a.tb.X[t][a.rs[q]];
+        if (X) v += X[(size_t)ri[q] * s.npair + P];
+      }
+      xs[a.x_off[c] + P] = v;
+    }
+  }
+  for (int64_t m = tid; m < M; m += nth) crow[m] = Cr[row * M + m];
+  __syncthreads();
+
+  T acc[kCPT];
+#pragma unroll
+  for (int j = 0; j < kCPT; j++) acc[j] = 0;
+
+  const int64_t mbase = (int64_t)tile * a.tile_cols + tid;
+
+  // ---- diagonal + column-side excitations -------------------------------------------
+#pragma unroll
+  for (int j = 0; j < kCPT; j++) {
+    if (j >= a.cpt) break;
+    const int64_t m = mbase + (int64_t)j * nth;
+    if (m >= M || m >= (int64_t)(tile + 1) * a.tile_cols) continue;
+    T sum = 0;
+    if (a.flags & F_DIAG) sum = (T)a.Hd[row * M + m] * crow[m];
+    if (a.flags & F_COLS) {
+      int ci[NC];
+      uint64_t cb[NC];
+      {
+        int64_t r = m;
+#pragma unroll
+        for (int c = NC - 1; c >= 0; --c) {
+          ci[c] = (int)(r % a.cn[c]);
+          r /= a.cn[c];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; c++) cb[c] = a.sl[a.cs[c]].bits[ci[c]];
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int t = a.cs[c];
+        const SlotDev& s = a.sl[t];
+        if (s.nocc == 0 || s.nvirt == 0) continue;
+        const uint32_t* tab = rt + a.rt_off[c];
+        const T* __restrict__ At = a.tb.A[t];
+        const T* xrow = xs + a.x_off[c];
+        const int it = ci[c];
+        const int64_t cst = a.cstride[c];
+        // singles of column string c
+        for_singles(cb[c], s.n_orb, [&](int e, int, int, int Q, int odd, uint64_t J) {
+          const int jt = str_index(s, tab, J);
+          if (jt < 0) return;
+          T v = At[(size_t)e * s.nstr + it] + xrow[Q];
+#pragma unroll
+          for (int c2 = 0; c2 < NC; c2++) {
+            if (c2 == c) continue;
+            const T* X = a.tb.X[t][a.cs[c2]];
+            if (X) v += X[(size_t)ci[c2] * s.npair + Q];
+          }
+          const T cv = crow[m + (int64_t)(jt - it) * cst];
+          sum += odd ? -v * cv : v * cv;
+        });
+        // same-string doubles
+        if (s.ndouble > 0) {
+          const T* __restrict__ Dt = a.tb.D[t];
+          for_doubles(cb[c], s.n_orb, [&](int e, int, int, int, int, uint64_t J) {
+            const int jt = str_index(s, tab, J);
+            if (jt < 0) return;
+            sum += Dt[(size_t)e * s.nstr + it] * crow[m + (int64_t)(jt - it) * cst];
+          });
+        }
+        // doubles across two column strings
+#pragma unroll
+        for (int c2 = c + 1; c2 < NC; c2++) {
+          const int u = a.cs[c2];
+          const SlotDev& s2 = a.sl[u];
+          const T* Vtu = a.tb.V[t][u];
+          if (!Vtu || s2.nocc == 0 || s2.nvirt == 0) continue;
+          const uint32_t* tab2 = rt + a.rt_off[c2];
+          const int iu = ci[c2];
+          const int64_t cst2 = a.cstride[c2];
+          for_singles(cb[c], s.n_orb, [&](int, int, int, int Q, int odd, uint64_t J) {
+            const int jt = str_index(s, tab, J);
+            if (jt < 0) return;
+            const int64_t mj = m + (int64_t)(jt - it) * cst;
+            const T* vrowQ = Vtu + (size_t)Q * s2.npair;
+            T part = 0;
+            for_singles(cb[c2], s2.n_orb, [&](int, int, int, int Q2, int odd2, uint64_t J2) {
+              const int ju = str_index(s2, tab2, J2);
+              if (ju < 0) return;
+              const T cv = crow[mj + (int64_t)(ju - iu) * cst2];
+              const T v = vrowQ[Q2];
+              part += odd2 ? -v * cv : v * cv;
+            });
+            sum += odd ? -part : part;
+          });
+        }
+      }
+    }
+    acc[j] = sum;
+  }
+
+  // ---- row single x column single doubles ---------------------------------------------
+  if ((a.flags & F_MIXED) && a.nr == 1) {
+    const int t0 = a.rs[0];
+    const SlotDev& s0 = a.sl[t0];
+    const uint64_t I0 = rb[0];
+    const int i0 = ri[0];
+    // the row slot's rank table lives in global memory (uniform lookups)
+    for_singles(I0, s0.n_orb, [&](int, int, int, int Q0, int odd0, uint64_t J0) {
+      const int j0 = str_index(s0, s0.rtab, J0);
+      if (j0 < 0) return;
+      __syncthreads();
+      const int64_t rowJ = row + (int64_t)(j0 - i0);
+      for (int64_t m = tid; m < M; m += nth) crow[m] = Cr[rowJ * M + m];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const int t = a.cs[c];
+        const T* V0t = a.tb.V[t0][t];
+        const int np = a.sl[t].npair;
+        for (int P = tid; P < np; P += nth) vr[a.v_off[c] + P] = V0t ? V0t[(size_t)Q0 * np + P] : (T)0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kCPT; j++) {
+        if (j >= a.cpt) break;
+        const int64_t m = mbase + (int64_t)j * nth;
+        if (m >= M || m >= (int64_t)(tile + 1) * a.tile_cols) continue;
+        int ci[NC];
+        uint64_t cb[NC];
+        {
+          int64_t r = m;
+#pragma unroll
+          for (int c = NC - 1; c >= 0; --c) {
+            ci[c] = (int)(r % a.cn[c]);
+            r /= a.cn[c];
+          }
+#pragma unroll
+          for (int c = 0; c < NC; c++) cb[c] = a.sl[a.cs[c]].bits[ci[c]];
+        }
+        T part = 0;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          const int t = a.cs[c];
+          const SlotDev& s = a.sl[t];
+          if (s.nocc == 0 || s.nvirt == 0 || !a.tb.V[t0][t]) continue;
+          const uint32_t* tab = rt + a.rt_off[c];
+          const T* vrow = vr + a.v_off[c];
+          const int it = ci[c];
+          const int64_t cst = a.cstride[c];
+          for_singles(cb[c], s.n_orb, [&](int, int, int, int Q, int odd, uint64_t J) {
+            const int jt = str_index(s, tab, J);
+            if (jt < 0) return;
+            const T cv = crow[m + (int64_t)(jt - it) * cst];
+            const T v = vrow[Q];
+            part += odd ? -v * cv : v * cv;
+          });
+        }
+        acc[j] += odd0 ? -part : part;
+      }
+    });
+  }
+
+  // ---- write ---------------------------------------------------------------------------
+  T* __restrict__ Sr = a.S + (int64_t)root * a.lds;
+#pragma unroll
+  for (int j = 0; j < kCPT; j++) {
+    if (j >= a.cpt) break;
+    const int64_t m = mbase + (int64_t)j * nth;
+    if (m >= M || m >= (int64_t)(tile + 1) * a.tile_cols) continue;
+    if (a.flags & F_ACCUM)
+      Sr[row * M + m] += acc[j];
+    else
+      Sr[row * M + m] = acc[j];
+  }
+}
+
+// out[c * rows + r] (+)= in[r * cols + c] for each of k vectors; 64 x 64 tiles through LDS
+template <typename T>
+__global__ __launch_bounds__(256) void k_transpose(const T* __restrict__ in, int64_t ldi, T* __restrict__ out,
+                                                   int64_t ldo, int64_t rows, int64_t cols, int accumulate) {
+  __shared__ T tile[64][65];
+  const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+  const T* src = in + (int64_t)blockIdx.z * ldi;
+  T* dst = out + (int64_t)blockIdx.z * ldo;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int i = ty; i < 64; i += 4) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r < rows && c < cols) tile[i][tx] = src[r * cols + c];
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) {
+    const int64_t c = c0 + i, r = r0 + tx;
+    if (r < rows && c < cols) {
+      if (accumulate)
+        dst[c * rows + r] += tile[tx][i];
+      else
+        dst[c * rows + r] = tile[tx][i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// boundary permutations, masks, conversions
+// ---------------------------------------------------------------------------------------
+struct PermArgs {
+  const int32_t* r2i[kSlots];
+  int64_t nref[kSlots];
+  int64_t stride[kSlots];
+  int nslot;
+  const int64_t* table;  // truncated spaces: explicit map
+  int64_t n;
+};
+
+__device__ __forceinline__ int64_t ref_to_int(const PermArgs& p, int64_t a) {
+  if (p.table) return p.table[a];
+  int64_t r = a, addr = 0;
+  for (int t = p.nslot - 1; t >= 0; --t) {
+    const int64_t d = r % p.nref[t];
+    r /= p.nref[t];
+    addr += (int64_t)p.r2i[t][d] * p.stride[t];
+  }
+  return addr;
+}
+
+__global__ void k_ref2int(PermArgs p, const double* __restrict__ ref, int64_t ldr, double* __restrict__ in, int64_t ldi) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= p.n) return;
+  const int k = blockIdx.y;
+  in[k * ldi + ref_to_int(p, a)] = ref[k * ldr + a];
+}
+
+__global__ void k_int2ref(PermArgs p, const double* __restrict__ in, int64_t ldi, double* __restrict__ ref, int64_t ldr) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= p.n) return;
+  const int k = blockIdx.y;
+  ref[k * ldr + a] = in[k * ldi + ref_to_int(p, a)];
+}
+
+template <typename T>
+__global__ void k_mask(const uint8_t* __restrict__ valid, T* __restrict__ v, int64_t ld, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !valid[i]) v[blockIdx.y * ld + i] = 0;
+}
+
+__global__ void k_d2f(const double* __restrict__ x, float* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// element evaluation (Slater_Condon, slater_condon.cpp:4-100) and parity dumps
+// ---------------------------------------------------------------------------------------
+struct ElemArgs {
+  SlotDev sl[kSlots];
+  Tables<double> tb;
+  int64_t stride[kSlots];
+  int nslot;
+  const double* Hd;
+};
+
+__device__ __forceinline__ int rank_among(uint64_t set, int bit) { return __popcll(set & low_mask(bit)); }
+
+__global__ void k_elements(ElemArgs a, const int64_t* __restrict__ I, const int64_t* __restrict__ J, int64_t npairs,
+                           double* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npairs) return;
+  const int64_t ia = I[q], ja = J[q];
+  if (ia == ja) {
+    out[q] = a.Hd[ia];
+    return;
+  }
+  int ii[kSlots], jj[kSlots], deg[kSlots];
+  uint64_t bi[kSlots], bj[kSlots];
+  int64_t ri = ia, rj = ja;
+  int total = 0;
+  for (int t = a.nslot - 1; t >= 0; --t) {
+    ii[t] = (int)(ri % a.sl[t].nstr);
+    ri /= a.sl[t].nstr;
+    jj[t] = (int)(rj % a.sl[t].nstr);
+    rj /= a.sl[t].nstr;
+    bi[t] = a.sl[t].bits[ii[t]];
+    bj[t] = a.sl[t].bits[jj[t]];
+    deg[t] = __popcll(bi[t] ^ bj[t]) / 2;
+    total += deg[t];
+  }
+  double v = 0.0;
+  if (total == 1) {
+    int t = 0;
+    while (deg[t] == 0) t++;
+    const SlotDev& s = a.sl[t];
+    const int h = ctz64(bi[t] & ~bj[t]), p = ctz64(bj[t] & ~bi[t]);
+    const uint64_t vm = ~bi[t] & low_mask(s.n_orb);
+    const int e = rank_among(bi[t], h) * s.nvirt + rank_among(vm, p);
+    const int Q = pairidx(h, p);
+    v = a.tb.A[t][(size_t)e * s.nstr + ii[t]];
+    for (int u = 0; u < a.nslot; u++)
+      if (u != t && a.tb.X[t][u]) v += a.tb.X[t][u][(size_t)ii[u] * s.npair + Q];
+    if (__popcll(bi[t] & between_mask(min(h, p), max(h, p))) & 1) v = -v;
+  } else if (total == 2) {
+    int t = -1, u = -1;
+    for (int k = 0; k < a.nslot; k++)
+      if (deg[k]) {
+        if (t < 0)
+          t = k;
+        else
+          u = k;
+      }
+    if (u < 0) {  // double within one string
+      const SlotDev& s = a.sl[t];
+      const uint64_t hs = bi[t] & ~bj[t], ps = bj[t] & ~bi[t];
+      const int h0 = ctz64(hs), h1 = 63 - __clzll(hs), p0 = ctz64(ps), p1 = 63 - __clzll(ps);
+      const uint64_t vm = ~bi[t] & low_mask(s.n_orb);
+      const int no = s.nocc, nv = s.nvirt;
+      const int a0 = rank_among(bi[t], h0), a1 = rank_among(bi[t], h1);
+      const int b0 = rank_among(vm, p0), b1 = rank_among(vm, p1);
+      const int eh = a0 * (2 * no - a0 - 1) / 2 + (a1 - a0 - 1);
+      const int ep = b0 * (2 * nv - b0 - 1) / 2 + (b1 - b0 - 1);
+      const int e = eh * (nv * (nv - 1) / 2) + ep;
+      v = a.tb.D[t][(size_t)e * s.nstr + ii[t]];
+    } else if (a.tb.V[t][u]) {  // one single in each of two strings
+      const int ht = ctz64(bi[t] & ~bj[t]), pt = ctz64(bj[t] & ~bi[t]);
+      const int hu = ctz64(bi[u] & ~bj[u]), pu = ctz64(bj[u] & ~bi[u]);
+      v = a.tb.V[t][u][(size_t)pairidx(ht, pt) * a.sl[u].npair + pairidx(hu, pu)];
+      int odd = __popcll(bi[t] & between_mask(min(ht, pt), max(ht, pt))) +
+                __popcll(bi[u] & between_mask(min(hu, pu), max(hu, pu)));
+      if (odd & 1) v = -v;
+    }
+  }
+  out[q] = v;
+}
+
+// For every string I of a set: its singles (which = 1) or doubles (which = 2) as
+// (excited string internal index, phase bit), in enumeration order.
+__global__ void k_dump_excitations(SlotDev s, int which, int per, int32_t* __restrict__ J, int8_t* __restrict__ odd) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= s.nstr) return;
+  const uint64_t b = s.bits[I];
+  if (which == 1) {
+    for_singles(b, s.n_orb, [&](int e, int, int, int, int o, uint64_t Jb) {
+      J[(size_t)I * per + e] = str_index(s, s.rtab, Jb);
+      odd[(size_t)I * per + e] = (int8_t)o;
+    });
+  } else {
+    for_doubles(b, s.n_orb, [&](int e, int h0, int h1, int p0, int p1, uint64_t Jb) {
+      J[(size_t)I * per + e] = str_index(s, s.rtab, Jb);
+      odd[(size_t)I * per + e] = (int8_t)double_phase_odd(b, h0, h1, p0, p1);
+    });
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------
+static uint64_t binom(int n, int k) {
+  if (k < 0 || k > n) return 0;
+  k = std::min(k, n - k);
+  unsigned __int128 r = 1;
+  for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+  return (uint64_t)r;
+}
+
+static int grid1(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// Level-major strings (create_det + create_unique_excitation, determinant_creation.cpp:6-146):
+// the HF string, then for k = 1..max_level every choice of k occupied orbitals (outer, in
+// lexicographic order) and k virtual orbitals (inner, lexicographic).
+static std::vector<uint64_t> reference_strings(int n, int nocc, int max_level) {
+  std::vector<uint64_t> out;
+  const uint64_t hf = nocc ? ((nocc >= 64) ? ~0ull : ((1ull << nocc) - 1)) : 0ull;
+  out.push_back(hf);
+  const int nv = n - nocc;
+  for (int k = 1; k <= max_level; k++) {
+    PQG_REQUIRE(k <= nv, PQG_ERR_ARG, "Excitation level exceeds virtual size!");
+    if (k > nocc) continue;  // no combination of k occupied orbitals
+    // lexicographic k-subsets of {0..m-1} as index vectors
+    auto subsets = [](int m, int kk) {
+      std::vector<std::vector<int>> all;
+      std::vector<int> c(kk);
+      std::iota(c.begin(), c.end(), 0);
+      while (true) {
+        all.push_back(c);
+        int i = kk - 1;
+        while (i >= 0 && c[i] == m - kk + i) i--;
+        if (i < 0) break;
+        c[i]++;
+        for (int j = i + 1; j < kk; j++) c[j] = c[j - 1] + 1;
+      }
+      return all;
+    };
+    const auto occs = subsets(nocc, k);
+    const auto virts = subsets(nv, k);
+    for (const auto& oc : occs)
+      for (const auto& vc : virts) {
+        uint64_t s = hf;
+        for (int o : oc) s &= ~(1ull << o);
+        for (int v : vc) s |= 1ull << (nocc + v);
+        out.push_back(s);
+      }
+  }
+  return out;
+}
+
+// colex rank of a string with k set bits
+static uint64_t colex_rank(uint64_t s) {
+  uint64_t r = 0;
+  int j = 1;
+  for (uint64_t b = s; b; b &= b - 1, j++) r += binom(__builtin_ctzll(b), j);
+  return r;
+}
+
+static void build_slot(pqg_ci* ci, Slot& sl, int n, int nocc, int max_level) {
+  pqg_ctx* ctx = ci->ctx;
+  PQG_REQUIRE(n >= 1 && n <= 63, PQG_ERR_UNSUPPORTED, "n_orb must be in 1..63 (one 64-bit word per string)");
+  PQG_REQUIRE(nocc >= 0 && nocc <= n, PQG_ERR_ARG, "particle count exceeds the orbital count");
+  sl.n_orb = n;
+  sl.nocc = nocc;
+  sl.nvirt = n - nocc;
+  sl.npair = n * (n + 1) / 2;
+  sl.nchunk = (n + 7) / 8;
+  sl.ref = reference_strings(n, nocc, max_level);
+  const uint64_t total = binom(n, nocc);
+  PQG_REQUIRE(total < (1ull << 31), PQG_ERR_UNSUPPORTED, "string set too large for 32-bit string indices");
+  sl.nstr = (int)sl.ref.size();
+  sl.complete = ((uint64_t)sl.nstr == total);
+  sl.sorted = sl.ref;
+  std::sort(sl.sorted.begin(), sl.sorted.end());
+  std::unordered_map<uint64_t, int> pos;
+  pos.reserve(sl.nstr * 2);
+  for (int i = 0; i < sl.nstr; i++) pos[sl.sorted[i]] = i;
+  sl.ref2int.resize(sl.nstr);
+  sl.int2ref.resize(sl.nstr);
+  sl.deg.resize(sl.nstr);
+  for (int r = 0; r < sl.nstr; r++) {
+    const int i = pos[sl.ref[r]];
+    sl.ref2int[r] = i;
+    sl.int2ref[i] = r;
+    sl.deg[r] = __builtin_popcountll(sl.ref[r] ^ sl.ref[0]) / 2;
+  }
+  // rank tables: T_c[x][e] = sum_j C(g_j, e - m + j) over the m set bits g_1 < .. < g_m of x
+  const int w = nocc + 1;
+  std::vector<uint32_t> tab((size_t)sl.nchunk * 256 * w, 0);
+  for (int c = 0; c < sl.nchunk; c++)
+    for (int x = 0; x < 256; x++) {
+      const int m = __builtin_popcount(x);
+      for (int e = m; e <= nocc; e++) {
+        uint64_t v = 0;
+        int j = 1;
+        for (int bb = 0; bb < 8; bb++)
+          if ((x >> bb) & 1) {
+            v += binom(8 * c + bb, e - m + j);
+            j++;
+          }
+        tab[((size_t)c * 256 + x) * w + e] = (uint32_t)v;
+      }
+    }
+  sl.rtab.reset(ctx, tab.size());
+  PQG_HIP(hipMemcpy(sl.rtab.get(), tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+  sl.bits.reset(ctx, sl.nstr);
+  PQG_HIP(hipMemcpy(sl.bits.get(), sl.sorted.data(), sl.nstr * 8, hipMemcpyHostToDevice));
+  if (!sl.complete) {
+    std::vector<int32_t> r2i(total, -1);
+    for (int i = 0; i < sl.nstr; i++) r2i[colex_rank(sl.sorted[i])] = i;
+    sl.r2i.reset(ctx, total);
+    PQG_HIP(hipMemcpy(sl.r2i.get(), r2i.data(), total * 4, hipMemcpyHostToDevice));
+  }
+  sl.d_ref2int.reset(ctx, sl.nstr);
+  PQG_HIP(hipMemcpy(sl.d_ref2int.get(), sl.ref2int.data(), sl.nstr * 4, hipMemcpyHostToDevice));
+  sl.nsingle = (int64_t)sl.nocc * sl.nvirt;
+  sl.ndouble = (int64_t)(sl.nocc * (sl.nocc - 1) / 2) * (sl.nvirt * (sl.nvirt - 1) / 2);
+}
+
+static PermArgs perm_args(const pqg_ci* ci) {
+  PermArgs p{};
+  p.nslot = ci->nslot;
+  for (int t = 0; t < ci->nslot; t++) {
+    p.r2i[t] = ci->slot[t].d_ref2int.get();
+    p.nref[t] = ci->slot[t].nstr;
+    p.stride[t] = ci->stride[t];
+  }
+  p.table = ci->fci ? nullptr : ci->ref_addr.get();
+  p.n = ci->n_ref;
+  return p;
+}
+
+void to_internal(pqg_ci* ci, const double* ref, int64_t ldr, double* in, int64_t ldi, int k, hipStream_t st) {
+  if (!ci->fci)
+    for (int j = 0; j < k; j++) PQG_HIP(hipMemsetAsync(in + j * ldi, 0, ci->n_int * 8, st));
+  dim3 g(grid1(ci->n_ref, 256), k);
+  k_ref2int<<<g, 256, 0, st>>>(perm_args(ci), ref, ldr, in, ldi);
+  PQG_HIP(hipGetLastError());
+}
+
+void to_reference(pqg_ci* ci, const double* in, int64_t ldi, double* ref, int64_t ldr, int k, hipStream_t st) {
+  dim3 g(grid1(ci->n_ref, 256), k);
+  k_int2ref<<<g, 256, 0, st>>>(perm_args(ci), in, ldi, ref, ldr);
+  PQG_HIP(hipGetLastError());
+}
+
+template <typename T>
+void mask_invalid(pqg_ci* ci, T* v, int64_t ld, int k, hipStream_t st) {
+  if (ci->fci) return;
+  dim3 g(grid1(ci->n_int, 256), k);
+  k_mask<T><<<g, 256, 0, st>>>(ci->valid.get(), v, ld, ci->n_int);
+  PQG_HIP(hipGetLastError());
+}
+template void mask_invalid<double>(pqg_ci*, double*, int64_t, int, hipStream_t);
+template void mask_invalid<float>(pqg_ci*, float*, int64_t, int, hipStream_t);
+
+// row-major device block V_tu = (P|Q) with P a pair of set t, Q of set u; charge factor f.
+static void upload_blocks(pqg_ci* ci, const pqg_ci_desc* d) {
+  pqg_ctx* ctx = ci->ctx;
+  for (int t = 0; t < ci->nslot; t++)
+    for (int u = 0; u < ci->nslot; u++) {
+      const Slot &st = ci->slot[t], &su = ci->slot[u];
+      int a = st.species, sa = st.spinblk, b = su.species, sb = su.spinblk;
+      bool transpose = false;
+      if (b < a) {
+        std::swap(a, b);
+        std::swap(sa, sb);
+        transpose = true;
+      }
+      const double* src = d->eri[a][sa][b][sb];
+      PQG_REQUIRE(src != nullptr, PQG_ERR_ARG,
+                  "missing two-body block eri[" + std::to_string(a) + "][" + std::to_string(sa) + "][" +
+                      std::to_string(b) + "][" + std::to_string(sb) + "]");
+      const int np_t = st.npair, np_u = su.npair;
+      const int na = transpose ? np_u : np_t;  // rows of the stored (column-major) block
+      const double f = (st.species == su.species) ? 1.0 : ci->charge[0] * ci->charge[1];
+      std::vector<double> h((size_t)np_t * np_u);
+      for (int P = 0; P < np_t; P++)
+        for (int Q = 0; Q < np_u; Q++)
+          h[(size_t)P * np_u + Q] = f * (transpose ? src[(size_t)Q + (size_t)P * na] : src[(size_t)P + (size_t)Q * na]);
+      ci->V64[t][u].reset(ctx, h.size());
+      PQG_HIP(hipMemcpy(ci->V64[t][u].get(), h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    }
+}
+
+Tables<double> tables64(pqg_ci* ci) {
+  Tables<double> tb;
+  for (int t = 0; t < ci->nslot; t++) {
+    tb.A[t] = ci->slot[t].A.get();
+    tb.D[t] = ci->slot[t].D.get();
+    for (int u = 0; u < ci->nslot; u++) {
+      tb.X[t][u] = ci->X64[t][u].get();
+      tb.V[t][u] = (t != u) ? ci->V64[t][u].get() : nullptr;
+    }
+  }
+  return tb;
+}
+
+Tables<float> tables32(pqg_ci* ci) {
+  Tables<float> tb;
+  for (int t = 0; t < ci->nslot; t++) {
+    tb.A[t] = ci->slot[t].A32.get();
+    tb.D[t] = ci->slot[t].D32.get();
+    for (int u = 0; u < ci->nslot; u++) {
+      tb.X[t][u] = ci->X32[t][u].get();
+      tb.V[t][u] = (t != u) ? ci->V32[t][u].get() : nullptr;
+    }
+  }
+  return tb;
+}
+
+static void to_f32(pqg_ctx* ctx, hipStream_t st, const DevBuf<double>& src, DevBuf<float>& dst) {
+  if (!src.get()) return;
+  dst.reset(ctx, src.n);
+  k_d2f<<<grid1(src.n, 256), 256, 0, st>>>(src.get(), dst.get(), src.n);
+  PQG_HIP(hipGetLastError());
+}
+
+void ensure_mixed(pqg_ci* ci) {
+  if (ci->mixed_ready) return;
+  pqg_ctx* ctx = ci->ctx;
+  for (int t = 0; t < ci->nslot; t++) {
+    to_f32(ctx, ci->stream, ci->slot[t].A, ci->slot[t].A32);
+    to_f32(ctx, ci->stream, ci->slot[t].D, ci->slot[t].D32);
+    for (int u = 0; u < ci->nslot; u++) {
+      to_f32(ctx, ci->stream, ci->X64[t][u], ci->X32[t][u]);
+      if (t != u) to_f32(ctx, ci->stream, ci->V64[t][u], ci->V32[t][u]);
+    }
+  }
+  PQG_HIP(hipStreamSynchronize(ci->stream));
+  ci->mixed_ready = true;
+}
+
+pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* d, hipStream_t stream) {
+  PQG_REQUIRE(ctx && d, PQG_ERR_ARG, "NULL argument");
+  PQG_REQUIRE(d->nspecies == 1 || d->nspecies == 2, PQG_ERR_ARG,
+              "The CI code can currently handle up to a maximum of 2 unique quantum particle types.");
+  PQG_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<pqg_ci> ci(new pqg_ci());
+  ci->ctx = ctx;
+  ci->ctx_ref.set(ctx);
+  ci->stream = stream ? stream : ctx->stream;
+  ci->nsp = d->nspecies;
+  ci->nslot = 2 * d->nspecies;
+  for (int p = 0; p < ci->nsp; p++) {
+    ci->charge[p] = d->charge[p];
+    ci->nspin[p] = d->nspin_blocks[p];
+    PQG_REQUIRE(ci->nspin[p] == 1 || ci->nspin[p] == 2, PQG_ERR_ARG, "nspin_blocks must be 1 or 2");
+    PQG_REQUIRE(d->n_orb[p] >= 1 && d->n_orb[p] <= 63, PQG_ERR_UNSUPPORTED,
+                "n_orb must be in 1..63 (one 64-bit word per string)");
+    for (int s = 0; s < 2; s++) {
+      Slot& sl = ci->slot[2 * p + s];
+      sl.species = p;
+      sl.spin = s;
+      sl.spinblk = s % ci->nspin[p];
+      build_slot(ci.get(), sl, d->n_orb[p], s == 0 ? d->n_alpha[p] : d->n_beta[p], d->ex_level[p][s]);
+    }
+  }
+  // ---- determinant space (create_excitation, determinant_creation.cpp:148-207) ----------
+  for (int t = 0; t < kSlots; t++) ci->nref_str[t] = (t < ci->nslot) ? ci->slot[t].nstr : 1;
+  ci->n_int = 1;
+  for (int t = ci->nslot - 1; t >= 0; --t) {
+    ci->stride[t] = ci->n_int;
+    ci->n_int *= ci->slot[t].nstr;
+  }
+  auto maxdeg = [&](int t) { return *std::max_element(ci->slot[t].deg.begin(), ci->slot[t].deg.end()); };
+  const int tot0 = d->ex_level[0][2];
+  const int tot1 = ci->nsp == 2 ? d->ex_level[1][2] : 0;
+  const int md0 = maxdeg(0) + maxdeg(1);
+  const int md1 = ci->nsp == 2 ? maxdeg(2) + maxdeg(3) : 0;
+  ci->fci = md0 <= tot0 && (ci->nsp == 1 || md1 <= tot1) && md0 + md1 <= d->max_collective;
+  if (ci->fci) {
+    ci->n_ref = ci->n_int;
+  } else {
+    std::vector<int64_t> addr;
+    std::vector<uint8_t> valid(ci->n_int, 0);
+    const auto& s0 = ci->slot[0];
+    const auto& s1 = ci->slot[1];
+    const int n2 = ci->nsp == 2 ? ci->slot[2].nstr : 1, n3 = ci->nsp == 2 ? ci->slot[3].nstr : 1;
+    for (int i = 0; i < s0.nstr; i++)
+      for (int j = 0; j < s1.nstr; j++) {
+        const int g0 = s0.deg[i] + s1.deg[j];
+        if (g0 > tot0) continue;
+        for (int k = 0; k < n2; k++)
+          for (int l = 0; l < n3; l++) {
+            int g1 = 0;
+            if (ci->nsp == 2) {
+              g1 = ci->slot[2].deg[k] + ci->slot[3].deg[l];
+              if (g1 > tot1) continue;
+            }
+            if (g0 + g1 > d->max_collective) continue;
+            int64_t ia = (int64_t)s0.ref2int[i] * ci->stride[0] + (int64_t)s1.ref2int[j] * ci->stride[1];
+            if (ci->nsp == 2)
+              ia += (int64_t)ci->slot[2].ref2int[k] * ci->stride[2] + (int64_t)ci->slot[3].ref2int[l] * ci->stride[3];
+            addr.push_back(ia);
+            valid[ia] = 1;
+            ci->host_unfold.insert(ci->host_unfold.end(), {i, j, k, l});
+          }
+      }
+    ci->n_ref = (int64_t)addr.size();
+    ci->ref_addr.reset(ctx, addr.size());
+    PQG_HIP(hipMemcpy(ci->ref_addr.get(), addr.data(), addr.size() * 8, hipMemcpyHostToDevice));
+    ci->valid.reset(ctx, valid.size());
+    PQG_HIP(hipMemcpy(ci->valid.get(), valid.data(), valid.size(), hipMemcpyHostToDevice));
+  }
+
+  // ---- integrals and per-string tables ---------------------------------------------------
+  upload_blocks(ci.get(), d);
+  std::vector<DevBuf<double>> hdev(ci->nslot);
+  for (int t = 0; t < ci->nslot; t++) {
+    const Slot& sl = ci->slot[t];
+    const double* hs = d->h[sl.species][sl.spinblk];
+    PQG_REQUIRE(hs != nullptr, PQG_ERR_ARG, "missing one-body block");
+    const int n = sl.n_orb;
+    std::vector<double> hr((size_t)n * n);
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) hr[(size_t)i * n + j] = hs[i + (size_t)j * n];
+    hdev[t].reset(ctx, hr.size());
+    PQG_HIP(hipMemcpy(hdev[t].get(), hr.data(), hr.size() * 8, hipMemcpyHostToDevice));
+  }
+  hipStream_t st = ci->stream;
+  for (int t = 0; t < ci->nslot; t++) {
+    Slot& sl = ci->slot[t];
+    const double* Vtt = ci->V64[t][t].get();
+    const SlotDev sd = sl.dev();
+    if (sl.nsingle > 0) {
+      sl.A.reset(ctx, (size_t)sl.nsingle * sl.nstr);
+      k_tab_singles<<<grid1(sl.nstr, 128), 128, 0, st>>>(sd, hdev[t].get(), Vtt, sl.npair, sl.A.get());
+      PQG_HIP(hipGetLastError());
+    }
+    if (sl.ndouble > 0) {
+      sl.D.reset(ctx, (size_t)sl.ndouble * sl.nstr);
+      k_tab_doubles<<<grid1(sl.nstr, 128), 128, 0, st>>>(sd, Vtt, sl.npair, sl.D.get());
+      PQG_HIP(hipGetLastError());
+    }
+    sl.E1.reset(ctx, sl.nstr);
+    k_tab_e1<<<grid1(sl.nstr, 128), 128, 0, st>>>(sd, hdev[t].get(), Vtt, sl.npair, sl.E1.get());
+    PQG_HIP(hipGetLastError());
+  }
+  for (int t = 0; t < ci->nslot; t++)
+    for (int u = 0; u < ci->nslot; u++) {
+      if (u == t) continue;
+      const Slot &stt = ci->slot[t], &su = ci->slot[u];
+      if (su.nocc > 0 && stt.nsingle > 0) {  // spectator sums for singles of t
+        const int64_t n = (int64_t)su.nstr * stt.npair;
+        ci->X64[t][u].reset(ctx, n);
+        k_tab_spectator<<<grid1(n, 256), 256, 0, st>>>(su.dev(), ci->V64[t][u].get(), stt.npair, su.npair,
+                                                       ci->X64[t][u].get());
+        PQG_HIP(hipGetLastError());
+      }
+      if (u > t && su.nocc > 0 && stt.nocc > 0) {  // diagonal pair terms
+        const int64_t n = (int64_t)su.nstr * stt.n_orb;
+        ci->DW[t][u].reset(ctx, n);
+        const bool same = stt.species == su.species;
+        k_tab_diagpair<<<grid1(n, 256), 256, 0, st>>>(su.dev(), stt.n_orb, ci->V64[t][u].get(),
+                                                      same ? ci->V64[u][t].get() : nullptr, stt.npair, su.npair,
+                                                      ci->DW[t][u].get());
+        PQG_HIP(hipGetLastError());
+      }
+    }
+  // ---- diagonal ----------------------------------------------------------------------------
+  DiagArgs da{};
+  da.nslot = ci->nslot;
+  da.n = ci->n_int;
+  da.valid = ci->fci ? nullptr : ci->valid.get();
+  for (int t = 0; t < ci->nslot; t++) {
+    da.sl[t] = ci->slot[t].dev();
+    da.E1[t] = ci->slot[t].E1.get();
+    da.stride[t] = ci->stride[t];
+    for (int u = 0; u < ci->nslot; u++) da.DW[t][u] = ci->DW[t][u].get();
+  }
+  ci->hdiag.reset(ctx, ci->n_int);
+  k_hdiag<<<grid1(ci->n_int, 256), 256, 0, st>>>(da, ci->hdiag.get());
+  PQG_HIP(hipGetLastError());
+  PQG_HIP(hipStreamSynchronize(st));
+  return ci.release();
+}
+
+// ---------------------------------------------------------------------------------------
+// sigma driver
+// ---------------------------------------------------------------------------------------
+template <typename T>
+static void launch_pass(pqg_ci* ci, const Tables<T>& tb, const std::vector<int>& rows, const std::vector<int>& cols,
+                        const T* C, int64_t ldc, T* S, int64_t lds, int k, int flags, hipStream_t st) {
+  PassArgs<T> a{};
+  for (int t = 0; t < ci->nslot; t++) a.sl[t] = ci->slot[t].dev();
+  a.tb = tb;
+  a.nr = (int)rows.size();
+  a.nc = (int)cols.size();
+  int64_t nrows = 1, M = 1;
+  for (int q = 0; q < a.nr; q++) {
+    a.rs[q] = rows[q];
+    a.rn[q] = ci->slot[rows[q]].nstr;
+    nrows *= a.rn[q];
+  }
+  for (int c = 0; c < a.nc; c++) {
+    a.cs[c] = cols[c];
+    a.cn[c] = ci->slot[cols[c]].nstr;
+    M *= a.cn[c];
+  }
+  int64_t s = 1;
+  for (int c = a.nc - 1; c >= 0; --c) {
+    a.cstride[c] = s;
+    s *= a.cn[c];
+  }
+  a.M = M;
+  a.C = C;
+  a.ldc = ldc;
+  a.S = S;
+  a.lds = lds;
+  a.Hd = ci->hdiag.get();
+  a.flags = flags;
+  // block: about 4 columns per thread for small rows, up to 1024 threads
+  const int max_threads = a.nc == 1 ? pass_threads<1>() : pass_threads<3>();
+  int nth = (int)std::min<int64_t>(max_threads, std::max<int64_t>(64, ((M + 3) / 4 + 63) / 64 * 64));
+  int64_t tile_cols = std::min<int64_t>(M, (int64_t)nth * kCPT);
+  a.tiles = (int)((M + tile_cols - 1) / tile_cols);
+  a.tile_cols = (int)tile_cols;
+  a.cpt = (int)((tile_cols + nth - 1) / nth);
+  // LDS layout
+  size_t off = (size_t)M * sizeof(T);
+  off = (off + 15) / 16 * 16;
+  a.off_x = (int)off;
+  size_t xo = 0;
+  for (int c = 0; c < a.nc; c++) {
+    a.x_off[c] = (int)xo;
+    xo += ci->slot[cols[c]].npair;
+  }
+  off += xo * sizeof(T);
+  off = (off + 15) / 16 * 16;
+  a.off_v = (int)off;
+  for (int c = 0; c < a.nc; c++) a.v_off[c] = a.x_off[c];
+  off += xo * sizeof(T);
+  off = (off + 15) / 16 * 16;
+  a.off_rt = (int)off;
+  size_t ro = 0;
+  for (int c = 0; c < a.nc; c++) {
+    a.rt_off[c] = (int)ro;
+    const Slot& sl = ci->slot[cols[c]];
+    ro += (size_t)sl.nchunk * 256 * (sl.nocc + 1);
+  }
+  off += ro * 4;
+  PQG_REQUIRE(off <= 160 * 1024, PQG_ERR_UNSUPPORTED,
+              "sigma row (" + std::to_string(M) + " columns) does not fit in LDS");
+  dim3 grid((unsigned)(nrows * a.tiles), (unsigned)k);
+  if (a.nc == 1) {
+    auto kern = k_sigma_pass<T, 1>;
+    PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)off));
+    kern<<<grid, nth, off, st>>>(a);
+  } else {
+    auto kern = k_sigma_pass<T, 3>;
+    PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)off));
+    kern<<<grid, nth, off, st>>>(a);
+  }
+  PQG_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void transpose(const T* in, int64_t ldi, T* out, int64_t ldo, int64_t rows, int64_t cols, int k, bool acc,
+                      hipStream_t st) {
+  dim3 g((unsigned)((cols + 63) / 64), (unsigned)((rows + 63) / 64), (unsigned)k);
+  k_transpose<T><<<g, 256, 0, st>>>(in, ldi, out, ldo, rows, cols, acc ? 1 : 0);
+  PQG_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k, int64_t ld, hipStream_t st) {
+  const int64_t N = ci->n_int;
+  const int64_t n0 = ci->slot[0].nstr;
+  const int64_t M = N / n0;
+  std::vector<int> rowsA = {0}, colsA, rowsB;
+  for (int t = 1; t < ci->nslot; t++) {
+    colsA.push_back(t);
+    rowsB.push_back(t);
+  }
+  const bool need_b = ci->slot[0].nsingle > 0;
+  // pass A: diagonal, column-side excitations, row x column doubles
+  launch_pass<T>(ci, tb, rowsA, colsA, C, ld, S, ld, k, F_DIAG | F_COLS | F_MIXED, st);
+  if (need_b) {
+    const size_t need = 2 * (size_t)N * k * sizeof(T);
+    if (ci->scratch.n < need) ci->scratch.reset(ci->ctx, need);
+    T* CT = reinterpret_cast<T*>(ci->scratch.get());
+    T* ST = CT + (size_t)N * k;
+    transpose<T>(C, ld, CT, N, n0, M, k, false, st);
+    // pass B: excitations of the first string, on the transposed vector
+    launch_pass<T>(ci, tb, rowsB, {0}, CT, N, ST, N, k, F_COLS, st);
+    transpose<T>(ST, N, S, ld, M, n0, k, true, st);
+  }
+  mask_invalid<T>(ci, S, ld, k, st);
+}
+
+void sigma_internal(pqg_ci* ci, const double* C, double* S, int k, int64_t ld, hipStream_t st) {
+  sigma_impl<double>(ci, tables64(ci), C, S, k, ld, st);
+}
+void sigma_internal(pqg_ci* ci, const float* C, float* S, int k, int64_t ld, hipStream_t st) {
+  ensure_mixed(ci);
+  sigma_impl<float>(ci, tables32(ci), C, S, k, ld, st);
+}
+
+__global__ void k_f2d(const float* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (double)x[i];
+}
+
+uint64_t estimate_ci_bytes(const pqg_ci_desc* d, int nvec_max, int precision) {
+  uint64_t N = 1, tables = 0;
+  for (int p = 0; p < d->nspecies; p++) {
+    const int n = d->n_orb[p];
+    const uint64_t np = (uint64_t)n * (n + 1) / 2;
+    for (int s = 0; s < 2; s++) {
+      const int k = s == 0 ? d->n_alpha[p] : d->n_beta[p];
+      const uint64_t ns = binom(n, k);
+      N *= ns;
+      const uint64_t nv = n - k;
+      tables += ns * (8 + 4 + 8) + (uint64_t)((n + 7) / 8) * 256 * (k + 1) * 4;
+      tables += ns * (uint64_t)k * nv * 12;                                  // A (+FP32)
+      tables += ns * (uint64_t)(k * (k - 1) / 2) * (nv * (nv - 1) / 2) * 12;  // D (+FP32)
+      tables += ns * np * 12 * 2;                                            // spectator sums
+    }
+    tables += np * np * 12 * 8;  // coupling blocks
+  }
+  const uint64_t w = 8;
+  uint64_t vec = N * w * (2ull * nvec_max + 8);  // V, W, work vectors
+  vec += N * 8;                                  // diagonal
+  vec += N * (precision == PQG_MIXED ? 4 : 8) * 4ull * 2;  // sigma transposes
+  return tables + vec + (64ull << 20);
+}
+
+}  // namespace pqg
+
+using namespace pqg;
+
+extern "C" {
+
+pqg_status pqg_ci_create(pqg_ctx* ctx, const pqg_ci_desc* desc, pqg_ci** out) {
+  return guard([&] {
+    PQG_REQUIRE(out, PQG_ERR_ARG, "out is NULL");
+    *out = create_ci(ctx, desc, nullptr);
+  });
+}
+
+pqg_status pqg_ci_destroy(pqg_ci* ci) {
+  return guard([&] {
+    if (!ci) return;
+    (void)hipStreamSynchronize(ci->stream);
+    delete ci;
+  });
+}
+
+int64_t pqg_ci_ndets(const pqg_ci* ci) { return ci ? ci->n_ref : -1; }
+
+int64_t pqg_ci_nstrings(const pqg_ci* ci, int part, int spin) {
+  if (!ci || part < 0 || part >= ci->nsp || spin < 0 || spin > 1) return -1;
+  return ci->slot[2 * part + spin].nstr;
+}
+
+pqg_status pqg_ci_diag(pqg_ci* ci, double* hdiag) {
+  return guard([&] {
+    PQG_REQUIRE(ci && hdiag, PQG_ERR_ARG, "NULL argument");
+    hipStream_t st = ci->stream;
+    DevBuf<double> out(ci->ctx, ci->n_ref);
+    to_reference(ci, ci->hdiag.get(), ci->n_int, out.get(), ci->n_ref, 1, st);
+    PQG_HIP(hipMemcpyAsync(hdiag, out.get(), ci->n_ref * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+  });
+}
+
+static int64_t ref_to_int_host(const pqg_ci* ci, int64_t a) {
+  int t4[4] = {0, 0, 0, 0};
+  if (ci->fci) {
+    int64_t r = a;
+    for (int t = ci->nslot - 1; t >= 0; --t) {
+      t4[t] = (int)(r % ci->slot[t].nstr);
+      r /= ci->slot[t].nstr;
+    }
+  } else {
+    for (int t = 0; t < 4; t++) t4[t] = ci->host_unfold[(size_t)a * 4 + t];
+  }
+  int64_t ia = 0;
+  for (int t = 0; t < ci->nslot; t++) ia += (int64_t)ci->slot[t].ref2int[t4[t]] * ci->stride[t];
+  return ia;
+}
+
+pqg_status pqg_ci_elements(pqg_ci* ci, const int64_t* i, const int64_t* j, int64_t npairs, double* out) {
+  return guard([&] {
+    PQG_REQUIRE(ci && i && j && out, PQG_ERR_ARG, "NULL argument");
+    if (npairs <= 0) return;
+    std::vector<int64_t> ii(npairs), jj(npairs);
+    for (int64_t q = 0; q < npairs; q++) {
+      PQG_REQUIRE(i[q] >= 0 && i[q] < ci->n_ref && j[q] >= 0 && j[q] < ci->n_ref, PQG_ERR_ARG,
+                  "determinant index out of range");
+      ii[q] = ref_to_int_host(ci, i[q]);
+      jj[q] = ref_to_int_host(ci, j[q]);
+    }
+    hipStream_t st = ci->stream;
+    DevBuf<int64_t> di(ci->ctx, npairs), dj(ci->ctx, npairs);
+    DevBuf<double> dout(ci->ctx, npairs);
+    PQG_HIP(hipMemcpyAsync(di.get(), ii.data(), npairs * 8, hipMemcpyHostToDevice, st));
+    PQG_HIP(hipMemcpyAsync(dj.get(), jj.data(), npairs * 8, hipMemcpyHostToDevice, st));
+    ElemArgs a{};
+    a.nslot = ci->nslot;
+    a.tb = tables64(ci);
+    a.Hd = ci->hdiag.get();
+    for (int t = 0; t < ci->nslot; t++) {
+      a.sl[t] = ci->slot[t].dev();
+      a.stride[t] = ci->stride[t];
+    }
+    k_elements<<<grid1(npairs, 256), 256, 0, st>>>(a, di.get(), dj.get(), npairs, dout.get());
+    PQG_HIP(hipGetLastError());
+    PQG_HIP(hipMemcpyAsync(out, dout.get(), npairs * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+  });
+}
+
+pqg_status pqg_ci_sigma(pqg_ci* ci, const double* C, int64_t ldc, int k, double* S, int64_t lds, int on_device,
+                        int precision) {
+  return guard([&] {
+    PQG_REQUIRE(ci && C && S && k >= 1, PQG_ERR_ARG, "invalid argument");
+    PQG_REQUIRE(ldc >= ci->n_ref && lds >= ci->n_ref, PQG_ERR_ARG, "leading dimension smaller than N");
+    PQG_REQUIRE(precision == PQG_FP64 || precision == PQG_MIXED, PQG_ERR_ARG, "unknown precision");
+    hipStream_t st = ci->stream;
+    const int64_t N = ci->n_int, Nr = ci->n_ref;
+    DevBuf<double> cin(ci->ctx, (size_t)N * k), sout(ci->ctx, (size_t)N * k);
+    const double* Cd = C;
+    DevBuf<double> cref, sref;
+    if (!on_device) {
+      cref.reset(ci->ctx, (size_t)Nr * k);
+      PQG_HIP(hipMemcpy2DAsync(cref.get(), Nr * 8, C, ldc * 8, Nr * 8, k, hipMemcpyHostToDevice, st));
+      Cd = cref.get();
+      ldc = Nr;
+    }
+    to_internal(ci, Cd, ldc, cin.get(), N, k, st);
+    if (precision == PQG_FP64) {
+      sigma_internal(ci, cin.get(), sout.get(), k, N, st);
+    } else {
+      DevBuf<float> c32(ci->ctx, (size_t)N * k), s32(ci->ctx, (size_t)N * k);
+      k_d2f<<<grid1((int64_t)N * k, 256), 256, 0, st>>>(cin.get(), c32.get(), (int64_t)N * k);
+      sigma_internal(ci, c32.get(), s32.get(), k, N, st);
+      k_f2d<<<grid1((int64_t)N * k, 256), 256, 0, st>>>(s32.get(), sout.get(), (int64_t)N * k);
+      PQG_HIP(hipStreamSynchronize(st));
+    }
+    if (on_device) {
+      to_reference(ci, sout.get(), N, S, lds, k, st);
+    } else {
+      sref.reset(ci->ctx, (size_t)Nr * k);
+      to_reference(ci, sout.get(), N, sref.get(), Nr, k, st);
+      PQG_HIP(hipMemcpy2DAsync(S, lds * 8, sref.get(), Nr * 8, Nr * 8, k, hipMemcpyDeviceToHost, st));
+    }
+    PQG_HIP(hipStreamSynchronize(st));
+  });
+}
+
+pqg_status pqg_ci_dump_strings(pqg_ci* ci, int part, int spin, uint64_t* out) {
+  return guard([&] {
+    PQG_REQUIRE(ci && out && part >= 0 && part < ci->nsp && (spin == 0 || spin == 1), PQG_ERR_ARG, "invalid argument");
+    const Slot& sl = ci->slot[2 * part + spin];
+    std::vector<uint64_t> dev(sl.nstr);
+    PQG_HIP(hipMemcpy(dev.data(), sl.bits.get(), sl.nstr * 8, hipMemcpyDeviceToHost));
+    for (int r = 0; r < sl.nstr; r++) out[r] = dev[sl.ref2int[r]];
+  });
+}
+
+pqg_status pqg_ci_dump_unfold(pqg_ci* ci, int32_t* out) {
+  return guard([&] {
+    PQG_REQUIRE(ci && out, PQG_ERR_ARG, "invalid argument");
+    if (!ci->fci) {
+      std::memcpy(out, ci->host_unfold.data(), ci->host_unfold.size() * 4);
+      return;
+    }
+    for (int64_t a = 0; a < ci->n_ref; a++) {
+      int64_t r = a;
+      int t4[4] = {0, 0, 0, 0};
+      for (int t = ci->nslot - 1; t >= 0; --t) {
+        t4[t] = (int)(r % ci->slot[t].nstr);
+        r /= ci->slot[t].nstr;
+      }
+      for (int t = 0; t < 4; t++) out[a * 4 + t] = t4[t];
+    }
+  });
+}
+
+pqg_status pqg_ci_dump_excitations(pqg_ci* ci, int which, int part, int spin, int64_t* off, int32_t* J, int8_t* phase) {
+  return guard([&] {
+    PQG_REQUIRE(ci && off && (which == 1 || which == 2) && part >= 0 && part < ci->nsp && (spin == 0 || spin == 1),
+                PQG_ERR_ARG, "invalid argument");
+    const Slot& sl = ci->slot[2 * part + spin];
+    const int per = (int)(which == 1 ? sl.nsingle : sl.ndouble);
+    std::vector<int32_t> hj((size_t)sl.nstr * std::max(per, 1));
+    std::vector<int8_t> ho((size_t)sl.nstr * std::max(per, 1));
+    if (per > 0) {
+      DevBuf<int32_t> dj(ci->ctx, (size_t)sl.nstr * per);
+      DevBuf<int8_t> dodd(ci->ctx, (size_t)sl.nstr * per);
+      hipStream_t st = ci->stream;
+      k_dump_excitations<<<grid1(sl.nstr, 128), 128, 0, st>>>(sl.dev(), which, per, dj.get(), dodd.get());
+      PQG_HIP(hipGetLastError());
+      PQG_HIP(hipMemcpyAsync(hj.data(), dj.get(), hj.size() * 4, hipMemcpyDeviceToHost, st));
+      PQG_HIP(hipMemcpyAsync(ho.data(), dodd.get(), ho.size(), hipMemcpyDeviceToHost, st));
+      PQG_HIP(hipStreamSynchronize(st));
+    }
+    // the reference skips doubles when the HF string's word is < 2 (determinant_creation.cpp:306)
+    const bool skip = (which == 2 && sl.ref[0] < 2);
+    int64_t pos = 0;
+    std::vector<std::pair<int32_t, int8_t>> row;
+    for (int r = 0; r < sl.nstr; r++) {
+      off[r] = pos;
+      row.clear();
+      if (!skip) {
+        const int I = sl.ref2int[r];
+        for (int e = 0; e < per; e++) {
+          const int32_t ji = hj[(size_t)I * per + e];
+          if (ji >= 0) row.push_back({sl.int2ref[ji], ho[(size_t)I * per + e]});
+        }
+        std::sort(row.begin(), row.end());
+      }
+      if (J)
+        for (size_t q = 0; q < row.size(); q++) {
+          J[pos + q] = row[q].first;
+          phase[pos + q] = row[q].second ? -1 : 1;
+        }
+      pos += (int64_t)row.size();
+    }
+    off[sl.nstr] = pos;
+  });
+}
+
+}  // extern "C"

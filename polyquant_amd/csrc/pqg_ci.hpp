@@ -1,0 +1,113 @@
+// CI handle: determinant set, integrals and per-string tables resident in HBM.
+#pragma once
+
+#include "pqg_internal.hpp"
+
+namespace pqg {
+
+constexpr int kSlots = 4;  // (electron alpha, electron beta, positron alpha, positron beta)
+
+// Device view of one string set ("unique_dets[part][spin]" in internal order).
+struct SlotDev {
+  const uint64_t* bits;  // [nstr] ascending integer value
+  const uint32_t* rtab;  // rank table [nchunk][256][nocc+1]
+  const int32_t* r2i;    // colex rank -> index in the set, or nullptr when the set is complete
+  int nstr, n_orb, nocc, nvirt, nchunk, npair, nsingle, ndouble;
+};
+
+// Host bookkeeping for one string set.
+struct Slot {
+  int species = 0, spin = 0, spinblk = 0;
+  int n_orb = 1, nocc = 0, nvirt = 0, nstr = 1, nchunk = 1, npair = 1;
+  bool complete = true;
+  int64_t nsingle = 0, ndouble = 0;
+  std::vector<uint64_t> ref;       // reference order (level-major)
+  std::vector<uint64_t> sorted;    // internal order (ascending value)
+  std::vector<int32_t> ref2int;    // reference index -> internal index
+  std::vector<int32_t> int2ref;
+  std::vector<int> deg;            // excitation degree vs the HF string, reference order
+  DevBuf<uint64_t> bits;
+  DevBuf<uint32_t> rtab;
+  DevBuf<int32_t> r2i;
+  DevBuf<int32_t> d_ref2int;
+  DevBuf<double> A, D, E1;         // singles same-slot part, doubles values, diagonal part
+  DevBuf<float> A32, D32;
+  SlotDev dev() const {
+    return SlotDev{bits.get(), rtab.get(), complete ? nullptr : r2i.get(), nstr, n_orb, nocc, nvirt,
+                   nchunk, npair, (int)nsingle, (int)ndouble};
+  }
+};
+
+template <typename T>
+struct Tables {
+  const T* A[kSlots] = {};
+  const T* D[kSlots] = {};
+  const T* X[kSlots][kSlots] = {};  // spectator sums [nstr_u][npair_t], charge factor included
+  const T* V[kSlots][kSlots] = {};  // coupling blocks [npair_t][npair_u], charge factor included
+};
+
+}  // namespace pqg
+
+namespace pqg {
+// Holds one context reference; declared first in pqg_ci so it is released after every
+// device buffer of the handle has been returned to the context's arena.
+struct CtxRef {
+  pqg_ctx* c = nullptr;
+  void set(pqg_ctx* x) {
+    c = x;
+    ctx_retain(x);
+  }
+  ~CtxRef() {
+    if (c) ctx_release(c);
+  }
+};
+}  // namespace pqg
+
+struct pqg_ci {
+  pqg::CtxRef ctx_ref;
+  pqg_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;  // all work of this handle is ordered on it
+  int nsp = 1;
+  int nslot = 2;               // 2 (one species) or 4
+  pqg::Slot slot[pqg::kSlots];
+  double charge[2] = {-1.0, 1.0};
+  int nspin[2] = {1, 1};
+  int64_t n_ref = 0;           // reference determinant count (N_dets)
+  int64_t n_int = 0;           // internal product-space size
+  bool fci = true;             // every product tuple is a determinant
+  int64_t stride[pqg::kSlots] = {1, 1, 1, 1};  // internal address strides
+  int64_t nref_str[pqg::kSlots] = {1, 1, 1, 1};
+  pqg::DevBuf<int64_t> ref_addr;   // truncated spaces: reference address -> internal address
+  pqg::DevBuf<uint8_t> valid;      // truncated spaces: internal address -> in space
+  std::vector<int32_t> host_unfold;  // truncated spaces: reference address -> tuple (N x 4)
+  pqg::DevBuf<double> hdiag;       // internal order; +1e30 outside the space
+  // integrals (device, row-major blocks) and derived tables
+  pqg::DevBuf<double> V64[pqg::kSlots][pqg::kSlots];
+  pqg::DevBuf<float> V32[pqg::kSlots][pqg::kSlots];
+  pqg::DevBuf<double> X64[pqg::kSlots][pqg::kSlots];
+  pqg::DevBuf<float> X32[pqg::kSlots][pqg::kSlots];
+  pqg::DevBuf<double> DW[pqg::kSlots][pqg::kSlots];
+  bool mixed_ready = false;
+  // scratch for sigma (transposed copies), grown on demand
+  pqg::DevBuf<char> scratch;
+  // boundary staging
+  pqg::DevBuf<double> stage_in, stage_out;
+};
+
+namespace pqg {
+
+// Sigma on internal-order device vectors: S = H C for k vectors (leading dimension ld).
+void sigma_internal(pqg_ci* ci, const double* C, double* S, int k, int64_t ld, hipStream_t st);
+void sigma_internal(pqg_ci* ci, const float* C, float* S, int k, int64_t ld, hipStream_t st);
+void ensure_mixed(pqg_ci* ci);
+// Reference-order <-> internal-order vector permutation (device).
+void to_internal(pqg_ci* ci, const double* ref, int64_t ldr, double* in, int64_t ldi, int k, hipStream_t st);
+void to_reference(pqg_ci* ci, const double* in, int64_t ldi, double* ref, int64_t ldr, int k, hipStream_t st);
+// zero entries outside a truncated space (no-op for full CI)
+template <typename T>
+void mask_invalid(pqg_ci* ci, T* v, int64_t ld, int k, hipStream_t st);
+// bytes of device memory a CI of this description needs (admission control)
+uint64_t estimate_ci_bytes(const pqg_ci_desc* d, int nvec_max, int precision);
+pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* desc, hipStream_t stream);
+
+}  // namespace pqg

@@ -1,0 +1,172 @@
+// Context, error reporting and the device-memory arena.
+//
+// The arena is the device memory manager: one context owns an HBM budget, obtains memory
+// from hipMalloc in large chunks (at least 256 MiB, never past the budget) and hands out
+// 256-byte-aligned blocks first-fit, merging neighbours on free.  CI handles, Davidson
+// subspaces and serving slots all allocate from it, so admission control can ask
+// `fits()` before taking a request.
+#include <algorithm>
+#include <cstring>
+
+#include "pqg_internal.hpp"
+
+namespace pqg {
+
+static thread_local std::string t_err;
+void set_error(const std::string& msg) { t_err = msg; }
+
+static constexpr uint64_t kAlign = 256;
+static constexpr uint64_t kMinChunk = 256ull << 20;
+
+static uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+Arena::~Arena() {
+  for (auto& c : chunks_) (void)hipFree(c.base);
+}
+
+bool Arena::fits(uint64_t bytes) const {
+  std::lock_guard<std::mutex> lk(mu_);
+  bytes = round_up(std::max<uint64_t>(bytes, 1), kAlign);
+  for (const auto& c : chunks_)
+    for (const auto& [off, sz] : c.free_)
+      if (sz >= bytes) return true;
+  return reserved_ + std::max(bytes, std::min(kMinChunk, budget_ - std::min(budget_, reserved_))) <= budget_ ||
+         reserved_ + bytes <= budget_;
+}
+
+void* Arena::alloc(uint64_t bytes) {
+  std::lock_guard<std::mutex> lk(mu_);
+  bytes = round_up(std::max<uint64_t>(bytes, 1), kAlign);
+  for (size_t ci = 0; ci < chunks_.size(); ci++) {
+    auto& c = chunks_[ci];
+    for (auto it = c.free_.begin(); it != c.free_.end(); ++it) {
+      if (it->second >= bytes) {
+        uint64_t off = it->first, sz = it->second;
+        c.free_.erase(it);
+        if (sz > bytes) c.free_[off + bytes] = sz - bytes;
+        char* p = c.base + off;
+        live_[p] = {(int)ci, bytes};
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        return p;
+      }
+    }
+  }
+  // new chunk
+  if (reserved_ + bytes > budget_)
+    throw Error{PQG_ERR_OOM, "HBM budget exceeded: need " + std::to_string(bytes) + " B, " +
+                                 std::to_string(budget_ - std::min(budget_, used_)) + " B free of budget"};
+  uint64_t csize = std::max(bytes, kMinChunk);
+  csize = std::min(csize, budget_ - reserved_);
+  csize = std::max(csize, bytes);
+  char* base = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), csize);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    if (csize > bytes) {  // retry with the exact size
+      csize = bytes;
+      e = hipMalloc(reinterpret_cast<void**>(&base), csize);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error{PQG_ERR_OOM, std::string("hipMalloc failed: ") + hipGetErrorString(e)};
+    }
+  }
+  reserved_ += csize;
+  Chunk c{base, csize, {}};
+  if (csize > bytes) c.free_[bytes] = csize - bytes;
+  chunks_.push_back(std::move(c));
+  live_[base] = {(int)chunks_.size() - 1, bytes};
+  used_ += bytes;
+  peak_ = std::max(peak_, used_);
+  return base;
+}
+
+void Arena::free(void* vp) {
+  if (!vp) return;
+  std::lock_guard<std::mutex> lk(mu_);
+  char* p = static_cast<char*>(vp);
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  auto [ci, sz] = it->second;
+  live_.erase(it);
+  used_ -= sz;
+  auto& c = chunks_[ci];
+  uint64_t off = p - c.base;
+  auto nx = c.free_.lower_bound(off);
+  if (nx != c.free_.end() && nx->first == off + sz) {  // merge right
+    sz += nx->second;
+    nx = c.free_.erase(nx);
+  }
+  if (nx != c.free_.begin()) {  // merge left
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == off) {
+      pv->second += sz;
+      return;
+    }
+  }
+  c.free_[off] = sz;
+}
+
+void ctx_retain(pqg_ctx* c) { c->refs.fetch_add(1); }
+
+void ctx_release(pqg_ctx* c) {
+  if (c->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamDestroy(c->stream);
+  c->arena.reset();
+  delete c;
+}
+
+}  // namespace pqg
+
+extern "C" {
+
+const char* pqg_last_error(void) { return pqg::t_err.c_str(); }
+
+pqg_status pqg_ctx_create(int device, uint64_t hbm_budget, pqg_ctx** out) {
+  return pqg::guard([&] {
+    PQG_REQUIRE(out != nullptr, PQG_ERR_ARG, "out is NULL");
+    int n = 0;
+    PQG_HIP(hipGetDeviceCount(&n));
+    PQG_REQUIRE(device >= 0 && device < n, PQG_ERR_ARG, "no such device");
+    PQG_HIP(hipSetDevice(device));
+    auto* c = new pqg_ctx();
+    c->device = device;
+    size_t free_b = 0, total_b = 0;
+    PQG_HIP(hipMemGetInfo(&free_b, &total_b));
+    if (hbm_budget == 0 || hbm_budget > free_b) hbm_budget = (uint64_t)(free_b * 0.95);
+    c->arena = std::make_unique<pqg::Arena>(device, hbm_budget);
+    PQG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    PQG_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cu = prop.multiProcessorCount;
+    *out = c;
+  });
+}
+
+pqg_status pqg_ctx_destroy(pqg_ctx* ctx) {
+  return pqg::guard([&] {
+    if (!ctx) return;
+    pqg::ctx_release(ctx);
+  });
+}
+
+pqg_status pqg_ctx_memory(pqg_ctx* ctx, uint64_t* used, uint64_t* budget, uint64_t* peak) {
+  return pqg::guard([&] {
+    PQG_REQUIRE(ctx, PQG_ERR_ARG, "ctx is NULL");
+    if (used) *used = ctx->arena->used();
+    if (budget) *budget = ctx->arena->budget();
+    if (peak) *peak = ctx->arena->peak();
+  });
+}
+
+pqg_status pqg_ctx_sync(pqg_ctx* ctx) {
+  return pqg::guard([&] {
+    PQG_REQUIRE(ctx, PQG_ERR_ARG, "ctx is NULL");
+    PQG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// Davidson eigensolver resident on the GPU, as a step-wise state machine so that a scheduler
+// can interleave many solves and fuse their sigma builds.
+#pragma once
+
+#include <chrono>
+
+#include "pqg_ci.hpp"
+
+namespace pqg {
+
+// A block of vectors whose sigma the solver needs next.
+struct SigmaWork {
+  int k = 0;               // number of vectors (0: nothing pending)
+  int precision = PQG_FP64;
+  const double* C = nullptr;  // internal order, leading dimension ld
+  double* S = nullptr;
+  int64_t ld = 0;
+};
+
+class Davidson {
+ public:
+  Davidson(pqg_ci* ci, const pqg_dav_params& p);
+  // Work the solver needs before post_sigma() can run (k == 0 when finished).
+  SigmaWork pending() const { return work_; }
+  // Runs the rest of one iteration after the pending sigma has been written.
+  void post_sigma(hipStream_t st);
+  // Run the pending sigma itself (FP32 conversion included for mixed precision).
+  void run_sigma(hipStream_t st);
+  bool done() const { return done_; }
+  pqg_status status() const { return status_; }
+  int nev() const { return nev_; }
+  const std::vector<double>& energies() const { return theta_h_; }
+  const std::vector<double>& residuals() const { return rn_h_; }
+  const double* ritz_vectors() const { return X_.get(); }  // internal order, N x nev
+  pqg_dav_report report() const;
+  uint64_t device_bytes() const;
+
+ private:
+  void initial_guess(hipStream_t st);
+  void add_corrections(hipStream_t st, const double* R);
+  int orthonormalize_append(hipStream_t st, double* T, int nt);
+  void rayleigh_ritz(hipStream_t st);
+  void restart(hipStream_t st, const double* Vk, const double* Wk, int keep);
+  void ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, const double* B, int64_t ldb, int nb,
+                  double* out, int ldo);
+  void ts_gemm_nn(hipStream_t st, const double* A, int64_t lda, int ka, const double* Bs, int ldb, int nb,
+                  double* out, int64_t ldo, double alpha, double beta);
+
+  pqg_ci* ci_;
+  pqg_dav_params p_;
+  int64_t N_;
+  int nev_, ninit_, nmax_;
+  bool mixed_;
+  DevBuf<double> V_, W_, G_, Y_, theta_, X_, AX_, R_, T_, tmp_, part_, small_, W64_;
+  DevBuf<float> c32_, s32_;
+  int m_ = 0, nW_ = 0, iter_ = 0, restarts_ = 0, since_check_ = 0;
+  int sigma_calls_ = 0, sigma_calls64_ = 0;
+  bool done_ = false, checking_ = false;
+  pqg_status status_ = PQG_OK;
+  double last_true_ = 1e300;
+  std::vector<double> theta_h_, rn_h_;
+  SigmaWork work_;
+  std::chrono::steady_clock::time_point t0_;
+  double seconds_ = 0.0;
+  int nchunk_ = 1;
+};
+
+}  // namespace pqg

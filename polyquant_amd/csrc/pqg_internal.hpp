@@ -1,0 +1,139 @@
+// Internal declarations shared by the pqgpu translation units (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pqgpu.h"
+
+namespace pqg {
+
+// ---------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+struct Error {
+  pqg_status code;
+  std::string msg;
+};
+
+#define PQG_HIP(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      throw ::pqg::Error{PQG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)}; \
+  } while (0)
+
+#define PQG_REQUIRE(cond, code, msg)                     \
+  do {                                                   \
+    if (!(cond)) throw ::pqg::Error{(code), (msg)};      \
+  } while (0)
+
+// Runs f, converting exceptions into a status + thread-local message.
+template <class F>
+pqg_status guard(F&& f) {
+  try {
+    f();
+    return PQG_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return PQG_ERR_OOM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return PQG_ERR_ARG;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// device memory manager: first-fit arena inside an HBM budget
+// ---------------------------------------------------------------------------------------
+class Arena {
+ public:
+  Arena(int device, uint64_t budget) : device_(device), budget_(budget) {}
+  ~Arena();
+  void* alloc(uint64_t bytes);  // throws PQG_ERR_OOM past the budget
+  void free(void* p);
+  bool fits(uint64_t bytes) const;  // would an allocation of this size fit right now?
+  uint64_t used() const { return used_; }
+  uint64_t peak() const { return peak_; }
+  uint64_t budget() const { return budget_; }
+
+ private:
+  struct Chunk {
+    char* base;
+    uint64_t size;
+    std::map<uint64_t, uint64_t> free_;  // offset -> size
+  };
+  int device_;
+  uint64_t budget_;
+  uint64_t reserved_ = 0;  // bytes obtained from hipMalloc
+  uint64_t used_ = 0, peak_ = 0;
+  std::vector<Chunk> chunks_;
+  std::map<char*, std::pair<int, uint64_t>> live_;  // ptr -> (chunk, size)
+  mutable std::mutex mu_;
+};
+
+}  // namespace pqg
+
+struct pqg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<pqg::Arena> arena;
+  int num_cu = 0;
+  // The caller's handle plus every live CI handle hold a reference; the context (and its
+  // arena) is torn down when the last one is released.
+  std::atomic<int> refs{1};
+};
+
+namespace pqg {
+void ctx_retain(pqg_ctx* c);
+void ctx_release(pqg_ctx* c);
+}  // namespace pqg
+
+namespace pqg {
+
+// RAII device buffer from the context arena
+template <class T>
+struct DevBuf {
+  pqg_ctx* ctx = nullptr;
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(pqg_ctx* c, size_t count) { reset(c, count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    release();
+    ctx = o.ctx; p = o.p; n = o.n;
+    o.p = nullptr; o.n = 0;
+    return *this;
+  }
+  void reset(pqg_ctx* c, size_t count) {
+    release();
+    ctx = c;
+    n = count;
+    p = count ? static_cast<T*>(c->arena->alloc(count * sizeof(T))) : nullptr;
+  }
+  void release() {
+    if (p && ctx) ctx->arena->free(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+  T* get() const { return p; }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+}  // namespace pqg

@@ -1,0 +1,49 @@
+"""Host-side workload generation: determinism, sizes, integral symmetries."""
+import math
+
+import numpy as np
+
+from polyquant_amd import workloads as wl
+
+
+def test_splitmix_known_values():
+    # SplitMix64 with seed 0: first outputs (Steele, Lea & Flood 2014 reference stream)
+    r = wl.SplitMix64(0).raw(3)
+    assert [int(x) for x in r] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_generator_is_deterministic():
+    a = wl.config2()
+    b = wl.config2()
+    for k in a.eri:
+        assert np.array_equal(a.eri[k], b.eri[k])
+    assert np.array_equal(a.h[0][0], b.h[0][0])
+
+
+def test_config_sizes():
+    assert wl.config2().n_dets_fci() == 48 * 48 * 48
+    assert wl.config3().n_dets_fci() == 12870 ** 2 == 165_636_900
+    assert math.comb(13, 5) ** 2 == 1_656_369
+
+
+def test_integral_symmetries():
+    p = wl.small_problem(5, 6, 2, 2, positron=(5, 1, 0))
+    v = p.eri[(0, 0, 0, 0)]
+    assert np.allclose(v, v.T)
+    h = p.h[0][0]
+    assert np.allclose(h, h.T)
+    # Cholesky-built two-body matrices are positive semidefinite
+    assert np.linalg.eigvalsh(v).min() > -1e-10
+    assert p.eri[(0, 0, 1, 0)].shape == (21, 15)
+
+
+def test_serving_mix_draw_rules():
+    reqs = wl.serving_requests(64)
+    ts = [t for t, _ in reqs]
+    assert all(b > a for a, b in zip(ts, ts[1:]))
+    for _, pr in reqs:
+        sp = pr.species[0]
+        assert 6 <= sp.n_orb <= 14 and 2 <= sp.n_alpha <= sp.n_orb // 2 and 2 <= sp.n_beta <= sp.n_orb // 2
+        assert 1 <= pr.nroots <= 4
+        if pr.nspecies == 2:
+            assert 6 <= pr.species[1].n_orb <= 12
