@@ -86,6 +86,13 @@ pqg_status pqg_ctx_destroy(pqg_ctx* ctx);
 pqg_status pqg_ctx_memory(pqg_ctx* ctx, uint64_t* used, uint64_t* budget, uint64_t* peak);
 pqg_status pqg_ctx_sync(pqg_ctx* ctx);
 const char* pqg_last_error(void);
+/* Kernel timing with HIP events recorded on the stream each kernel is launched on (the role of
+ * POLYQUANT_TIMER, src/io/timer.hpp:15-154, for device work).  enable = 1 starts recording,
+ * 0 stops.  pqg_ctx_profile_read sums the launches recorded under `name` since the last read
+ * of that name: launch count, total milliseconds and total work units (vectors for sigma). */
+pqg_status pqg_ctx_profile(pqg_ctx* ctx, int enable);
+pqg_status pqg_ctx_profile_read(pqg_ctx* ctx, const char* name, int64_t* launches, double* total_ms,
+                                int64_t* total_units);
 
 /* --- determinant set (replaces POLYQUANT_DETSET<uint64_t>) ----------------------------- */
 /* Builds strings in the reference order (determinant_creation.cpp:6-207, epci.cpp:334-388),

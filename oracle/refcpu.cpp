@@ -606,7 +606,7 @@ struct Scatter {
 };
 
 // sigma_one_species_class_singleshot (direct_singleparticle.cpp:239-372)
-void sigma_one(const DetSet& d, const double* C, int64_t ldc, int k, double* out, int64_t lds) {
+void sigma_one(const DetSet& d, const double* C, int64_t ldc, int k, double* out, int64_t lds, int64_t stride = 1) {
   const int64_t N = d.N;
   int nth = omp_get_max_threads();
   std::vector<std::vector<double>> priv(nth);
@@ -616,7 +616,8 @@ void sigma_one(const DetSet& d, const double* C, int64_t ldc, int k, double* out
     int nt = omp_get_num_threads();
     priv[tid].assign((size_t)N * k, 0.0);
     Scatter sc{C, ldc, k, priv[tid].data(), N};
-    for (int64_t i = tid; i < N; i += nt) {
+    for (int64_t q = tid; q * stride < N; q += nt) {
+      const int64_t i = q * stride;
       const int32_t* I = d.unfold[i].data();
       const int32_t IA = I[0], IB = I[1];
       for (int s = 0; s < k; s++) priv[tid][i + s * N] += d.hdiag[i] * C[i + s * ldc];
@@ -661,7 +662,7 @@ void sigma_one(const DetSet& d, const double* C, int64_t ldc, int k, double* out
 }
 
 // sigma_two_species_class_singleshot (direct_mixedparticle.cpp:270-611)
-void sigma_two(const DetSet& d, const double* C, int64_t ldc, int k, double* out, int64_t lds) {
+void sigma_two(const DetSet& d, const double* C, int64_t ldc, int k, double* out, int64_t lds, int64_t stride = 1) {
   const int64_t N = d.N;
   const double cf = d.charge[0] * d.charge[1];
   int nth = omp_get_max_threads();
@@ -672,7 +673,8 @@ void sigma_two(const DetSet& d, const double* C, int64_t ldc, int k, double* out
     int nt = omp_get_num_threads();
     priv[tid].assign((size_t)N * k, 0.0);
     Scatter sc{C, ldc, k, priv[tid].data(), N};
-    for (int64_t i = tid; i < N; i += nt) {
+    for (int64_t q = tid; q * stride < N; q += nt) {
+      const int64_t i = q * stride;
       const int32_t* I = d.unfold[i].data();
       const int32_t A = I[0], B = I[1], Cc = I[2], D = I[3];
       for (int s = 0; s < k; s++) priv[tid][i + s * N] += d.hdiag[i] * C[i + s * ldc];
@@ -884,6 +886,18 @@ int rc_sigma(void* h, const double* C, int64_t ldc, int k, double* S, int64_t ld
     sigma_one(*d, C, ldc, k, S, lds);
   else
     sigma_two(*d, C, ldc, k, S, lds);
+  return 0;
+}
+
+// The singleshot sigma restricted to rows i_det = 0, stride, 2*stride, ... of the
+// symmetric-scatter loop: a representative sample for timing the CPU path.
+int rc_sigma_sample(void* h, const double* C, int64_t ldc, int k, double* S, int64_t lds, int64_t stride) {
+  auto* d = static_cast<DetSet*>(h);
+  if (stride < 1) stride = 1;
+  if (d->nsp == 1)
+    sigma_one(*d, C, ldc, k, S, lds, stride);
+  else
+    sigma_two(*d, C, ldc, k, S, lds, stride);
   return 0;
 }
 

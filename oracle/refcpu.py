@@ -60,6 +60,7 @@ def lib():
         L.rc_slater_condon.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int64]
         L.rc_sigma.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64]
         L.rc_sigma_slow.argtypes = L.rc_sigma.argtypes
+        L.rc_sigma_sample.argtypes = L.rc_sigma.argtypes + [ct.c_int64]
         L.rc_davidson.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double,
                                   ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.rc_get_holes.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p]
@@ -205,6 +206,12 @@ class OracleCI:
         C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)
         S = np.zeros_like(C, order="F")
         lib().rc_sigma(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(S), S.shape[0])
+        return S
+
+    def sigma_sample(self, C, stride):
+        C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)
+        S = np.zeros_like(C, order="F")
+        lib().rc_sigma_sample(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(S), S.shape[0], int(stride))
         return S
 
     def sigma_slow(self, C):

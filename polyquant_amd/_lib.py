@@ -77,6 +77,9 @@ SIGNATURES = [
     ("pqg_ctx_memory", ct.c_int, [_vp, ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_uint64)]),
     ("pqg_ctx_sync", ct.c_int, [_vp]),
     ("pqg_last_error", ct.c_char_p, []),
+    ("pqg_ctx_profile", ct.c_int, [_vp, ct.c_int]),
+    ("pqg_ctx_profile_read", ct.c_int, [_vp, ct.c_char_p, ct.POINTER(ct.c_int64), ct.POINTER(ct.c_double),
+                                        ct.POINTER(ct.c_int64)]),
     ("pqg_ci_create", ct.c_int, [_vp, ct.POINTER(CiDesc), ct.POINTER(_vp)]),
     ("pqg_ci_destroy", ct.c_int, [_vp]),
     ("pqg_ci_ndets", ct.c_int64, [_vp]),

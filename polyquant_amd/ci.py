@@ -55,6 +55,15 @@ class Context:
         check(load().pqg_ctx_memory(self.h, ct.byref(used), ct.byref(budget), ct.byref(peak)))
         return used.value, budget.value, peak.value
 
+    def profile(self, enable: bool = True):
+        check(load().pqg_ctx_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self, name: str):
+        """(launches, total_ms, total_units) recorded for kernel `name` since the last read."""
+        n, ms, u = ct.c_int64(), ct.c_double(), ct.c_int64()
+        check(load().pqg_ctx_profile_read(self.h, name.encode(), ct.byref(n), ct.byref(ms), ct.byref(u)))
+        return n.value, ms.value, u.value
+
     def close(self):
         if self.h:
             load().pqg_ctx_destroy(self.h)

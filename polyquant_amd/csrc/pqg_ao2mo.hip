@@ -207,9 +207,12 @@ __global__ void k_cvt_d2f(const double* __restrict__ x, float* __restrict__ y, i
 
 static int gr(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
+static thread_local pqg_ctx* g_prof_ctx = nullptr;  // context of the running transform
+
 template <typename T>
 static void bgemm(const T* A, const T* Cm, T* out, int n, int nb, bool transA, hipStream_t st) {
   dim3 g(gr(n, BN), gr(n, BM), nb);
+  ProfScope prof(g_prof_ctx, st, std::is_same<T, double>::value ? "ao2mo_gemm_f64" : "ao2mo_gemm_bf16x3", nb);
   if constexpr (std::is_same<T, double>::value) {
     if (transA)
       k_bgemm_f64<1><<<g, 256, 0, st>>>(A, Cm, out, n);
@@ -304,6 +307,7 @@ extern "C" pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* er
     PQG_HIP(hipEventCreate(&t0));
     PQG_HIP(hipEventCreate(&t1));
     PQG_HIP(hipEventRecord(t0, st));
+    g_prof_ctx = ctx;
     if (mode == PQG_AO2MO_FP64)
       ao2mo_impl<double>(ctx, n, de8, dC, dout, st);
     else

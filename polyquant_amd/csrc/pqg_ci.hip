@@ -43,6 +43,7 @@ __device__ __forceinline__ uint64_t low_mask(int n) { return n >= 64 ? ~0ull : (
 __device__ __forceinline__ uint64_t between_mask(int lo, int hi) { return low_mask(hi) & ~low_mask(lo + 1); }
 
 __device__ __forceinline__ int str_index(const SlotDev& s, const uint32_t* tab, uint64_t J) {
+  if (s.nocc == 1 && !s.r2i) return __builtin_ctzll(J);  // colex rank of {p} is p
   uint32_t r = 0;
   int e = s.nocc;
   const int w = s.nocc + 1;
@@ -252,7 +253,124 @@ struct PassArgs {
 template <int NC>
 constexpr int pass_threads() { return NC == 1 ? 1024 : 256; }
 
+// Column strings of column index m.
 template <typename T, int NC>
+__device__ __forceinline__ void decode_cols(const PassArgs<T>& a, int64_t m, int (&ci)[NC], uint64_t (&cb)[NC]) {
+  int64_t r = m;
+#pragma unroll
+  for (int c = NC - 1; c >= 0; --c) {
+    ci[c] = (int)(r % a.cn[c]);
+    r /= a.cn[c];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; c++) cb[c] = a.sl[a.cs[c]].bits[ci[c]];
+}
+
+// Diagonal and every excitation that changes only column strings, gathered from the staged
+// row C(row, :).
+template <typename T, int NC>
+__device__ __forceinline__ T column_terms(const PassArgs<T>& a, int64_t row, int64_t m, const T* crow, const T* xs,
+                                          const uint32_t* rt) {
+  T sum = 0;
+  if (a.flags & F_DIAG) sum = (T)a.Hd[row * a.M + m] * crow[m];
+  if (!(a.flags & F_COLS)) return sum;
+  int ci[NC];
+  uint64_t cb[NC];
+  decode_cols<T, NC>(a, m, ci, cb);
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int t = a.cs[c];
+    const SlotDev& s = a.sl[t];
+    if (s.nocc == 0 || s.nvirt == 0) continue;
+    const uint32_t* tab = rt + a.rt_off[c];
+    const T* __restrict__ At = a.tb.A[t];
+    const T* xrow = xs + a.x_off[c];
+    const int it = ci[c];
+    const int64_t cst = a.cstride[c];
+    // singles of column string c: s * (A + spectator sums) * C(row, m')
+    for_singles(cb[c], s.n_orb, [&](int e, int, int, int Q, int odd, uint64_t J) {
+      const int jt = str_index(s, tab, J);
+      if (jt < 0) return;
+      T v = At[(size_t)e * s.nstr + it] + xrow[Q];
+#pragma unroll
+      for (int c2 = 0; c2 < NC; c2++) {
+        if (c2 == c) continue;
+        const T* X = a.tb.X[t][a.cs[c2]];
+        if (X) v += X[(size_t)ci[c2] * s.npair + Q];
+      }
+      const T cv = crow[m + (int64_t)(jt - it) * cst];
+      sum += odd ? -v * cv : v * cv;
+    });
+    // same-string doubles
+    if (s.ndouble > 0) {
+      const T* __restrict__ Dt = a.tb.D[t];
+      for_doubles(cb[c], s.n_orb, [&](int e, int, int, int, int, uint64_t J) {
+        const int jt = str_index(s, tab, J);
+        if (jt < 0) return;
+        sum += Dt[(size_t)e * s.nstr + it] * crow[m + (int64_t)(jt - it) * cst];
+      });
+    }
+    // one single in each of two column strings
+#pragma unroll
+    for (int c2 = c + 1; c2 < NC; c2++) {
+      const int u = a.cs[c2];
+      const SlotDev& s2 = a.sl[u];
+      const T* Vtu = a.tb.V[t][u];
+      if (!Vtu || s2.nocc == 0 || s2.nvirt == 0) continue;
+      const uint32_t* tab2 = rt + a.rt_off[c2];
+      const int iu = ci[c2];
+      const int64_t cst2 = a.cstride[c2];
+      const uint64_t cb2 = cb[c2];
+      for_singles(cb[c], s.n_orb, [&](int, int, int, int Q, int odd, uint64_t J) {
+        const int jt = str_index(s, tab, J);
+        if (jt < 0) return;
+        const int64_t mj = m + (int64_t)(jt - it) * cst;
+        const T* vrowQ = Vtu + (size_t)Q * s2.npair;
+        T part = 0;
+        for_singles(cb2, s2.n_orb, [&](int, int, int, int Q2, int odd2, uint64_t J2) {
+          const int ju = str_index(s2, tab2, J2);
+          if (ju < 0) return;
+          const T cv = crow[mj + (int64_t)(ju - iu) * cst2];
+          const T v = vrowQ[Q2];
+          part += odd2 ? -v * cv : v * cv;
+        });
+        sum += odd ? -part : part;
+      });
+    }
+  }
+  return sum;
+}
+
+// For one single excitation of the row string (its coupling row staged in vr and the excited
+// row C(J, :) in crow): sum over single excitations of the column strings.
+template <typename T, int NC>
+__device__ __forceinline__ T mixed_terms(const PassArgs<T>& a, int t0, int64_t m, const T* crow, const T* vr,
+                                         const uint32_t* rt) {
+  int ci[NC];
+  uint64_t cb[NC];
+  decode_cols<T, NC>(a, m, ci, cb);
+  T part = 0;
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    const int t = a.cs[c];
+    const SlotDev& s = a.sl[t];
+    if (s.nocc == 0 || s.nvirt == 0 || !a.tb.V[t0][t]) continue;
+    const uint32_t* tab = rt + a.rt_off[c];
+    const T* vrow = vr + a.v_off[c];
+    const int it = ci[c];
+    const int64_t cst = a.cstride[c];
+    for_singles(cb[c], s.n_orb, [&](int, int, int, int Q, int odd, uint64_t J) {
+      const int jt = str_index(s, tab, J);
+      if (jt < 0) return;
+      const T cv = crow[m + (int64_t)(jt - it) * cst];
+      const T v = vrow[Q];
+      part += odd ? -v * cv : v * cv;
+    });
+  }
+  return part;
+}
+
+template <typename T, int NC, int CPT>
 __global__ __launch_bounds__(pass_threads<NC>()) void k_sigma_pass(const PassArgs<T> a) {
   extern __shared__ __align__(16) char smem[];
   T* crow = reinterpret_cast<T*>(smem);
@@ -268,16 +386,14 @@ __global__ __launch_bounds__(pass_threads<NC>()) void k_sigma_pass(const PassArg
   const int64_t M = a.M;
 
   int ri[3];
-  uint64_t rb[3];
   {
     int64_t r = row;
     for (int q = a.nr - 1; q >= 0; --q) {
       ri[q] = (int)(r % a.rn[q]);
       r /= a.rn[q];
     }
-    for (int q = 0; q < a.nr; q++) rb[q] = a.sl[a.rs[q]].bits[ri[q]];
   }
-  // rank tables and spectator sums for the column slots
+  // rank tables and spectator sums for the column strings
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const SlotDev& s = a.sl[a.cs[c]];
@@ -296,104 +412,20 @@ __global__ __launch_bounds__(pass_threads<NC>()) void k_sigma_pass(const PassArg
   for (int64_t m = tid; m < M; m += nth) crow[m] = Cr[row * M + m];
   __syncthreads();
 
-  T acc[kCPT];
-#pragma unroll
-  for (int j = 0; j < kCPT; j++) acc[j] = 0;
-
   const int64_t mbase = (int64_t)tile * a.tile_cols + tid;
-
-  // ---- diagonal + column-side excitations -------------------------------------------
+  const int64_t mend = min(M, (int64_t)(tile + 1) * a.tile_cols);
+  T acc[CPT];
 #pragma unroll
-  for (int j = 0; j < kCPT; j++) {
-    if (j >= a.cpt) break;
+  for (int j = 0; j < CPT; j++) {
     const int64_t m = mbase + (int64_t)j * nth;
-    if (m >= M || m >= (int64_t)(tile + 1) * a.tile_cols) continue;
-    T sum = 0;
-    if (a.flags & F_DIAG) sum = (T)a.Hd[row * M + m] * crow[m];
-    if (a.flags & F_COLS) {
-      int ci[NC];
-      uint64_t cb[NC];
-      {
-        int64_t r = m;
-#pragma unroll
-        for (int c = NC - 1; c >= 0; --c) {
-          ci[c] = (int)(r % a.cn[c]);
-          r /= a.cn[c];
-        }
-#pragma unroll
-        for (int c = 0; c < NC; c++) cb[c] = a.sl[a.cs[c]].bits[ci[c]];
-      }
-#pragma unroll
-      for (int c = 0; c < NC; c++) {
-        const int t = a.cs[c];
-        const SlotDev& s = a.sl[t];
-        if (s.nocc == 0 || s.nvirt == 0) continue;
-        const uint32_t* tab = rt + a.rt_off[c];
-        const T* __restrict__ At = a.tb.A[t];
-        const T* xrow = xs + a.x_off[c];
-        const int it = ci[c];
-        const int64_t cst = a.cstride[c];
-        // singles of column string c
-        for_singles(cb[c], s.n_orb, [&](int e, int, int, int Q, int odd, uint64_t J) {
-          const int jt = str_index(s, tab, J);
-          if (jt < 0) return;
-          T v = At[(size_t)e * s.nstr + it] + xrow[Q];
-#pragma unroll
-          for (int c2 = 0; c2 < NC; c2++) {
-            if (c2 == c) continue;
-            const T* X = a.tb.X[t][a.cs[c2]];
-            if (X) v += X[(size_t)ci[c2] * s.npair + Q];
-          }
-          const T cv = crow[m + (int64_t)(jt - it) * cst];
-          sum += odd ? -v * cv : v * cv;
-        });
-        // same-string doubles
-        if (s.ndouble > 0) {
-          const T* __restrict__ Dt = a.tb.D[t];
-          for_doubles(cb[c], s.n_orb, [&](int e, int, int, int, int, uint64_t J) {
-            const int jt = str_index(s, tab, J);
-            if (jt < 0) return;
-            sum += Dt[(size_t)e * s.nstr + it] * crow[m + (int64_t)(jt - it) * cst];
-          });
-        }
-        // doubles across two column strings
-#pragma unroll
-        for (int c2 = c + 1; c2 < NC; c2++) {
-          const int u = a.cs[c2];
-          const SlotDev& s2 = a.sl[u];
-          const T* Vtu = a.tb.V[t][u];
-          if (!Vtu || s2.nocc == 0 || s2.nvirt == 0) continue;
-          const uint32_t* tab2 = rt + a.rt_off[c2];
-          const int iu = ci[c2];
-          const int64_t cst2 = a.cstride[c2];
-          for_singles(cb[c], s.n_orb, [&](int, int, int, int Q, int odd, uint64_t J) {
-            const int jt = str_index(s, tab, J);
-            if (jt < 0) return;
-            const int64_t mj = m + (int64_t)(jt - it) * cst;
-            const T* vrowQ = Vtu + (size_t)Q * s2.npair;
-            T part = 0;
-            for_singles(cb[c2], s2.n_orb, [&](int, int, int, int Q2, int odd2, uint64_t J2) {
-              const int ju = str_index(s2, tab2, J2);
-              if (ju < 0) return;
-              const T cv = crow[mj + (int64_t)(ju - iu) * cst2];
-              const T v = vrowQ[Q2];
-              part += odd2 ? -v * cv : v * cv;
-            });
-            sum += odd ? -part : part;
-          });
-        }
-      }
-    }
-    acc[j] = sum;
+    acc[j] = (m < mend) ? column_terms<T, NC>(a, row, m, crow, xs, rt) : (T)0;
   }
 
-  // ---- row single x column single doubles ---------------------------------------------
   if ((a.flags & F_MIXED) && a.nr == 1) {
     const int t0 = a.rs[0];
     const SlotDev& s0 = a.sl[t0];
-    const uint64_t I0 = rb[0];
     const int i0 = ri[0];
-    // the row slot's rank table lives in global memory (uniform lookups)
+    const uint64_t I0 = s0.bits[i0];
     for_singles(I0, s0.n_orb, [&](int, int, int, int Q0, int odd0, uint64_t J0) {
       const int j0 = str_index(s0, s0.rtab, J0);
       if (j0 < 0) return;
@@ -402,63 +434,32 @@ __global__ __launch_bounds__(pass_threads<NC>()) void k_sigma_pass(const PassArg
       for (int64_t m = tid; m < M; m += nth) crow[m] = Cr[rowJ * M + m];
 #pragma unroll
       for (int c = 0; c < NC; c++) {
-        const int t = a.cs[c];
-        const T* V0t = a.tb.V[t0][t];
-        const int np = a.sl[t].npair;
+        const T* V0t = a.tb.V[t0][a.cs[c]];
+        const int np = a.sl[a.cs[c]].npair;
         for (int P = tid; P < np; P += nth) vr[a.v_off[c] + P] = V0t ? V0t[(size_t)Q0 * np + P] : (T)0;
       }
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < kCPT; j++) {
-        if (j >= a.cpt) break;
+      for (int j = 0; j < CPT; j++) {
         const int64_t m = mbase + (int64_t)j * nth;
-        if (m >= M || m >= (int64_t)(tile + 1) * a.tile_cols) continue;
-        int ci[NC];
-        uint64_t cb[NC];
-        {
-          int64_t r = m;
-#pragma unroll
-          for (int c = NC - 1; c >= 0; --c) {
-            ci[c] = (int)(r % a.cn[c]);
-            r /= a.cn[c];
-          }
-#pragma unroll
-          for (int c = 0; c < NC; c++) cb[c] = a.sl[a.cs[c]].bits[ci[c]];
+        if (m < mend) {
+          const T part = mixed_terms<T, NC>(a, t0, m, crow, vr, rt);
+          acc[j] += odd0 ? -part : part;
         }
-        T part = 0;
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
-          const int t = a.cs[c];
-          const SlotDev& s = a.sl[t];
-          if (s.nocc == 0 || s.nvirt == 0 || !a.tb.V[t0][t]) continue;
-          const uint32_t* tab = rt + a.rt_off[c];
-          const T* vrow = vr + a.v_off[c];
-          const int it = ci[c];
-          const int64_t cst = a.cstride[c];
-          for_singles(cb[c], s.n_orb, [&](int, int, int, int Q, int odd, uint64_t J) {
-            const int jt = str_index(s, tab, J);
-            if (jt < 0) return;
-            const T cv = crow[m + (int64_t)(jt - it) * cst];
-            const T v = vrow[Q];
-            part += odd ? -v * cv : v * cv;
-          });
-        }
-        acc[j] += odd0 ? -part : part;
       }
     });
   }
 
-  // ---- write ---------------------------------------------------------------------------
   T* __restrict__ Sr = a.S + (int64_t)root * a.lds;
 #pragma unroll
-  for (int j = 0; j < kCPT; j++) {
-    if (j >= a.cpt) break;
+  for (int j = 0; j < CPT; j++) {
     const int64_t m = mbase + (int64_t)j * nth;
-    if (m >= M || m >= (int64_t)(tile + 1) * a.tile_cols) continue;
-    if (a.flags & F_ACCUM)
-      Sr[row * M + m] += acc[j];
-    else
-      Sr[row * M + m] = acc[j];
+    if (m < mend) {
+      if (a.flags & F_ACCUM)
+        Sr[row * M + m] += acc[j];
+      else
+        Sr[row * M + m] = acc[j];
+    }
   }
 }
 
@@ -1053,7 +1054,11 @@ static void launch_pass(pqg_ci* ci, const Tables<T>& tb, const std::vector<int>&
   // block: about 4 columns per thread for small rows, up to 1024 threads
   const int max_threads = a.nc == 1 ? pass_threads<1>() : pass_threads<3>();
   int nth = (int)std::min<int64_t>(max_threads, std::max<int64_t>(64, ((M + 3) / 4 + 63) / 64 * 64));
-  int64_t tile_cols = std::min<int64_t>(M, (int64_t)nth * kCPT);
+  // columns per thread: up to 16 for one column string; the three-string variant keeps
+  // registers (no scratch) only up to 2, so it uses more column tiles instead
+  const int cpt_max = a.nc == 1 ? kCPT : 2;
+  if (a.nc != 1) nth = max_threads;
+  int64_t tile_cols = std::min<int64_t>(M, (int64_t)nth * cpt_max);
   a.tiles = (int)((M + tile_cols - 1) / tile_cols);
   a.tile_cols = (int)tile_cols;
   a.cpt = (int)((tile_cols + nth - 1) / nth);
@@ -1083,14 +1088,23 @@ static void launch_pass(pqg_ci* ci, const Tables<T>& tb, const std::vector<int>&
   PQG_REQUIRE(off <= 160 * 1024, PQG_ERR_UNSUPPORTED,
               "sigma row (" + std::to_string(M) + " columns) does not fit in LDS");
   dim3 grid((unsigned)(nrows * a.tiles), (unsigned)k);
+  ProfScope prof(ci->ctx, st, a.nc == 1 ? (sizeof(T) == 8 ? "sigma_pass_nc1_f64" : "sigma_pass_nc1_f32")
+                                        : (sizeof(T) == 8 ? "sigma_pass_nc3_f64" : "sigma_pass_nc3_f32"),
+                 k);
+  auto go = [&](auto kern) {
+    PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)off));
+    kern<<<grid, nth, off, st>>>(a);
+  };
+  const int cpt = a.cpt;
   if (a.nc == 1) {
-    auto kern = k_sigma_pass<T, 1>;
-    PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)off));
-    kern<<<grid, nth, off, st>>>(a);
+    if (cpt <= 1) go(k_sigma_pass<T, 1, 1>);
+    else if (cpt <= 2) go(k_sigma_pass<T, 1, 2>);
+    else if (cpt <= 4) go(k_sigma_pass<T, 1, 4>);
+    else if (cpt <= 8) go(k_sigma_pass<T, 1, 8>);
+    else go(k_sigma_pass<T, 1, 16>);
   } else {
-    auto kern = k_sigma_pass<T, 3>;
-    PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)off));
-    kern<<<grid, nth, off, st>>>(a);
+    if (cpt <= 1) go(k_sigma_pass<T, 3, 1>);
+    else go(k_sigma_pass<T, 3, 2>);
   }
   PQG_HIP(hipGetLastError());
 }
@@ -1105,6 +1119,7 @@ static void transpose(const T* in, int64_t ldi, T* out, int64_t ldo, int64_t row
 
 template <typename T>
 static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k, int64_t ld, hipStream_t st) {
+  ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "sigma_f64" : "sigma_f32", k);
   const int64_t N = ci->n_int;
   const int64_t n0 = ci->slot[0].nstr;
   const int64_t M = N / n0;

@@ -113,6 +113,10 @@ void ctx_retain(pqg_ctx* c) { c->refs.fetch_add(1); }
 void ctx_release(pqg_ctx* c) {
   if (c->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(c->device);
+  for (auto& r : c->prof) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
   (void)hipStreamSynchronize(c->stream);
   (void)hipStreamDestroy(c->stream);
   c->arena.reset();
@@ -166,6 +170,42 @@ pqg_status pqg_ctx_sync(pqg_ctx* ctx) {
   return pqg::guard([&] {
     PQG_REQUIRE(ctx, PQG_ERR_ARG, "ctx is NULL");
     PQG_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+pqg_status pqg_ctx_profile(pqg_ctx* ctx, int enable) {
+  return pqg::guard([&] {
+    PQG_REQUIRE(ctx, PQG_ERR_ARG, "ctx is NULL");
+    ctx->profiling.store(enable != 0);
+  });
+}
+
+pqg_status pqg_ctx_profile_read(pqg_ctx* ctx, const char* name, int64_t* launches, double* total_ms,
+                                int64_t* total_units) {
+  return pqg::guard([&] {
+    PQG_REQUIRE(ctx && name, PQG_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->prof_mu);
+    int64_t n = 0, u = 0;
+    double ms = 0.0;
+    std::vector<pqg_ctx::ProfRec> keep;
+    for (auto& r : ctx->prof) {
+      if (r.name != name) {
+        keep.push_back(r);
+        continue;
+      }
+      PQG_HIP(hipEventSynchronize(r.stop));
+      float t = 0.f;
+      PQG_HIP(hipEventElapsedTime(&t, r.start, r.stop));
+      ms += t;
+      n++;
+      u += r.units;
+      (void)hipEventDestroy(r.start);
+      (void)hipEventDestroy(r.stop);
+    }
+    ctx->prof.swap(keep);
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+    if (total_units) *total_units = u;
   });
 }
 

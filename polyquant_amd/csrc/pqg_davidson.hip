@@ -521,8 +521,35 @@ void Davidson::post_sigma(hipStream_t st) {
       seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
       return;
     }
-    // defect correction: restart on the Ritz vectors with their exact sigma
-    restart(st, X_.get(), W64_.get(), nev_);
+    // defect correction: restart on the Ritz vectors, the first nev with their exact FP64
+    // sigma, the next ones (up to nvec_init) with the sigma of the current basis
+    const int keep = std::max(nev_, std::min(ninit_, m_));
+    ts_gemm_nn(st, V_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
+    PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
+    ts_gemm_nn(st, W_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
+    PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
+    PQG_HIP(hipMemcpyAsync(W_.get(), W64_.get(), (size_t)N_ * nev_ * 8, hipMemcpyDeviceToDevice, st));
+    m_ = nW_ = keep;
+    // projected matrix: couplings to the exact columns are taken as v^T (H x) (FP64)
+    ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, small_.get(), 64);
+    {
+      std::vector<double> q(64 * 64), gk(64 * 64, 0.0);
+      PQG_HIP(hipMemcpyAsync(q.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
+      PQG_HIP(hipStreamSynchronize(st));
+      for (int a = 0; a < keep; a++)
+        for (int b = 0; b < keep; b++) {
+          double v;
+          if (a < nev_ && b >= nev_)
+            v = q[b + a * 64];  // v_b^T W64_a
+          else if (b < nev_ && a >= nev_)
+            v = q[a + b * 64];
+          else
+            v = 0.5 * (q[a + b * 64] + q[b + a * 64]);
+          gk[a + b * 64] = v;
+        }
+      PQG_HIP(hipMemcpyAsync(G_.get(), gk.data(), 64 * 64 * 8, hipMemcpyHostToDevice, st));
+    }
+    restarts_++;
     const int before = m_;
     orthonormalize_append(st, T_.get(), nev_);
     work_ = SigmaWork{m_ - before, PQG_MIXED, V_.get() + (int64_t)before * N_, W_.get() + (int64_t)before * N_, N_};

@@ -94,11 +94,44 @@ struct pqg_ctx {
   // The caller's handle plus every live CI handle hold a reference; the context (and its
   // arena) is torn down when the last one is released.
   std::atomic<int> refs{1};
+  // kernel timing (pqg_ctx_profile)
+  std::atomic<bool> profiling{false};
+  struct ProfRec {
+    std::string name;
+    hipEvent_t start, stop;
+    int64_t units;
+  };
+  std::mutex prof_mu;
+  std::vector<ProfRec> prof;
 };
 
 namespace pqg {
 void ctx_retain(pqg_ctx* c);
 void ctx_release(pqg_ctx* c);
+
+// Brackets one kernel launch with HIP events on `st` while the context is profiling.
+class ProfScope {
+ public:
+  ProfScope(pqg_ctx* c, hipStream_t st, const char* name, int64_t units) : c_(c), st_(st) {
+    if (!c_->profiling.load()) return;
+    rec_.name = name;
+    rec_.units = units;
+    if (hipEventCreate(&rec_.start) != hipSuccess || hipEventCreate(&rec_.stop) != hipSuccess) return;
+    active_ = hipEventRecord(rec_.start, st_) == hipSuccess;
+  }
+  ~ProfScope() {
+    if (!active_) return;
+    if (hipEventRecord(rec_.stop, st_) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(c_->prof_mu);
+    c_->prof.push_back(rec_);
+  }
+
+ private:
+  pqg_ctx* c_;
+  hipStream_t st_;
+  pqg_ctx::ProfRec rec_{};
+  bool active_ = false;
+};
 }  // namespace pqg
 
 namespace pqg {
