@@ -66,7 +66,7 @@ def cpu_baseline(prob, sigma_calls, nvec, threads):
 
 def traffic_from_profiles(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
-    path = os.path.join(ROOT, "profiles", "sigma_pass_nc3_f64_ep_fci_48_pmc.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_ep_fci_48_pmc.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
@@ -131,7 +131,8 @@ def main():
         solve(PQG_FP64)
     torch.cuda.synchronize()
     ctx.profile(True)
-    for name in ("sigma_pass_nc3_f64", "sigma_pass_nc1_f64", "sigma_f64"):
+    kernels = ("tensor_gemm_f64", "sigma_pass_nc3_f64", "sigma_pass_nc1_f64", "sigma_f64")
+    for name in kernels:
         ctx.profile_read(name)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -142,28 +143,38 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ctx.profile(False)
-    launches, kern_ms, units = ctx.profile_read("sigma_pass_nc3_f64")
-    s_launch, s_ms, s_units = ctx.profile_read("sigma_f64")
-    b_launch, b_ms, _ = ctx.profile_read("sigma_pass_nc1_f64")
+    prof = {name: ctx.profile_read(name) for name in kernels}
+    s_launch, s_ms, _ = prof["sigma_f64"]
     ms_per_step = dt * 1e3 / args.steps
     value = args.steps / dt
     iters = reps[-1].iterations
     sigma_calls = s_launch // args.steps
     log(f"[bench] fp64: {ms_per_step:.1f} ms/solve, {iters} iterations, {sigma_calls} sigma calls, E={E}")
+    log(f"[bench] kernel profile: {prof}")
 
-    # roofline of the dominant kernel
-    npair = 48 * 49 // 2
-    coupling = 3 * npair * npair * 8
-    alg_bytes = 2 * 8 * N * units + 8 * N * launches + coupling * launches
+    # roofline of the dominant sigma kernel
+    dominant = max(kernels[:3], key=lambda n: prof[n][1])
+    launches, kern_ms, units = prof[dominant]
     avg_s = kern_ms * 1e-3 / max(launches, 1)
-    achieved = alg_bytes / max(launches, 1) / avg_s / 1e9
-    peak = 8000.0
-    traffic = traffic_from_profiles("sigma_pass_nc3_f64")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": peak, "unit": "GB/s",
-                "frac": round(achieved / peak, 6), "traffic": traffic,
-                "kernel": "k_sigma_pass<double,3>", "avg_launch_ms": round(avg_s * 1e3, 4),
-                "alg_bytes_per_launch": int(alg_bytes / max(launches, 1)),
-                "launches": launches}
+    traffic = traffic_from_profiles(dominant)
+    if dominant == "tensor_gemm_f64":
+        # one-particle strings: dense pair GEMMs on FP64 MFMA; units = algorithmic flops
+        achieved = units / max(kern_ms * 1e-3, 1e-30) / 1e12
+        peak = 78.6
+        roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                    "frac": round(achieved / peak, 5), "traffic": traffic,
+                    "kernel": "k_gemm_f64 (pair-GEMM sigma)", "avg_launch_ms": round(avg_s * 1e3, 5),
+                    "alg_flops_per_launch": int(units / max(launches, 1)), "launches": launches}
+    else:
+        npair = 48 * 49 // 2
+        coupling = 3 * npair * npair * 8
+        alg_bytes = 2 * 8 * N * units + 8 * N * launches + coupling * launches
+        achieved = alg_bytes / max(launches, 1) / avg_s / 1e9
+        peak = 8000.0
+        roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": peak, "unit": "GB/s",
+                    "frac": round(achieved / peak, 6), "traffic": traffic,
+                    "kernel": "k_sigma_pass<double,3>", "avg_launch_ms": round(avg_s * 1e3, 4),
+                    "alg_bytes_per_launch": int(alg_bytes / max(launches, 1)), "launches": launches}
 
     # mixed precision beside it
     for _ in range(args.warmup):
@@ -207,8 +218,7 @@ def main():
                    "n_dets": N, "roots": nev, "setup_s": round(setup_s, 4),
                    "num_subspace_vec": nvec_init, "max_subspace": nvec_max,
                    "davidson_iterations": iters, "sigma_calls": sigma_calls,
-                   "sigma_ms_avg": round(s_ms / max(s_launch, 1), 4),
-                   "pass_b_ms_avg": round(b_ms / max(b_launch, 1), 4)},
+                   "sigma_ms_avg": round(s_ms / max(s_launch, 1), 4)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <unordered_map>
@@ -999,6 +1000,12 @@ pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* d, hipStream_t stream) {
         PQG_HIP(hipGetLastError());
       }
     }
+  // ---- single-particle strings: dense pair matrices -----------------------------------------
+  if (tensor_eligible(ci.get()) && !std::getenv("PQG_NO_TENSOR_PATH")) {
+    std::vector<const double*> h1(kSlots, nullptr);
+    for (int t = 0; t < ci->nslot; t++) h1[t] = hdev[t].get();
+    tensor_setup(ci.get(), h1);
+  }
   // ---- diagonal ----------------------------------------------------------------------------
   DiagArgs da{};
   da.nslot = ci->nslot;
@@ -1120,6 +1127,10 @@ static void transpose(const T* in, int64_t ldi, T* out, int64_t ldo, int64_t row
 template <typename T>
 static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k, int64_t ld, hipStream_t st) {
   ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "sigma_f64" : "sigma_f32", k);
+  if (ci->tensor.enabled) {
+    tensor_sigma(ci, C, S, k, ld, st);
+    return;
+  }
   const int64_t N = ci->n_int;
   const int64_t n0 = ci->slot[0].nstr;
   const int64_t M = N / n0;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "pqg_internal.hpp"
+#include "pqg_tensor.hpp"
 
 namespace pqg {
 
@@ -88,6 +89,7 @@ struct pqg_ci {
   pqg::DevBuf<float> X32[pqg::kSlots][pqg::kSlots];
   pqg::DevBuf<double> DW[pqg::kSlots][pqg::kSlots];
   bool mixed_ready = false;
+  pqg::TensorPath tensor;          // single-particle strings: dense pair-GEMM sigma
   // scratch for sigma (transposed copies), grown on demand
   pqg::DevBuf<char> scratch;
   // boundary staging
@@ -109,5 +111,9 @@ void mask_invalid(pqg_ci* ci, T* v, int64_t ld, int k, hipStream_t st);
 // bytes of device memory a CI of this description needs (admission control)
 uint64_t estimate_ci_bytes(const pqg_ci_desc* d, int nvec_max, int precision);
 pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* desc, hipStream_t stream);
+bool tensor_eligible(const pqg_ci* ci);
+void tensor_setup(pqg_ci* ci, const std::vector<const double*>& h1);
+void tensor_sigma(pqg_ci* ci, const double* C, double* S, int k, int64_t ld, hipStream_t st);
+void tensor_sigma(pqg_ci* ci, const float* C, float* S, int k, int64_t ld, hipStream_t st);
 
 }  // namespace pqg

@@ -149,60 +149,100 @@ __global__ void k_scale_cols(double* __restrict__ v, int64_t n, int64_t ld, cons
   for (int j = 0; j < k; j++) v[i + j * ld] *= s[j];
 }
 
-// Parallel-order (round-robin) cyclic Jacobi for a symmetric m x m matrix, m <= 64, one
-// workgroup.  Outputs ascending eigenvalues and the matching eigenvectors (columns of Y).
-__global__ __launch_bounds__(256) void k_jacobi(const double* __restrict__ G, int ldg, int m, double* __restrict__ theta,
-                                                double* __restrict__ Y, int ldy) {
+// Parallel-order (round-robin) cyclic Jacobi for a symmetric m x m matrix, m <= 64, in one
+// workgroup, warm-started: with P = blockdiag(Yp (warm x warm), I), it diagonalises P^T G P
+// and accumulates the rotations onto P, so a projected matrix that grew by a few rows since the
+// previous iteration needs fewer sweeps.  In each step the indices form m/2 disjoint pairs
+// (circle method); thread (bi, bj) owns the 2x2 block of A at the rows of pair bi and the
+// columns of pair bj, and two rows of V at the columns of pair bj, so the rotation of a step
+// is applied in place with no exchange between threads.  Outputs ascending eigenvalues and the
+// matching eigenvectors (columns of Y).
+__device__ __forceinline__ void rr_pair(int mm, int step, int k, int& p, int& q) {
+  // circle method: index mm-1 stays, the others rotate
+  const int n1 = mm - 1;
+  if (k == 0) {
+    p = n1;
+    q = step;
+  } else {
+    p = (step + k) % n1;
+    q = (step - k + n1) % n1;
+  }
+  if (p > q) {
+    const int t = p;
+    p = q;
+    q = t;
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int tid = threadIdx.x;
+  __syncthreads();
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
+  return t;
+}
+
+__global__ __launch_bounds__(1024) void k_jacobi(const double* __restrict__ G, int ldg, int m,
+                                                 const double* __restrict__ Yp, int warm, double* __restrict__ theta,
+                                                 double* __restrict__ Y, int ldy) {
   __shared__ double A[64][65];
   __shared__ double Vv[64][65];
-  __shared__ int P[64];
+  __shared__ double Tm[64][65];
   __shared__ double cs[32], sn[32];
-  __shared__ double red[256];
+  __shared__ int pp[32], qq[32];
+  __shared__ double red[16];
   __shared__ int order[64];
   const int tid = threadIdx.x;
-  const int mm = m + (m & 1);
-  const int np = mm / 2;
-  for (int i = tid; i < 64 * 64; i += 256) {
+  const int mm = m + (m & 1), half = mm / 2;
+  for (int i = tid; i < 64 * 64; i += 1024) {
     const int r = i / 64, c = i % 64;
+    double v = (r == c) ? 1.0 : 0.0;
+    if (r < warm && c < warm) v = Yp[r + c * 64];
+    Vv[r][c] = v;
     A[r][c] = (r < m && c < m) ? 0.5 * (G[r + c * ldg] + G[c + r * ldg]) : 0.0;
-    Vv[r][c] = (r == c) ? 1.0 : 0.0;
   }
-  if (tid < 64) P[tid] = tid;
   __syncthreads();
-  for (int sweep = 0; sweep < 60; sweep++) {
-    double off = 0.0, dia = 0.0;
-    for (int i = tid; i < m * m; i += 256) {
+  if (warm > 0) {  // A = P^T Gs P
+    for (int i = tid; i < m * m; i += 1024) {
       const int r = i / m, c = i % m;
+      double v = 0.0;
+      for (int k = 0; k < m; k++) v += A[r][k] * Vv[k][c];
+      Tm[r][c] = v;
+    }
+    __syncthreads();
+    for (int i = tid; i < m * m; i += 1024) {
+      const int r = i / m, c = i % m;
+      double v = 0.0;
+      for (int k = 0; k < m; k++) v += Vv[k][r] * Tm[k][c];
+      A[r][c] = v;
+    }
+    __syncthreads();
+  }
+  const int bi = tid >> 5, bj = tid & 31;
+  const bool active = bi < half && bj < half;
+  const int vr0 = 2 * bi;  // rows of V owned by this thread
+  for (int sweep = 0; sweep < 40; sweep++) {
+    double off = 0.0, dia = 0.0;
+    for (int i = tid; i < m * m; i += 1024) {
+      const int r = i / m, c = i % m;
+      const double a = A[r][c];
       if (r != c)
-        off += A[r][c] * A[r][c];
+        off += a * a;
       else
-        dia += A[r][c] * A[r][c];
+        dia += a * a;
     }
-    red[tid] = off;
+    const double offs = block_sum(off, red);
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    const double offs = red[0];
-    __syncthreads();
-    red[tid] = dia;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    const double dias = red[0];
-    __syncthreads();
+    const double dias = block_sum(dia, red);
     if (offs <= 1e-30 * dias || offs < 1e-300) break;
     for (int step = 0; step < mm - 1; step++) {
-      if (tid < np) {
-        int p = P[tid], q = P[mm - 1 - tid];
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
+      __syncthreads();  // previous step's updates are visible
+      if (tid < half) {
+        int p, q;
+        rr_pair(mm, step, tid, p, q);
         double c = 1.0, s = 0.0;
         if (q < m) {
           const double apq = A[p][q];
@@ -215,68 +255,80 @@ __global__ __launch_bounds__(256) void k_jacobi(const double* __restrict__ G, in
         }
         cs[tid] = c;
         sn[tid] = s;
+        pp[tid] = p;
+        qq[tid] = q;
       }
       __syncthreads();
-      for (int i = tid; i < np * m; i += 256) {  // rows
-        const int k = i / m, j = i % m;
-        int p = P[k], q = P[mm - 1 - k];
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
+      if (active) {
+        // new column p = c*col p - s*col q, new column q = s*col p + c*col q (same for rows)
+        const int pi = pp[bi], qi = qq[bi], pj = pp[bj], qj = qq[bj];
+        const double ci = cs[bi], si = sn[bi], cj = cs[bj], sj = sn[bj];
+        const bool qiok = qi < m, qjok = qj < m;
+        const double a00 = A[pi][pj];
+        const double a01 = qjok ? A[pi][qj] : 0.0;
+        const double a10 = qiok ? A[qi][pj] : 0.0;
+        const double a11 = (qiok && qjok) ? A[qi][qj] : 0.0;
+        // rows
+        const double r00 = ci * a00 - si * a10, r01 = ci * a01 - si * a11;
+        const double r10 = si * a00 + ci * a10, r11 = si * a01 + ci * a11;
+        // columns
+        const double n00 = cj * r00 - sj * r01, n01 = sj * r00 + cj * r01;
+        const double n10 = cj * r10 - sj * r11, n11 = sj * r10 + cj * r11;
+        const bool diag = bi == bj;  // the rotated pair's coupling is annihilated exactly
+        A[pi][pj] = n00;
+        if (qjok) A[pi][qj] = diag ? 0.0 : n01;
+        if (qiok) A[qi][pj] = diag ? 0.0 : n10;
+        if (qiok && qjok) A[qi][qj] = n11;
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+          const int r = vr0 + e;
+          if (r < m) {
+            const double vp = Vv[r][pj], vq = qjok ? Vv[r][qj] : 0.0;
+            Vv[r][pj] = cj * vp - sj * vq;
+            if (qjok) Vv[r][qj] = sj * vp + cj * vq;
+          }
         }
-        if (q >= m) continue;
-        const double c = cs[k], s = sn[k];
-        const double ap = A[p][j], aq = A[q][j];
-        A[p][j] = c * ap - s * aq;
-        A[q][j] = s * ap + c * aq;
       }
-      __syncthreads();
-      for (int i = tid; i < np * m; i += 256) {  // columns + eigenvectors
-        const int k = i / m, j = i % m;
-        int p = P[k], q = P[mm - 1 - k];
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        if (q >= m) continue;
-        const double c = cs[k], s = sn[k];
-        const double ap = A[j][p], aq = A[j][q];
-        A[j][p] = c * ap - s * aq;
-        A[j][q] = s * ap + c * aq;
-        const double vp = Vv[j][p], vq = Vv[j][q];
-        Vv[j][p] = c * vp - s * vq;
-        Vv[j][q] = s * vp + c * vq;
-      }
-      __syncthreads();
-      if (tid == 0) {  // rotate players 1..mm-1
-        const int last = P[mm - 1];
-        for (int i = mm - 1; i > 1; i--) P[i] = P[i - 1];
-        P[1] = last;
-      }
-      __syncthreads();
     }
-  }
-  if (tid == 0) {
-    for (int i = 0; i < m; i++) order[i] = i;
-    for (int i = 0; i < m; i++)
-      for (int j = i + 1; j < m; j++)
-        if (A[order[j]][order[j]] < A[order[i]][order[i]]) {
-          const int t = order[i];
-          order[i] = order[j];
-          order[j] = t;
-        }
+    __syncthreads();
   }
   __syncthreads();
-  for (int i = tid; i < m; i += 256) theta[i] = A[order[i]][order[i]];
-  for (int i = tid; i < m * m; i += 256) {
+  if (tid < m) {  // rank of each eigenvalue (ties by index)
+    const double d = A[tid][tid];
+    int rk = 0;
+    for (int j = 0; j < m; j++) {
+      const double e = A[j][j];
+      rk += (e < d || (e == d && j < tid)) ? 1 : 0;
+    }
+    order[rk] = tid;
+  }
+  __syncthreads();
+  for (int i = tid; i < m; i += 1024) theta[i] = A[order[i]][order[i]];
+  for (int i = tid; i < m * m; i += 1024) {
     const int r = i % m, c = i / m;
     Y[r + c * ldy] = Vv[r][order[c]];
   }
 }
 
 static int g1(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// G (64 x 64): columns [old, m) are new.  Fill their mirror rows; gt (newc x old, ld 64),
+// when given, supplies the new-vs-old couplings.
+__global__ void k_symmetrize(double* __restrict__ G, int old, int m, const double* __restrict__ gt) {
+  for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
+    const int r = i % 64, c = i / 64;
+    if (c < old || c >= m || r >= m) continue;
+    if (r < old) {
+      const double v = gt ? gt[(c - old) + r * 64] : G[r + c * 64];
+      G[r + c * 64] = v;
+      G[c + r * 64] = v;
+    } else if (r > c) {
+      const double v = 0.5 * (G[r + c * 64] + G[c + r * 64]);
+      G[r + c * 64] = v;
+      G[c + r * 64] = v;
+    }
+  }
+}
 
 // Per-iteration log on stderr when PQG_DAV_LOG is set: the fields of the reference's
 // DavidsonDerivedLogger (src/io/davidson_logging.hpp:17-81) -- iteration, converged roots,
@@ -310,6 +362,7 @@ Davidson::Davidson(pqg_ci* ci, const pqg_dav_params& p) : ci_(ci), p_(p) {
   W_.reset(ctx, (size_t)N_ * nmax_);
   G_.reset(ctx, 64 * 64);
   Y_.reset(ctx, 64 * 64);
+  Yprev_.reset(ctx, 64 * 64);
   theta_.reset(ctx, 64);
   X_.reset(ctx, (size_t)N_ * nev_);
   AX_.reset(ctx, (size_t)N_ * nev_);
@@ -391,82 +444,107 @@ void Davidson::run_sigma(hipStream_t st) {
   sigma_calls_ += k;
 }
 
-// Orthogonalise T (nt columns) against V (m columns) twice, then among themselves (MGS);
-// appends the surviving columns to V.  Returns how many were appended.
+// Cholesky factor of a small Gram matrix B (n x n, leading dimension 64) with column dropping:
+// column j is kept when its residual norm after the kept columns before it exceeds thr[j].
+// Returns M (n x nk, leading dimension 64) with Q = T M orthonormal, and the kept count.
+static int small_cholqr(const std::vector<double>& B, int n, const std::vector<double>& thr, std::vector<double>& M) {
+  std::vector<int> kept;
+  std::vector<double> R(64 * 64, 0.0);  // R[i + j*64], i <= j, over original column indices
+  for (int j = 0; j < n; j++) {
+    double d = B[j + j * 64];
+    for (size_t a = 0; a < kept.size(); a++) {
+      const int i = kept[a];
+      double v = B[i + j * 64];
+      for (size_t b = 0; b < a; b++) v -= R[kept[b] + i * 64] * R[kept[b] + j * 64];
+      R[i + j * 64] = v / R[i + i * 64];
+      d -= R[i + j * 64] * R[i + j * 64];
+    }
+    if (d > thr[j] * thr[j] && d > 0.0) {
+      R[j + j * 64] = std::sqrt(d);
+      kept.push_back(j);
+    }
+  }
+  const int nk = (int)kept.size();
+  // inverse of the kept upper-triangular factor, placed on the kept rows of M
+  std::vector<double> Rk(64 * 64, 0.0), Ri(64 * 64, 0.0);
+  for (int a = 0; a < nk; a++)
+    for (int b = a; b < nk; b++) Rk[a + b * 64] = R[kept[a] + kept[b] * 64];
+  for (int b = 0; b < nk; b++) {
+    Ri[b + b * 64] = 1.0 / Rk[b + b * 64];
+    for (int a = b - 1; a >= 0; a--) {
+      double v = 0.0;
+      for (int c = a + 1; c <= b; c++) v += Rk[a + c * 64] * Ri[c + b * 64];
+      Ri[a + b * 64] = -v / Rk[a + a * 64];
+    }
+  }
+  M.assign(64 * 64, 0.0);
+  for (int a = 0; a < nk; a++)
+    for (int b = 0; b < nk; b++) M[kept[a] + b * 64] = Ri[a + b * 64];
+  return nk;
+}
+
+// Orthogonalise T (nt columns) against V (m columns) with two block Gram-Schmidt sweeps, then
+// orthonormalise the block by two Cholesky-QR passes (columns that vanish are dropped), and
+// append it to V.  Returns how many columns were appended.
 int Davidson::orthonormalize_append(hipStream_t st, double* T, int nt) {
-  std::vector<double> nrm0(nt), nrm(nt);
-  // norms before
+  nt = std::min(nt, nmax_ - m_);
+  if (nt <= 0) return 0;
+  std::vector<double> g(64 * 64), M;
   ts_gemm_tn(st, T, N_, nt, T, N_, nt, small_.get(), 64);
-  std::vector<double> g(64 * 64);
   PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
   PQG_HIP(hipStreamSynchronize(st));
-  for (int j = 0; j < nt; j++) nrm0[j] = std::sqrt(std::max(g[j + j * 64], 0.0));
+  std::vector<double> thr(nt);
+  for (int j = 0; j < nt; j++) thr[j] = std::max(1e-10 * std::sqrt(std::max(g[j + j * 64], 0.0)), 1e-30);
   for (int pass = 0; pass < 2; pass++) {
     ts_gemm_tn(st, V_.get(), N_, m_, T, N_, nt, small_.get(), 64);
     ts_gemm_nn(st, V_.get(), N_, m_, small_.get(), 64, nt, T, N_, -1.0, 1.0);
   }
-  int added = 0;
-  for (int j = 0; j < nt && m_ < nmax_; j++) {
-    double* t = T + (int64_t)j * N_;
-    double* dst = V_.get() + (int64_t)m_ * N_;
-    for (int pass = 0; pass < 2; pass++) {
-      if (added > 0) {
-        const double* Vnew = V_.get() + (int64_t)(m_ - added) * N_;
-        ts_gemm_tn(st, Vnew, N_, added, t, N_, 1, small_.get(), 64);
-        ts_gemm_nn(st, Vnew, N_, added, small_.get(), 64, 1, t, N_, -1.0, 1.0);
-      }
-    }
-    ts_gemm_tn(st, t, N_, 1, t, N_, 1, small_.get(), 64);
-    double nn = 0.0;
-    PQG_HIP(hipMemcpyAsync(&nn, small_.get(), 8, hipMemcpyDeviceToHost, st));
-    PQG_HIP(hipStreamSynchronize(st));
-    nn = std::sqrt(std::max(nn, 0.0));
-    if (!(nn > 1e-10 * std::max(nrm0[j], 1e-300)) || nn < 1e-30) continue;
-    const double inv = 1.0 / nn;
-    PQG_HIP(hipMemcpyAsync(small_.get(), &inv, 8, hipMemcpyHostToDevice, st));
-    PQG_HIP(hipMemcpyAsync(dst, t, N_ * 8, hipMemcpyDeviceToDevice, st));
-    k_scale_cols<<<g1(N_, 256), 256, 0, st>>>(dst, N_, N_, small_.get(), 1);
-    PQG_HIP(hipGetLastError());
-    PQG_HIP(hipStreamSynchronize(st));
-    m_++;
-    added++;
-  }
-  mask_invalid<double>(ci_, V_.get() + (int64_t)(m_ - added) * N_, N_, added, st);
-  return added;
+  // pass 1: drop vanishing columns, orthonormalise the rest straight into V[:, m:]
+  ts_gemm_tn(st, T, N_, nt, T, N_, nt, small_.get(), 64);
+  PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
+  PQG_HIP(hipStreamSynchronize(st));
+  const int nk = small_cholqr(g, nt, thr, M);
+  if (nk == 0) return 0;
+  double* dst = V_.get() + (int64_t)m_ * N_;
+  PQG_HIP(hipMemcpyAsync(small_.get(), M.data(), 64 * 64 * 8, hipMemcpyHostToDevice, st));
+  ts_gemm_nn(st, T, N_, nt, small_.get(), 64, nk, dst, N_, 1.0, 0.0);
+  // pass 2 (CholQR2): re-orthonormalise the new block
+  ts_gemm_tn(st, dst, N_, nk, dst, N_, nk, small_.get(), 64);
+  PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
+  PQG_HIP(hipStreamSynchronize(st));
+  std::vector<double> thr2(nk, 0.0);
+  const int nk2 = small_cholqr(g, nk, thr2, M);
+  if (nk2 != nk) return 0;  // numerically rank deficient: do not extend
+  PQG_HIP(hipMemcpyAsync(small_.get(), M.data(), 64 * 64 * 8, hipMemcpyHostToDevice, st));
+  ts_gemm_nn(st, dst, N_, nk, small_.get(), 64, nk, tmp_.get(), N_, 1.0, 0.0);
+  PQG_HIP(hipMemcpyAsync(dst, tmp_.get(), (size_t)N_ * nk * 8, hipMemcpyDeviceToDevice, st));
+  mask_invalid<double>(ci_, dst, N_, nk, st);
+  m_ += nk;
+  return nk;
 }
 
 void Davidson::rayleigh_ritz(hipStream_t st) {
-  // G[:, nW_old:m] = V^T W[:, nW_old:m]; the rows of the new columns against older columns
-  // follow by symmetry.
+  // G[:, old:m] = V^T W[:, old:m]; the rows of the new columns against older columns follow by
+  // symmetry.  Mixed precision: those couplings are taken as v_new^T (W v_old) instead, so that
+  // basis vectors whose sigma is FP64 (the Ritz vectors after a defect-correction restart)
+  // couple with FP64 accuracy.
   const int old = nW_;
   const int newc = m_ - nW_;
+  const double* gt = nullptr;
   if (newc > 0) {
     ts_gemm_tn(st, V_.get(), N_, m_, W_.get() + (int64_t)nW_ * N_, N_, newc, G_.get() + (int64_t)nW_ * 64, 64);
+    if (mixed_ && old > 0) {
+      ts_gemm_tn(st, V_.get() + (int64_t)old * N_, N_, newc, W_.get(), N_, old, small_.get(), 64);
+      gt = small_.get();
+    }
     nW_ = m_;
   }
-  // Mixed precision: the couplings of new directions to older basis vectors are taken as
-  // v_new^T (W v_old) rather than v_old^T (W v_new), so that basis vectors whose sigma is FP64
-  // (the Ritz vectors after a defect-correction restart) couple with FP64 accuracy.
-  std::vector<double> g(64 * 64), gt;
-  if (mixed_ && old > 0 && newc > 0) {
-    ts_gemm_tn(st, V_.get() + (int64_t)old * N_, N_, newc, W_.get(), N_, old, small_.get(), 64);
-    gt.resize(64 * 64);
-    PQG_HIP(hipMemcpyAsync(gt.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
-  }
-  PQG_HIP(hipMemcpyAsync(g.data(), G_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
-  PQG_HIP(hipStreamSynchronize(st));
-  for (int c = old; c < m_; c++)
-    for (int r = 0; r < m_; r++) {
-      if (r < old) {
-        const double v = gt.empty() ? g[r + c * 64] : gt[(c - old) + r * 64];
-        g[c + r * 64] = g[r + c * 64] = v;
-      } else if (r > c) {
-        g[r + c * 64] = g[c + r * 64] = 0.5 * (g[c + r * 64] + g[r + c * 64]);
-      }
-    }
-  PQG_HIP(hipMemcpyAsync(G_.get(), g.data(), 64 * 64 * 8, hipMemcpyHostToDevice, st));
-  k_jacobi<<<1, 256, 0, st>>>(G_.get(), 64, m_, theta_.get(), Y_.get(), 64);
+  k_symmetrize<<<1, 256, 0, st>>>(G_.get(), old, m_, gt);
   PQG_HIP(hipGetLastError());
+  k_jacobi<<<1, 1024, 0, st>>>(G_.get(), 64, m_, Yprev_.get(), std::min(warm_, m_), theta_.get(), Y_.get(), 64);
+  PQG_HIP(hipGetLastError());
+  PQG_HIP(hipMemcpyAsync(Yprev_.get(), Y_.get(), 64 * 64 * 8, hipMemcpyDeviceToDevice, st));
+  warm_ = m_;
 }
 
 void Davidson::restart(hipStream_t st, const double* Vk, const double* Wk, int keep) {
@@ -477,6 +555,7 @@ void Davidson::restart(hipStream_t st, const double* Vk, const double* Wk, int k
   // recompute the keep x keep projection exactly
   ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, G_.get(), 64);
   m_ = nW_ = keep;
+  warm_ = 0;
   restarts_++;
 }
 
@@ -530,6 +609,7 @@ void Davidson::post_sigma(hipStream_t st) {
     PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
     PQG_HIP(hipMemcpyAsync(W_.get(), W64_.get(), (size_t)N_ * nev_ * 8, hipMemcpyDeviceToDevice, st));
     m_ = nW_ = keep;
+    warm_ = 0;
     // projected matrix: couplings to the exact columns are taken as v^T (H x) (FP64)
     ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, small_.get(), 64);
     {
@@ -604,6 +684,7 @@ void Davidson::post_sigma(hipStream_t st) {
     ts_gemm_nn(st, W_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
     PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
     m_ = nW_ = keep;
+    warm_ = 0;
     // projection after restart = diag(theta[:keep]) up to rounding; recompute exactly
     ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, G_.get(), 64);
     restarts_++;

@@ -51,9 +51,9 @@ class Davidson {
   int64_t N_;
   int nev_, ninit_, nmax_;
   bool mixed_;
-  DevBuf<double> V_, W_, G_, Y_, theta_, X_, AX_, R_, T_, tmp_, part_, small_, W64_;
+  DevBuf<double> V_, W_, G_, Y_, Yprev_, theta_, X_, AX_, R_, T_, tmp_, part_, small_, W64_;
   DevBuf<float> c32_, s32_;
-  int m_ = 0, nW_ = 0, iter_ = 0, restarts_ = 0, since_check_ = 0;
+  int m_ = 0, nW_ = 0, iter_ = 0, restarts_ = 0, since_check_ = 0, warm_ = 0;
   int sigma_calls_ = 0, sigma_calls64_ = 0;
   bool done_ = false, checking_ = false;
   pqg_status status_ = PQG_OK;

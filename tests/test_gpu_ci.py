@@ -22,6 +22,10 @@ PROBLEMS = {
     "ep_truncated_psh_layout": lambda: wl.small_problem(108, 6, 1, 1, positron=(6, 1, 0), ex_level=(2, 2, 2),
                                                        pos_ex_level=(1, 0, 1), max_collective=2),
     "fci_single_electron_9o_1a0b": lambda: wl.small_problem(109, 9, 1, 0),
+    # one particle per string: the dense pair-GEMM sigma path
+    "ep_fci_1p_strings_9e_1a1b_8p_1": lambda: wl.small_problem(110, 9, 1, 1, positron=(8, 1, 0)),
+    "fci_1p_strings_11o_1a1b": lambda: wl.small_problem(111, 11, 1, 1),
+    "fci_1p_strings_unrestricted_7o_1a1b": lambda: wl.small_problem(112, 7, 1, 1, nspin=2),
 }
 
 
@@ -108,3 +112,20 @@ def test_davidson_mixed_reaches_fp64_energy(name, ctx):
     assert rep.converged == 1
     assert np.max(np.abs(E - w[:2])) < 1e-8
     assert rep.sigma_calls_fp64 >= 2  # FP64 residual checks happened
+
+
+@pytest.mark.parametrize("name", ["ep_fci_1p_strings_9e_1a1b_8p_1", "fci_1p_strings_11o_1a1b"])
+def test_tensor_path_equals_enumeration_path(name, ctx, monkeypatch):
+    """The pair-GEMM sigma for one-particle strings equals the general enumeration kernel."""
+    from polyquant_amd._lib import PQG_MIXED
+    from polyquant_amd.ci import DetSet
+
+    prob = PROBLEMS[name]()
+    C = np.random.default_rng(3).standard_normal((wl.CIProblem.n_dets_fci(prob), 2))
+    fast = DetSet(prob, ctx)
+    monkeypatch.setenv("PQG_NO_TENSOR_PATH", "1")
+    slow = DetSet(prob, ctx)
+    a, b = fast.sigma(C), slow.sigma(C)
+    assert np.max(np.abs(a - b)) < 1e-12 * np.max(np.abs(b))
+    a32 = fast.sigma(C, precision=PQG_MIXED)
+    assert np.max(np.abs(a32 - b)) < 2e-5 * np.max(np.abs(b))
