@@ -30,62 +30,75 @@ namespace pqg {
 
 constexpr int kTB = 256;
 
-// out[a][b] partials: sum over a row chunk of A[r][a] * B[r][b], a in [a0, a0+8), b in [b0, b0+4)
+// Partials of A^T B for tall-skinny A (nrows x na) and B (nrows x nb), column-major: block
+// (x, y, z) sums A[r][a] * B[r][b] for column a = y, b in [4z, 4z+4), rows r of chunk x into
+// part[x][a][b].  One column of A per block keeps the per-block reduction to four values and
+// lets many blocks stream A concurrently.
+
+// out[a][b] = sum over the chunks of the partials in a fixed order (one block per output)
+__global__ __launch_bounds__(256) void k_tn_reduce(const double* __restrict__ part, int nchunk, int na, int nb,
+                                                   double* __restrict__ out, int ldo) {
+  const int o = blockIdx.x, G = na * nb;
+  double v = 0.0;
+  for (int c = threadIdx.x; c < nchunk; c += 256) v += part[(size_t)c * G + o];
+  __shared__ double red[4];
+  for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[(o / nb) + (o % nb) * ldo] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ __launch_bounds__(kTB) void k_tn_partial(const double* __restrict__ A, int64_t lda, int na,
                                                     const double* __restrict__ B, int64_t ldb, int nb, int64_t nrows,
                                                     int64_t chunk, double* __restrict__ part) {
-  const int a0 = blockIdx.y * 8, b0 = blockIdx.z * 4;
+  const int a = blockIdx.y, b0 = blockIdx.z * 4;
   const int64_t r0 = (int64_t)blockIdx.x * chunk;
   const int64_t r1 = min(nrows, r0 + chunk);
-  double acc[8][4];
+  const double* __restrict__ Aa = A + (int64_t)a * lda;
+  const int nbl = min(4, nb - b0);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t r = r0 + threadIdx.x;
+  for (; r + 3 * kTB < r1; r += 4 * kTB) {
+    double av[4], bv[4][4];
 #pragma unroll
-  for (int i = 0; i < 8; i++)
+    for (int u = 0; u < 4; u++) av[u] = Aa[r + u * kTB];
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += kTB) {
-    double bv[4], av[8];
+    for (int j = 0; j < 4; j++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) bv[j] = (b0 + j < nb) ? B[r + (b0 + j) * ldb] : 0.0;
+      for (int u = 0; u < 4; u++) bv[j][u] = (j < nbl) ? B[r + u * kTB + (b0 + j) * ldb] : 0.0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) av[i] = (a0 + i < na) ? A[r + (a0 + i) * lda] : 0.0;
+    for (int j = 0; j < 4; j++)
 #pragma unroll
-    for (int i = 0; i < 8; i++)
-#pragma unroll
-      for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+      for (int u = 0; u < 4; u++) acc[j] += av[u] * bv[j][u];
   }
-  __shared__ double red[kTB / 64][32];
+  for (; r < r1; r += kTB) {
+    const double av = Aa[r];
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] += (j < nbl) ? av * B[r + (b0 + j) * ldb] : 0.0;
+  }
+  __shared__ double red[kTB];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int i = 0; i < 8; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      double v = acc[i][j];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-      if (lane == 0) red[wv][i * 4 + j] = v;
-    }
+  for (int j = 0; j < 4; j++) {
+    double v = acc[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) red[wv * 4 + j] = v;
+  }
   __syncthreads();
-  if (threadIdx.x < 32) {
-    const int i = threadIdx.x / 4, j = threadIdx.x % 4;
+  if (threadIdx.x < nbl) {
+    const int j = threadIdx.x;
     double v = 0.0;
-    for (int w = 0; w < kTB / 64; w++) v += red[w][threadIdx.x];
-    if (a0 + i < na && b0 + j < nb) part[((size_t)blockIdx.x * na + (a0 + i)) * nb + (b0 + j)] = v;
+    for (int w = 0; w < kTB / 64; w++) v += red[w * 4 + j];
+    part[((size_t)blockIdx.x * na + a) * nb + (b0 + j)] = v;
   }
 }
 
-__global__ void k_tn_reduce(const double* __restrict__ part, int nchunk, int na, int nb, double* __restrict__ out,
-                            int ldo) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= na * nb) return;
-  const int a = idx / nb, b = idx % nb;
-  double v = 0.0;
-  for (int c = 0; c < nchunk; c++) v += part[((size_t)c * na + a) * nb + b];
-  out[a + b * ldo] = v;
-}
-
-// out[r][j] = beta * out[r][j] + alpha * sum_c A[r][c] Bs[c][j]   (Bs small, ka x nb, in LDS)
-__global__ __launch_bounds__(kTB) void k_nn(const double* __restrict__ A, int64_t lda, int ka,
-                                            const double* __restrict__ Bs, int ldb, int nb, double* __restrict__ out,
-                                            int64_t ldo, int64_t nrows, double alpha, double beta) {
+// out[r][j] = beta * out[r][j] + alpha * sum_c A[r][c] Bs[c][j]   (Bs small, ka x nb, in LDS).
+// blockIdx.y selects (A0, out0) or (A1, out1), so two products with the same Bs share a launch.
+// A row is read completely before its outputs are written, so out may alias A when nb <= 8.
+__global__ __launch_bounds__(kTB) void k_nn(const double* A0, const double* A1, int64_t lda, int ka,
+                                            const double* __restrict__ Bs, int ldb, int nb, double* out0,
+                                            double* out1, int64_t ldo, int64_t nrows, double alpha, double beta) {
   __shared__ double bs[64 * 64];
   for (int i = threadIdx.x; i < ka * nb; i += kTB) {
     const int c = i % ka, j = i / ka;
@@ -94,10 +107,13 @@ __global__ __launch_bounds__(kTB) void k_nn(const double* __restrict__ A, int64_
   __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (r >= nrows) return;
+  const double* A = blockIdx.y ? A1 : A0;
+  double* out = blockIdx.y ? out1 : out0;
   for (int j0 = 0; j0 < nb; j0 += 8) {
     double acc[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) acc[j] = 0.0;
+#pragma unroll 16
     for (int c = 0; c < ka; c++) {
       const double a = A[r + c * lda];
 #pragma unroll
@@ -185,9 +201,9 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return t;
 }
 
-__global__ __launch_bounds__(1024) void k_jacobi(const double* __restrict__ G, int ldg, int m,
+__global__ __launch_bounds__(1024) void k_jacobi(double* __restrict__ G, int m, int old, const double* __restrict__ gt,
                                                  const double* __restrict__ Yp, int warm, double* __restrict__ theta,
-                                                 double* __restrict__ Y, int ldy) {
+                                                 double* __restrict__ Y, double* __restrict__ Ynext) {
   __shared__ double A[64][65];
   __shared__ double Vv[64][65];
   __shared__ double Tm[64][65];
@@ -197,14 +213,33 @@ __global__ __launch_bounds__(1024) void k_jacobi(const double* __restrict__ G, i
   __shared__ int order[64];
   const int tid = threadIdx.x;
   const int mm = m + (m & 1), half = mm / 2;
+  // G (leading dimension 64) holds V^T W, whose columns [old, m) belong to the new basis
+  // vectors; the couplings of new and old vectors are completed by symmetry (from
+  // gt[(c - old) + r * 64] = v_c^T W v_r when given), the rest is symmetrised, and G is written
+  // back symmetric.
   for (int i = tid; i < 64 * 64; i += 1024) {
     const int r = i / 64, c = i % 64;
     double v = (r == c) ? 1.0 : 0.0;
     if (r < warm && c < warm) v = Yp[r + c * 64];
     Vv[r][c] = v;
-    A[r][c] = (r < m && c < m) ? 0.5 * (G[r + c * ldg] + G[c + r * ldg]) : 0.0;
+    double g = 0.0;
+    if (r < m && c < m) {
+      if (r < old && c < old)
+        g = 0.5 * (G[r + c * 64] + G[c + r * 64]);
+      else if (r < old)
+        g = gt ? gt[(c - old) + r * 64] : G[r + c * 64];
+      else if (c < old)
+        g = gt ? gt[(r - old) + c * 64] : G[c + r * 64];
+      else
+        g = 0.5 * (G[r + c * 64] + G[c + r * 64]);
+    }
+    A[r][c] = g;
   }
   __syncthreads();
+  for (int i = tid; i < m * m; i += 1024) {
+    const int r = i / m, c = i % m;
+    G[r + c * 64] = A[r][c];
+  }
   if (warm > 0) {  // A = P^T Gs P
     for (int i = tid; i < m * m; i += 1024) {
       const int r = i / m, c = i % m;
@@ -306,7 +341,85 @@ __global__ __launch_bounds__(1024) void k_jacobi(const double* __restrict__ G, i
   for (int i = tid; i < m; i += 1024) theta[i] = A[order[i]][order[i]];
   for (int i = tid; i < m * m; i += 1024) {
     const int r = i % m, c = i / m;
-    Y[r + c * ldy] = Vv[r][order[c]];
+    const double v = Vv[r][order[c]];
+    Y[r + c * 64] = v;
+    Ynext[r + c * 64] = v;
+  }
+}
+
+// Cholesky factor of a small Gram matrix B (n x n, leading dimension 64) with column dropping,
+// on the device: column j is kept when its residual norm after the kept columns before it
+// exceeds thr_j = max(rel * sqrt(g0[j][j]), 1e-30) (thr_j = 0 without g0).  Writes M (n x nk,
+// leading dimension 64, zero elsewhere) with Q = T M orthonormal, and nk.  n comes from n_dev
+// when given.  One wave: lane 0 factors, lane b inverts column b of the triangular factor.
+__global__ __launch_bounds__(64) void k_cholqr_small(const double* __restrict__ B, const int* __restrict__ n_dev,
+                                                     int n_host, const double* __restrict__ g0, double rel,
+                                                     double* __restrict__ M, int* __restrict__ nk_out) {
+  __shared__ double R[64][65];
+  __shared__ double Ri[64][65];
+  __shared__ int kept[64];
+  __shared__ int nk_s;
+  const int lane = threadIdx.x;
+  const int n = n_dev ? *n_dev : n_host;
+  for (int i = lane; i < n * n; i += 64) {
+    R[i / n][i % n] = 0.0;
+    Ri[i / n][i % n] = 0.0;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    int nk = 0;
+    for (int j = 0; j < n; j++) {
+      double d = B[j + j * 64];
+      for (int a = 0; a < nk; a++) {
+        const int i = kept[a];
+        double v = B[i + j * 64];
+        for (int b = 0; b < a; b++) v -= R[kept[b]][i] * R[kept[b]][j];
+        R[i][j] = v / R[i][i];
+        d -= R[i][j] * R[i][j];
+      }
+      const double t = g0 ? fmax(rel * sqrt(fmax(g0[j + j * 64], 0.0)), 1e-30) : 0.0;
+      if (d > t * t && d > 0.0) {
+        R[j][j] = sqrt(d);
+        kept[nk++] = j;
+      }
+    }
+    nk_s = nk;
+  }
+  __syncthreads();
+  const int nk = nk_s;
+  if (lane < nk) {  // column b of the inverse of the kept factor Rk[a][c] = R[kept[a]][kept[c]]
+    const int b = lane;
+    Ri[b][b] = 1.0 / R[kept[b]][kept[b]];
+    for (int a = b - 1; a >= 0; a--) {
+      double v = 0.0;
+      for (int c = a + 1; c <= b; c++) v += R[kept[a]][kept[c]] * Ri[c][b];
+      Ri[a][b] = -v / R[kept[a]][kept[a]];
+    }
+  }
+  __shared__ int slot[64];  // position of row r among the kept columns, or -1
+  if (lane < 64) slot[lane] = -1;
+  __syncthreads();
+  if (lane < nk) slot[kept[lane]] = lane;
+  __syncthreads();
+  for (int i = lane; i < 64 * 64; i += 64) {
+    const int r = i % 64, c = i / 64;  // M[r + 64 c]
+    const int a = slot[r];
+    M[i] = (c < nk && a >= 0) ? Ri[a][c] : 0.0;
+  }
+  if (lane == 0) *nk_out = nk;
+}
+
+// status block read by the host once per iteration: residual norms, Ritz values, appended count
+__global__ void k_pack_status(const double* __restrict__ rgram, const double* __restrict__ theta,
+                              const int* __restrict__ nk, int nev, double* __restrict__ out) {
+  const int i = threadIdx.x;
+  if (i < 64) {
+    out[i] = i < nev ? sqrt(fmax(rgram[i + i * 64], 0.0)) : 0.0;
+    out[64 + i] = theta[i];
+  }
+  if (i == 0) {
+    out[128] = nk[0];
+    out[129] = nk[1];
   }
 }
 
@@ -314,22 +427,6 @@ static int g1(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 // G (64 x 64): columns [old, m) are new.  Fill their mirror rows; gt (newc x old, ld 64),
 // when given, supplies the new-vs-old couplings.
-__global__ void k_symmetrize(double* __restrict__ G, int old, int m, const double* __restrict__ gt) {
-  for (int i = threadIdx.x; i < 64 * 64; i += blockDim.x) {
-    const int r = i % 64, c = i / 64;
-    if (c < old || c >= m || r >= m) continue;
-    if (r < old) {
-      const double v = gt ? gt[(c - old) + r * 64] : G[r + c * 64];
-      G[r + c * 64] = v;
-      G[c + r * 64] = v;
-    } else if (r > c) {
-      const double v = 0.5 * (G[r + c * 64] + G[c + r * 64]);
-      G[r + c * 64] = v;
-      G[c + r * 64] = v;
-    }
-  }
-}
-
 // Per-iteration log on stderr when PQG_DAV_LOG is set: the fields of the reference's
 // DavidsonDerivedLogger (src/io/davidson_logging.hpp:17-81) -- iteration, converged roots,
 // subspace size, eigenvalue and residual of every root.
@@ -363,15 +460,23 @@ Davidson::Davidson(pqg_ci* ci, const pqg_dav_params& p) : ci_(ci), p_(p) {
   G_.reset(ctx, 64 * 64);
   Y_.reset(ctx, 64 * 64);
   Yprev_.reset(ctx, 64 * 64);
+  Ynext_.reset(ctx, 64 * 64);
   theta_.reset(ctx, 64);
   X_.reset(ctx, (size_t)N_ * nev_);
   AX_.reset(ctx, (size_t)N_ * nev_);
-  R_.reset(ctx, (size_t)N_ * nev_);
-  T_.reset(ctx, (size_t)N_ * nev_);
+  RT_.reset(ctx, (size_t)N_ * nev_ * 2);  // residuals, then corrections (one Gram covers both)
   tmp_.reset(ctx, (size_t)N_ * ninit_);
   small_.reset(ctx, 64 * 64);
-  nchunk_ = (int)std::max<int64_t>(1, std::min<int64_t>(512, N_ / 4096));
-  part_.reset(ctx, (size_t)nchunk_ * 64 * 64);
+  gram0_.reset(ctx, 64 * 64);
+  rgram_.reset(ctx, 64 * 64);
+  mbuf_.reset(ctx, 64 * 64);
+  nk_dev_.reset(ctx, 2);
+  status_dev_.reset(ctx, 130);
+  PQG_HIP(hipHostMalloc((void**)&status_pin_.p, 130 * 8, hipHostMallocDefault));
+  status_h_ = status_pin_.p;
+  // partials of the tall-skinny products: at most ceil(N / 512) row chunks
+  nchunk_ = (int)((N_ + 511) / 512);
+  part_.reset(ctx, (size_t)std::min<int64_t>(nchunk_, 1024) * 64 * 64);
   if (mixed_) {
     W64_.reset(ctx, (size_t)N_ * nev_);
     c32_.reset(ctx, (size_t)N_ * nev_ > (size_t)N_ * ninit_ ? (size_t)N_ * nev_ : (size_t)N_ * ninit_);
@@ -383,23 +488,34 @@ Davidson::Davidson(pqg_ci* ci, const pqg_dav_params& p) : ci_(ci), p_(p) {
 }
 
 uint64_t Davidson::device_bytes() const {
-  return V_.bytes() + W_.bytes() + X_.bytes() + AX_.bytes() + R_.bytes() + T_.bytes() + tmp_.bytes() + W64_.bytes() +
+  return V_.bytes() + W_.bytes() + X_.bytes() + AX_.bytes() + RT_.bytes() + tmp_.bytes() + W64_.bytes() +
          c32_.bytes() + s32_.bytes() + part_.bytes();
 }
 
 void Davidson::ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, const double* B, int64_t ldb, int nb,
                           double* out, int ldo) {
-  const int64_t chunk = (N_ + nchunk_ - 1) / nchunk_;
-  dim3 g(nchunk_, (na + 7) / 8, (nb + 3) / 4);
+  // rows per block: about 1024 blocks in flight, at least 512 rows each
+  const int64_t blocks_per_col = std::max<int64_t>(1, 1024 / std::max(1, na * ((nb + 3) / 4)));
+  const int64_t chunk = std::max<int64_t>(512, (N_ + blocks_per_col - 1) / blocks_per_col);
+  const int nchunk = (int)((N_ + chunk - 1) / chunk);
+  PQG_REQUIRE((size_t)nchunk * na * nb <= part_.n, PQG_ERR_ARG, "tall-skinny partial buffer too small");
+  dim3 g(nchunk, na, (nb + 3) / 4);
   k_tn_partial<<<g, kTB, 0, st>>>(A, lda, na, B, ldb, nb, N_, chunk, part_.get());
   PQG_HIP(hipGetLastError());
-  k_tn_reduce<<<g1(na * nb, 256), 256, 0, st>>>(part_.get(), nchunk_, na, nb, out, ldo);
+  k_tn_reduce<<<na * nb, 256, 0, st>>>(part_.get(), nchunk, na, nb, out, ldo);
   PQG_HIP(hipGetLastError());
 }
 
 void Davidson::ts_gemm_nn(hipStream_t st, const double* A, int64_t lda, int ka, const double* Bs, int ldb, int nb,
                           double* out, int64_t ldo, double alpha, double beta) {
-  k_nn<<<g1(N_, kTB), kTB, 0, st>>>(A, lda, ka, Bs, ldb, nb, out, ldo, N_, alpha, beta);
+  k_nn<<<g1(N_, kTB), kTB, 0, st>>>(A, A, lda, ka, Bs, ldb, nb, out, out, ldo, N_, alpha, beta);
+  PQG_HIP(hipGetLastError());
+}
+
+void Davidson::ts_gemm_nn2(hipStream_t st, const double* A0, const double* A1, int64_t lda, int ka, const double* Bs,
+                           int ldb, int nb, double* out0, double* out1, int64_t ldo) {
+  dim3 g(g1(N_, kTB), 2);
+  k_nn<<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, N_, 1.0, 0.0);
   PQG_HIP(hipGetLastError());
 }
 
@@ -444,83 +560,80 @@ void Davidson::run_sigma(hipStream_t st) {
   sigma_calls_ += k;
 }
 
-// Cholesky factor of a small Gram matrix B (n x n, leading dimension 64) with column dropping:
-// column j is kept when its residual norm after the kept columns before it exceeds thr[j].
-// Returns M (n x nk, leading dimension 64) with Q = T M orthonormal, and the kept count.
-static int small_cholqr(const std::vector<double>& B, int n, const std::vector<double>& thr, std::vector<double>& M) {
-  std::vector<int> kept;
-  std::vector<double> R(64 * 64, 0.0);  // R[i + j*64], i <= j, over original column indices
-  for (int j = 0; j < n; j++) {
-    double d = B[j + j * 64];
-    for (size_t a = 0; a < kept.size(); a++) {
-      const int i = kept[a];
-      double v = B[i + j * 64];
-      for (size_t b = 0; b < a; b++) v -= R[kept[b] + i * 64] * R[kept[b] + j * 64];
-      R[i + j * 64] = v / R[i + i * 64];
-      d -= R[i + j * 64] * R[i + j * 64];
-    }
-    if (d > thr[j] * thr[j] && d > 0.0) {
-      R[j + j * 64] = std::sqrt(d);
-      kept.push_back(j);
-    }
-  }
-  const int nk = (int)kept.size();
-  // inverse of the kept upper-triangular factor, placed on the kept rows of M
-  std::vector<double> Rk(64 * 64, 0.0), Ri(64 * 64, 0.0);
-  for (int a = 0; a < nk; a++)
-    for (int b = a; b < nk; b++) Rk[a + b * 64] = R[kept[a] + kept[b] * 64];
-  for (int b = 0; b < nk; b++) {
-    Ri[b + b * 64] = 1.0 / Rk[b + b * 64];
-    for (int a = b - 1; a >= 0; a--) {
-      double v = 0.0;
-      for (int c = a + 1; c <= b; c++) v += Rk[a + c * 64] * Ri[c + b * 64];
-      Ri[a + b * 64] = -v / Rk[a + a * 64];
-    }
-  }
-  M.assign(64 * 64, 0.0);
-  for (int a = 0; a < nk; a++)
-    for (int b = 0; b < nk; b++) M[kept[a] + b * 64] = Ri[a + b * 64];
-  return nk;
-}
-
 // Orthogonalise T (nt columns) against V (m columns) with two block Gram-Schmidt sweeps, then
-// orthonormalise the block by two Cholesky-QR passes (columns that vanish are dropped), and
-// append it to V.  Returns how many columns were appended.
-int Davidson::orthonormalize_append(hipStream_t st, double* T, int nt) {
+// orthonormalise the block by two Cholesky-QR passes (columns that vanish are dropped) into
+// V[:, m:m+nt], all on the stream; the kept count (and the second pass's count, which must agree)
+// lands in nk_dev_[0..1].  Nothing is read back here; finish_append() commits the columns.
+void Davidson::append_async(hipStream_t st, double* T, int nt, const double* g0) {
   nt = std::min(nt, nmax_ - m_);
-  if (nt <= 0) return 0;
-  std::vector<double> g(64 * 64), M;
-  ts_gemm_tn(st, T, N_, nt, T, N_, nt, small_.get(), 64);
-  PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
-  PQG_HIP(hipStreamSynchronize(st));
-  std::vector<double> thr(nt);
-  for (int j = 0; j < nt; j++) thr[j] = std::max(1e-10 * std::sqrt(std::max(g[j + j * 64], 0.0)), 1e-30);
+  pending_nt_ = nt;
+  if (nt <= 0) return;
+  if (!g0) {
+    ts_gemm_tn(st, T, N_, nt, T, N_, nt, gram0_.get(), 64);
+    g0 = gram0_.get();
+  }
   for (int pass = 0; pass < 2; pass++) {
     ts_gemm_tn(st, V_.get(), N_, m_, T, N_, nt, small_.get(), 64);
     ts_gemm_nn(st, V_.get(), N_, m_, small_.get(), 64, nt, T, N_, -1.0, 1.0);
   }
   // pass 1: drop vanishing columns, orthonormalise the rest straight into V[:, m:]
   ts_gemm_tn(st, T, N_, nt, T, N_, nt, small_.get(), 64);
-  PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
-  PQG_HIP(hipStreamSynchronize(st));
-  const int nk = small_cholqr(g, nt, thr, M);
-  if (nk == 0) return 0;
+  k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nullptr, nt, g0, 1e-10, mbuf_.get(), nk_dev_.get());
+  PQG_HIP(hipGetLastError());
   double* dst = V_.get() + (int64_t)m_ * N_;
-  PQG_HIP(hipMemcpyAsync(small_.get(), M.data(), 64 * 64 * 8, hipMemcpyHostToDevice, st));
-  ts_gemm_nn(st, T, N_, nt, small_.get(), 64, nk, dst, N_, 1.0, 0.0);
+  ts_gemm_nn(st, T, N_, nt, mbuf_.get(), 64, nt, dst, N_, 1.0, 0.0);
   // pass 2 (CholQR2): re-orthonormalise the new block
-  ts_gemm_tn(st, dst, N_, nk, dst, N_, nk, small_.get(), 64);
-  PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
-  PQG_HIP(hipStreamSynchronize(st));
-  std::vector<double> thr2(nk, 0.0);
-  const int nk2 = small_cholqr(g, nk, thr2, M);
-  if (nk2 != nk) return 0;  // numerically rank deficient: do not extend
-  PQG_HIP(hipMemcpyAsync(small_.get(), M.data(), 64 * 64 * 8, hipMemcpyHostToDevice, st));
-  ts_gemm_nn(st, dst, N_, nk, small_.get(), 64, nk, tmp_.get(), N_, 1.0, 0.0);
-  PQG_HIP(hipMemcpyAsync(dst, tmp_.get(), (size_t)N_ * nk * 8, hipMemcpyDeviceToDevice, st));
-  mask_invalid<double>(ci_, dst, N_, nk, st);
+  ts_gemm_tn(st, dst, N_, nt, dst, N_, nt, small_.get(), 64);
+  k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nk_dev_.get(), 0, nullptr, 0.0, mbuf_.get(), nk_dev_.get() + 1);
+  PQG_HIP(hipGetLastError());
+  if (nt <= 8) {  // in place: each row is read before it is written
+    ts_gemm_nn(st, dst, N_, nt, mbuf_.get(), 64, nt, dst, N_, 1.0, 0.0);
+  } else {
+    ts_gemm_nn(st, dst, N_, nt, mbuf_.get(), 64, nt, tmp_.get(), N_, 1.0, 0.0);
+    PQG_HIP(hipMemcpyAsync(dst, tmp_.get(), (size_t)N_ * nt * 8, hipMemcpyDeviceToDevice, st));
+  }
+  mask_invalid<double>(ci_, dst, N_, nt, st);
+}
+
+// Commits the block appended by append_async (status_h_ must hold the counts): returns how many
+// columns were added.
+int Davidson::finish_append() {
+  if (pending_nt_ <= 0) return 0;
+  pending_nt_ = 0;
+  const int nk = (int)status_h_[128], nk2 = (int)status_h_[129];
+  if (nk <= 0 || nk2 != nk) return 0;  // numerically rank deficient: do not extend
   m_ += nk;
   return nk;
+}
+
+// Reads residual norms, Ritz values and the appended count in one transfer (one sync).
+void Davidson::read_status(hipStream_t st, const double* rgram) {
+  k_pack_status<<<1, 64, 0, st>>>(rgram, theta_.get(), nk_dev_.get(), nev_, status_dev_.get());
+  PQG_HIP(hipGetLastError());
+  PQG_HIP(hipMemcpyAsync(status_h_, status_dev_.get(), 130 * 8, hipMemcpyDeviceToHost, st));
+  PQG_HIP(hipStreamSynchronize(st));
+}
+
+int Davidson::orthonormalize_append(hipStream_t st, double* T, int nt) {
+  append_async(st, T, nt, nullptr);
+  k_pack_status<<<1, 64, 0, st>>>(small_.get(), theta_.get(), nk_dev_.get(), 0, status_dev_.get());
+  PQG_HIP(hipGetLastError());
+  PQG_HIP(hipMemcpyAsync(status_h_, status_dev_.get(), 130 * 8, hipMemcpyDeviceToHost, st));
+  PQG_HIP(hipStreamSynchronize(st));
+  return finish_append();
+}
+
+// Thick restart on the first `keep` Ritz vectors of the current subspace (Y from the last
+// Rayleigh-Ritz); the projected matrix is recomputed exactly.
+void Davidson::restart_to_ritz(hipStream_t st, int keep) {
+  ts_gemm_nn(st, V_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
+  PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
+  ts_gemm_nn(st, W_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
+  PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
+  m_ = nW_ = keep;
+  warm_ = 0;
+  ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, G_.get(), 64);
+  restarts_++;
 }
 
 void Davidson::rayleigh_ritz(hipStream_t st) {
@@ -539,24 +652,12 @@ void Davidson::rayleigh_ritz(hipStream_t st) {
     }
     nW_ = m_;
   }
-  k_symmetrize<<<1, 256, 0, st>>>(G_.get(), old, m_, gt);
+  // Yprev_ (warm start) and Y_ are distinct buffers: the kernel reads one and writes both
+  k_jacobi<<<1, 1024, 0, st>>>(G_.get(), m_, old, gt, Yprev_.get(), std::min(warm_, m_), theta_.get(), Y_.get(),
+                               Ynext_.get());
   PQG_HIP(hipGetLastError());
-  k_jacobi<<<1, 1024, 0, st>>>(G_.get(), 64, m_, Yprev_.get(), std::min(warm_, m_), theta_.get(), Y_.get(), 64);
-  PQG_HIP(hipGetLastError());
-  PQG_HIP(hipMemcpyAsync(Yprev_.get(), Y_.get(), 64 * 64 * 8, hipMemcpyDeviceToDevice, st));
+  std::swap(Yprev_, Ynext_);
   warm_ = m_;
-}
-
-void Davidson::restart(hipStream_t st, const double* Vk, const double* Wk, int keep) {
-  // V[:, :keep] = Vk, W[:, :keep] = Wk; projected matrix becomes diag(theta)
-  PQG_HIP(hipMemcpyAsync(V_.get(), Vk, (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
-  PQG_HIP(hipMemcpyAsync(W_.get(), Wk, (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
-  m_ = nW_ = 0;
-  // recompute the keep x keep projection exactly
-  ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, G_.get(), 64);
-  m_ = nW_ = keep;
-  warm_ = 0;
-  restarts_++;
 }
 
 void Davidson::post_sigma(hipStream_t st) {
@@ -573,10 +674,10 @@ void Davidson::post_sigma(hipStream_t st) {
       for (int k = 0; k < nev_; k++) theta_h_[k] = q[k + k * 64];
       PQG_HIP(hipMemcpyAsync(theta_.get(), theta_h_.data(), nev_ * 8, hipMemcpyHostToDevice, st));
     }
-    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), W64_.get(), theta_.get(), ci_->hdiag.get(), R_.get(), T_.get(),
+    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), W64_.get(), theta_.get(), ci_->hdiag.get(), Rp(), Tp(),
                                             N_, nev_);
     PQG_HIP(hipGetLastError());
-    ts_gemm_tn(st, R_.get(), N_, nev_, R_.get(), N_, nev_, small_.get(), 64);
+    ts_gemm_tn(st, Rp(), N_, nev_, Rp(), N_, nev_, small_.get(), 64);
     std::vector<double> g(64 * 64);
     PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
     PQG_HIP(hipStreamSynchronize(st));
@@ -631,7 +732,7 @@ void Davidson::post_sigma(hipStream_t st) {
     }
     restarts_++;
     const int before = m_;
-    orthonormalize_append(st, T_.get(), nev_);
+    orthonormalize_append(st, Tp(), nev_);
     work_ = SigmaWork{m_ - before, PQG_MIXED, V_.get() + (int64_t)before * N_, W_.get() + (int64_t)before * N_, N_};
     if (work_.k == 0) {  // nothing new to add: accept
       done_ = true;
@@ -642,30 +743,39 @@ void Davidson::post_sigma(hipStream_t st) {
   iter_++;
   rayleigh_ritz(st);
   // Ritz vectors and residuals
-  ts_gemm_nn(st, V_.get(), N_, m_, Y_.get(), 64, nev_, X_.get(), N_, 1.0, 0.0);
-  ts_gemm_nn(st, W_.get(), N_, m_, Y_.get(), 64, nev_, AX_.get(), N_, 1.0, 0.0);
-  k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), ci_->hdiag.get(), R_.get(), T_.get(), N_,
+  ts_gemm_nn2(st, V_.get(), W_.get(), N_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), N_);
+  k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), ci_->hdiag.get(), Rp(), Tp(), N_,
                                           nev_);
   PQG_HIP(hipGetLastError());
-  ts_gemm_tn(st, R_.get(), N_, nev_, R_.get(), N_, nev_, small_.get(), 64);
-  std::vector<double> g(64 * 64);
-  theta_h_.resize(64);
-  rn_h_.resize(nev_);
-  PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
-  PQG_HIP(hipMemcpyAsync(theta_h_.data(), theta_.get(), 64 * 8, hipMemcpyDeviceToHost, st));
-  PQG_HIP(hipStreamSynchronize(st));
-  theta_h_.resize(nev_);
-  double worst = 0.0;
-  for (int k = 0; k < nev_; k++) {
-    rn_h_[k] = std::sqrt(std::max(g[k + k * 64], 0.0));
-    worst = std::max(worst, rn_h_[k]);
+  // one Gram of [R T]: R^T R (residual norms) and T^T T (drop thresholds of the corrections)
+  const double* g0 = nullptr;
+  if (2 * nev_ <= 64) {
+    ts_gemm_tn(st, Rp(), N_, 2 * nev_, Rp(), N_, 2 * nev_, rgram_.get(), 64);
+    g0 = rgram_.get() + nev_ + nev_ * 64;
+  } else {
+    ts_gemm_tn(st, Rp(), N_, nev_, Rp(), N_, nev_, rgram_.get(), 64);
   }
+  // The next block is prepared before the convergence test is read back, so an iteration costs
+  // one host synchronisation.  A mixed-precision solve may still need the current subspace for
+  // its FP64 check, so it does not restart ahead of the test.
+  const bool need_restart = m_ + nev_ > nmax_;
+  const bool speculate = !(mixed_ && need_restart);
+  if (speculate) {
+    if (need_restart) restart_to_ritz(st, ninit_);
+    append_async(st, Tp(), nev_, g0);
+  }
+  read_status(st, rgram_.get());
+  theta_h_.assign(status_h_ + 64, status_h_ + 64 + nev_);
+  rn_h_.assign(status_h_, status_h_ + nev_);
+  double worst = 0.0;
+  for (int k = 0; k < nev_; k++) worst = std::max(worst, rn_h_[k]);
   dav_log(mixed_ ? "fp32" : "fp64", iter_, m_, p_.tol, theta_h_, rn_h_);
   since_check_++;
   const double thr = std::max(p_.tol, std::min(1e-3, 0.05 * last_true_));
   const bool approx_conv = mixed_ ? (worst < thr || since_check_ >= 12) : (worst < p_.tol);
   if (approx_conv && mixed_) {
-    // FP64 check of the Ritz vectors
+    // FP64 check of the Ritz vectors (the speculative block is dropped: m_ is unchanged)
+    pending_nt_ = 0;
     checking_ = true;
     work_ = SigmaWork{nev_, PQG_FP64, X_.get(), W64_.get(), N_};
     return;
@@ -677,21 +787,14 @@ void Davidson::post_sigma(hipStream_t st) {
     seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
     return;
   }
-  if (m_ + nev_ > nmax_) {
-    const int keep = ninit_;
-    ts_gemm_nn(st, V_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
-    PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
-    ts_gemm_nn(st, W_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
-    PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
-    m_ = nW_ = keep;
-    warm_ = 0;
-    // projection after restart = diag(theta[:keep]) up to rounding; recompute exactly
-    ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, G_.get(), 64);
-    restarts_++;
+  int added;
+  const int before = need_restart ? ninit_ : m_;
+  if (speculate) {
+    added = finish_append();
+  } else {
+    restart_to_ritz(st, ninit_);
+    added = orthonormalize_append(st, Tp(), nev_);
   }
-  const int before = m_;
-  orthonormalize_append(st, T_.get(), nev_);
-  const int added = m_ - before;
   if (added == 0) {  // subspace exhausted
     done_ = true;
     status_ = worst < p_.tol ? PQG_OK : PQG_ERR_NOCONV;

@@ -8,6 +8,17 @@
 
 namespace pqg {
 
+// Page-locked host buffer (freed with the owner).
+struct PinnedHost {
+  double* p = nullptr;
+  PinnedHost() = default;
+  PinnedHost(const PinnedHost&) = delete;
+  PinnedHost& operator=(const PinnedHost&) = delete;
+  ~PinnedHost() {
+    if (p) (void)hipHostFree(p);
+  }
+};
+
 // A block of vectors whose sigma the solver needs next.
 struct SigmaWork {
   int k = 0;               // number of vectors (0: nothing pending)
@@ -37,21 +48,32 @@ class Davidson {
 
  private:
   void initial_guess(hipStream_t st);
-  void add_corrections(hipStream_t st, const double* R);
   int orthonormalize_append(hipStream_t st, double* T, int nt);
+  void append_async(hipStream_t st, double* T, int nt, const double* g0);
+  int finish_append();
+  void read_status(hipStream_t st, const double* rgram);
+  void restart_to_ritz(hipStream_t st, int keep);
   void rayleigh_ritz(hipStream_t st);
-  void restart(hipStream_t st, const double* Vk, const double* Wk, int keep);
   void ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, const double* B, int64_t ldb, int nb,
                   double* out, int ldo);
   void ts_gemm_nn(hipStream_t st, const double* A, int64_t lda, int ka, const double* Bs, int ldb, int nb,
                   double* out, int64_t ldo, double alpha, double beta);
+  void ts_gemm_nn2(hipStream_t st, const double* A0, const double* A1, int64_t lda, int ka, const double* Bs, int ldb,
+                   int nb, double* out0, double* out1, int64_t ldo);
+  double* Rp() const { return RT_.get(); }
+  double* Tp() const { return RT_.get() + (size_t)N_ * nev_; }
 
   pqg_ci* ci_;
   pqg_dav_params p_;
   int64_t N_;
   int nev_, ninit_, nmax_;
   bool mixed_;
-  DevBuf<double> V_, W_, G_, Y_, Yprev_, theta_, X_, AX_, R_, T_, tmp_, part_, small_, W64_;
+  DevBuf<double> V_, W_, G_, Y_, Yprev_, Ynext_, theta_, X_, AX_, RT_, tmp_, part_, small_, W64_;
+  DevBuf<double> gram0_, rgram_, mbuf_, status_dev_;
+  DevBuf<int> nk_dev_;
+  PinnedHost status_pin_;        // pinned host copy of status_dev_
+  double* status_h_ = nullptr;
+  int pending_nt_ = 0;
   DevBuf<float> c32_, s32_;
   int m_ = 0, nW_ = 0, iter_ = 0, restarts_ = 0, since_check_ = 0, warm_ = 0;
   int sigma_calls_ = 0, sigma_calls64_ = 0;
