@@ -156,9 +156,10 @@ pqg_status pqg_serve_destroy(pqg_server* srv);
 /* Queue one request (copies desc and params).  Returns its id.                           */
 pqg_status pqg_serve_submit(pqg_server* srv, const pqg_ci_desc* desc, const pqg_dav_params* params,
                             int64_t* request_id);
-/* Advance the loop by one scheduling step: admit queued requests whose memory fits, run one
- * Davidson iteration of every resident solve (their sigma builds fused into one launch),
- * retire finished ones.  Returns the number of results written (<= cap).                 */
+/* Collect finished requests.  Admission and the solves themselves run inside the server:
+ * each resident slot has a worker thread and its own HIP stream, so the solves of different
+ * slots overlap on the device; a freed slot admits the next queued request whose memory fits.
+ * Returns the number of results written (<= cap).                                        */
 pqg_status pqg_serve_step(pqg_server* srv, pqg_result* results, int cap, int* nresults);
 pqg_status pqg_serve_stats(pqg_server* srv, int* queued, int* resident, uint64_t* bytes_resident);
 double pqg_serve_clock(pqg_server* srv);

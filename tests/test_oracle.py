@@ -11,7 +11,9 @@
 * AO->MO transform against the closed form for Cholesky-built AO integrals.
 """
 import itertools
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -23,78 +25,37 @@ from tests import bruteforce
 EXTREMELYTIGHT = 1e-14  # src/io/utils.hpp:24
 
 
-def b(s):
-    return int(s, 2)
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ci_known_answers.json")))
 
 
-# --- ci_test.cpp:213-257 ---------------------------------------------------------------
-def test_get_holes_known_answers():
-    hf = [b("0011111")]
-    assert refcpu.get_holes(hf, [b("0101111")]) == [4]
-    assert refcpu.get_holes(hf, [b("1101101")]) == [1, 4]
-    big_hf = [b("0001111111"), 0]
-    big_dx = [b("0110111101"), 0]
-    assert refcpu.get_holes(big_hf, big_dx) == [64 + 1, 64 + 6]
-    big_hf[1], big_dx[1] = 1, 32
-    assert refcpu.get_holes(big_hf, big_dx) == [0, 64 + 1, 64 + 6]
-    assert refcpu.get_holes([7, 127], [519, 63]) == [6]
+@pytest.mark.parametrize("case", GOLDEN["holes"], ids=lambda c: c["src"])
+def test_get_holes_known_answers(case):
+    assert refcpu.get_holes(case["di"], case["dj"]) == case["expect"]
 
 
-# --- ci_test.cpp:259-302 ---------------------------------------------------------------
-def test_get_parts_known_answers():
-    hf = [b("0011111")]
-    assert refcpu.get_parts(hf, [b("0101111")]) == [5]
-    assert refcpu.get_parts(hf, [b("1101101")]) == [5, 6]
-    big_hf = [b("0001111111"), 0]
-    big_dx = [b("0110111101"), 0]
-    assert refcpu.get_parts(big_hf, big_dx) == [64 + 7, 64 + 8]
-    big_hf[1], big_dx[1] = 1, 32
-    assert refcpu.get_parts(big_hf, big_dx) == [5, 64 + 7, 64 + 8]
-    assert refcpu.get_parts([7, 127], [519, 63]) == [73]
+@pytest.mark.parametrize("case", GOLDEN["parts"], ids=lambda c: c["src"])
+def test_get_parts_known_answers(case):
+    assert refcpu.get_parts(case["di"], case["dj"]) == case["expect"]
 
 
 def _phase(di, dj):
     return refcpu.get_phase(di, refcpu.get_holes(di, dj), refcpu.get_parts(di, dj))
 
 
-# --- ci_test.cpp:304-370 ---------------------------------------------------------------
-def test_get_phase_known_answers():
-    hf, sx, dx = [b("0011111")], [b("0101111")], [b("1101101")]
-    assert _phase(hf, sx) == 1.0
-    assert _phase(hf, dx) == 1.0
-    assert _phase(dx, sx) == -1.0
-    assert _phase([0, hf[0]], [0, sx[0]]) == 1.0
-    assert _phase([7, 127], [519, 63]) == -1.0
-    # the "Be bug" regression pair
-    assert _phase([0, 1], [0, 16]) == 1.0
-    assert _phase([0, 17179869184], [4, 0]) == 1.0
+@pytest.mark.parametrize("case", GOLDEN["phase"], ids=lambda c: c["src"])
+def test_get_phase_known_answers(case):
+    assert _phase(case["di"], case["dj"]) == case["expect"]
 
 
-# --- ci_test.cpp:371-482 ---------------------------------------------------------------
-def test_get_occ_virt_known_answers():
-    occ, virt = refcpu.get_occ_virt([b("0011111")], 7)
-    assert occ == list(range(5)) and virt == [5, 6]
-    occ, virt = refcpu.get_occ_virt([b("0001111111"), 0], 64 + 10)
-    assert occ == [64 + i for i in range(7)]
-    assert virt == list(range(64)) + [64 + 7, 64 + 8, 64 + 9]
-    occ, virt = refcpu.get_occ_virt([b("0110111101"), 0], 64 + 10)
-    assert occ == [64, 66, 67, 68, 69, 71, 72]
-    assert virt == list(range(64)) + [65, 70, 73]
-    occ, virt = refcpu.get_occ_virt([7, 127], 100)
-    assert occ == list(range(7)) + [64, 65, 66]
-    assert virt == list(range(7, 64)) + list(range(67, 100))
-    occ, virt = refcpu.get_occ_virt([519, 63], 100)
-    assert occ == list(range(6)) + [64, 65, 66, 73]
-    assert len(virt) == 90 and virt[:58] == list(range(6, 64))
-    assert virt[58:] == [i for i in range(67, 100) if i != 73]
+@pytest.mark.parametrize("case", GOLDEN["occ_virt"], ids=lambda c: c["src"])
+def test_get_occ_virt_known_answers(case):
+    occ, virt = refcpu.get_occ_virt(case["d"], case["max_orb"])
+    assert occ == case["occ"] and virt == case["virt"]
 
 
-def test_idx2_known_answers():
-    # tests/unit_tests/integral/integral_test.cpp:10-14
-    assert wl.idx2(0, 0) == 0
-    assert wl.idx2(1, 0) == 1 and wl.idx2(0, 1) == 1
-    assert wl.idx2(1, 1) == 2
-    assert wl.idx2(3, 2) == wl.idx2(2, 3) == 8
+@pytest.mark.parametrize("case", GOLDEN["idx2"], ids=lambda c: c["src"])
+def test_idx2_known_answers(case):
+    assert wl.idx2(case["i"], case["j"]) == case["expect"]
 
 
 def test_phase_matches_permutation_sign_bruteforce():
@@ -136,8 +97,8 @@ def _cis_problem():
 def test_detset_construction_h2o_cis_layout():
     oc = refcpu.OracleCI(_cis_problem())
     assert oc.N == 21  # ci_test.cpp:83-84
-    assert int(oc.strings(0, 0)[0]) == b("0011111")
-    assert int(oc.strings(0, 1)[0]) == b("0011111")
+    assert int(oc.strings(0, 0)[0]) == GOLDEN["detset_h2o_sto3g_cis"]["hf_string"]
+    assert int(oc.strings(0, 1)[0]) == GOLDEN["detset_h2o_sto3g_cis"]["hf_string"]
     assert len(oc.strings(0, 0)) == 11
     # ci_test.cpp:566-575: first 11 addresses are (0, i), then (i - 10, 0)
     U = oc.unfold_all()
