@@ -80,6 +80,96 @@ def traffic_from_profiles(kernel):
     return float(m.group(1)) if m else None
 
 
+def bench_ao2mo(args):
+    """AO->MO transform, n_ao = n_mo = 320, 1280 Cholesky vectors (one JSON line)."""
+    import numpy as np
+    import torch
+
+    from polyquant_amd import workloads as wl
+    from polyquant_amd.ci import Context
+    from polyquant_amd.integral import BF16X3, FP64, transform_device
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    n, nvec = args.ao_n, 1280
+    npr = n * (n + 1) // 2
+    t0 = time.perf_counter()
+    L, C = wl.config4(n=n, nvec=nvec)
+    log(f"[ao2mo] n={n} npair={npr} inputs generated in {time.perf_counter() - t0:.1f}s")
+    dev = torch.device("cuda")
+    Lg = torch.from_numpy(L).to(dev)
+    del L
+    # 8-fold packed AO integrals: eri8[idx2(P, Q)] = sum_k L[k, P] L[k, Q], rows P in blocks
+    eri8 = torch.empty(npr * (npr + 1) // 2, dtype=torch.float64, device=dev)
+    blk = 2048
+    for p0 in range(0, npr, blk):
+        p1 = min(npr, p0 + blk)
+        M = Lg[:, p0:p1].T.contiguous() @ Lg[:, :p1]
+        mask = torch.arange(p1, device=dev)[None, :] <= torch.arange(p0, p1, device=dev)[:, None]
+        eri8[p0 * (p0 + 1) // 2: p1 * (p1 + 1) // 2] = M[mask]
+        del M, mask
+    Cg = torch.from_numpy(np.ascontiguousarray(C.ravel(order="F"))).to(dev)
+    out64 = torch.empty(npr * npr, dtype=torch.float64, device=dev)
+    out3 = torch.empty(npr * npr, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    log(f"[ao2mo] AO integrals built on the GPU ({eri8.numel() * 8 / 1e9:.2f} GB 8-fold packed)")
+    ctx = Context(0, hbm_budget=int(args.ao_budget_gb * (1 << 30)))
+    f_alg = 8.0 * n ** 3 * npr
+    res = {}
+    for mode, name, out, peak in ((FP64, "fp64", out64, 78.6), (BF16X3, "bf16x3", out3, 2500.0 / 3)):
+        gemm = "ao2mo_gemm_f64" if mode == FP64 else "ao2mo_gemm_bf16x3"
+        times = []
+        for rep in range(args.ao_reps):
+            ctx.profile(True)
+            ctx.profile_read(gemm)
+            sec = transform_device(ctx, n, eri8.data_ptr(), Cg.data_ptr(), out.data_ptr(), mode)
+            ctx.profile(False)
+            launches, gemm_ms, units = ctx.profile_read(gemm)
+            times.append((sec, gemm_ms, launches, units))
+            log(f"[ao2mo] {name} rep {rep}: {sec * 1e3:.1f} ms total, GEMM launches {launches} {gemm_ms:.1f} ms")
+        sec, gemm_ms, launches, units = min(times)
+        res[name] = {"seconds": round(sec, 5), "tflops": round(f_alg / sec / 1e12, 3),
+                     "frac_of_peak": round(f_alg / sec / 1e12 / peak, 4), "peak_tflops": round(peak, 1),
+                     "gemm_ms": round(gemm_ms, 3), "gemm_launches": launches,
+                     "gemm_tflops": round(units / max(gemm_ms * 1e-3, 1e-30) / 1e12, 3)}
+    torch.cuda.synchronize()
+    # BF16x3 vs FP64 on every element
+    dmax, amax = 0.0, 0.0
+    step = 1 << 28
+    for i in range(0, npr * npr, step):
+        a, b = out64[i:i + step], out3[i:i + step]
+        dmax = max(dmax, float((a - b).abs().max()))
+        amax = max(amax, float(a.abs().max()))
+    # FP64 result against the closed form on sampled columns: (pq|rs) = sum_k (C^T L_k C)_pq (C^T L_k C)_rs
+    P = torch.from_numpy(wl.pair_index_matrix(n).astype(np.int64)).to(dev)
+    Cm = Cg.reshape(n, n).T.contiguous()  # row-major C[mu][i]
+    ii, jj = np.tril_indices(n)
+    flat = torch.from_numpy((ii * n + jj).astype(np.int64)).to(dev)
+    Lmo = torch.empty(nvec, npr, dtype=torch.float64, device=dev)
+    for k0 in range(0, nvec, 128):
+        Lk = Lg[k0:k0 + 128][:, P.reshape(-1)].reshape(-1, n, n)
+        Mk = torch.matmul(torch.matmul(Cm.T.contiguous(), Lk), Cm)
+        Lmo[k0:k0 + 128] = Mk.reshape(-1, n * n)[:, flat]
+    sample = torch.from_numpy(np.random.default_rng(4).choice(npr, 256, replace=False)).to(dev)
+    ref = Lmo[:, sample].T.contiguous() @ Lmo
+    got = torch.stack([out64[int(q) * npr:(int(q) + 1) * npr] for q in sample.tolist()])
+    cf_err = float((ref - got).abs().max())
+    out = {
+        "metric": "AO->MO four-index transform time (FP64 MFMA vs BF16x3 MFMA)",
+        "value": res["fp64"]["seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"ao2mo_{n}: n_ao = n_mo = {n}, {nvec} Cholesky vectors N(0, 0.05^2), "
+                               f"8-fold packed AO input, 4-fold packed MO output", "npair": npr,
+                   "alg_flops": f_alg},
+        "fp64": res["fp64"], "bf16x3": res["bf16x3"],
+        "bf16x3_max_abs_err_vs_fp64": dmax, "max_abs_mo_integral": amax,
+        "fp64_max_abs_err_vs_closed_form_256_columns": cf_err,
+        "roofline": {"bound": "mfma", "achieved": res["fp64"]["tflops"], "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": res["fp64"]["frac_of_peak"]},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +177,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--workload", default="davidson", choices=["davidson", "ao2mo"],
+                    help="davidson (default): ep-FCI 48/48/48 Davidson solves; ao2mo: n = 320 transform")
+    ap.add_argument("--ao-n", type=int, default=320)
+    ap.add_argument("--ao-reps", type=int, default=2)
+    ap.add_argument("--ao-budget-gb", type=float, default=120.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 1 GiB device buffer (the known-size kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -101,6 +196,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": x.numel() * 8, "launches": 3}))
+        return
+    if args.workload == "ao2mo":
+        bench_ao2mo(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the CI path is built and "

@@ -212,7 +212,8 @@ static thread_local pqg_ctx* g_prof_ctx = nullptr;  // context of the running tr
 template <typename T>
 static void bgemm(const T* A, const T* Cm, T* out, int n, int nb, bool transA, hipStream_t st) {
   dim3 g(gr(n, BN), gr(n, BM), nb);
-  ProfScope prof(g_prof_ctx, st, std::is_same<T, double>::value ? "ao2mo_gemm_f64" : "ao2mo_gemm_bf16x3", nb);
+  ProfScope prof(g_prof_ctx, st, std::is_same<T, double>::value ? "ao2mo_gemm_f64" : "ao2mo_gemm_bf16x3",
+                 2ll * n * n * n * nb);  // units: flops of the batch
   if constexpr (std::is_same<T, double>::value) {
     if (transA)
       k_bgemm_f64<1><<<g, 256, 0, st>>>(A, Cm, out, n);

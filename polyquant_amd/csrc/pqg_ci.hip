@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <unordered_map>
 
 #include "pqg_ci.hpp"
@@ -228,7 +229,8 @@ __global__ void k_hdiag(DiagArgs a, double* __restrict__ Hd) {
 // ---------------------------------------------------------------------------------------
 // sigma pass kernel
 // ---------------------------------------------------------------------------------------
-constexpr int kCPT = 16;  // max columns per thread
+constexpr int kCPT = 16;   // max columns per thread
+constexpr int kGMax = 8;   // excited rows of the row string staged per block (row x column doubles)
 
 enum PassFlags { F_DIAG = 1, F_COLS = 2, F_MIXED = 4, F_ACCUM = 8 };
 
@@ -241,22 +243,97 @@ struct PassArgs {
   int64_t cstride[3];    // column-index stride of each column slot
   int64_t M;             // columns per row
   int tiles, tile_cols, cpt;
+  int G;                 // excited rows staged together (1..kGMax)
   const T* C;
   int64_t ldc;
   T* S;
   int64_t lds;
   const double* Hd;
   int flags;
-  int off_x, off_v, off_rt;  // LDS byte offsets
+  int off_x, off_v, off_rt, off_rl;  // LDS byte offsets
   int x_off[3], v_off[3], rt_off[3];
+  int vstride;                       // elements per staged coupling row (sum of npair of the column slots)
 };
 
 template <int NC>
 constexpr int pass_threads() { return NC == 1 ? 1024 : 256; }
 
+// String bit helpers on a W-bit word (W = uint32_t when every orbital count is <= 32).
+template <typename W>
+__device__ __forceinline__ int ctzw(W x) {
+  if constexpr (sizeof(W) == 4)
+    return __builtin_ctz(x);
+  else
+    return __builtin_ctzll(x);
+}
+template <typename W>
+__device__ __forceinline__ int popw(W x) {
+  if constexpr (sizeof(W) == 4)
+    return __popc(x);
+  else
+    return __popcll(x);
+}
+template <typename W>
+__device__ __forceinline__ W lowmaskw(int n) {
+  return n >= (int)(8 * sizeof(W)) ? ~W(0) : ((W(1) << n) - W(1));
+}
+
+// Visit all single excitations I -> J: (e, Q = idx2(h, p), odd phase, J), e enumerating
+// occupied h (ascending) outer and virtual p (ascending) inner.
+template <typename W, class F>
+__device__ __forceinline__ void for_singles_w(W I, int n_orb, F&& f) {
+  const W vm = ~I & lowmaskw<W>(n_orb);
+  int e = 0;
+  for (W occ = I; occ; occ &= occ - 1) {
+    const int h = ctzw(occ);
+    for (W vir = vm; vir; vir &= vir - 1) {
+      const int p = ctzw(vir);
+      const int lo = min(h, p), hi = max(h, p);
+      // occupied orbitals strictly between h and p
+      const W btw = lowmaskw<W>(hi) & ~lowmaskw<W>(lo + 1);
+      f(e, hi * (hi + 1) / 2 + lo, popw(I & btw) & 1, I ^ ((W(1) << h) | (W(1) << p)));
+      ++e;
+    }
+  }
+}
+
+template <typename W, class F>
+__device__ __forceinline__ void for_doubles_w(W I, int n_orb, F&& f) {
+  const W vm = ~I & lowmaskw<W>(n_orb);
+  int e = 0;
+  for (W o0 = I; o0; o0 &= o0 - 1) {
+    const W b0 = o0 & (~o0 + 1);
+    for (W o1 = o0 & (o0 - 1); o1; o1 &= o1 - 1) {
+      const W hh = b0 | (o1 & (~o1 + 1));
+      for (W v0 = vm; v0; v0 &= v0 - 1) {
+        const W c0 = v0 & (~v0 + 1);
+        for (W v1 = v0 & (v0 - 1); v1; v1 &= v1 - 1) {
+          f(e, I ^ (hh | c0 | (v1 & (~v1 + 1))));
+          ++e;
+        }
+      }
+    }
+  }
+}
+
+// colex rank of J (chunk tables in LDS) -> index in the set, or -1
+template <typename W>
+__device__ __forceinline__ int str_index_w(const SlotDev& s, const uint32_t* tab, W J) {
+  if (s.nocc == 1 && !s.r2i) return ctzw(J);
+  uint32_t r = 0;
+  int e = s.nocc;
+  const int w = s.nocc + 1;
+  for (int c = s.nchunk - 1; c >= 0; --c) {
+    const uint32_t x = (uint32_t)(J >> (8 * c)) & 255u;
+    r += tab[((c << 8) + x) * w + e];
+    e -= __popc(x);
+  }
+  return s.r2i ? s.r2i[r] : (int)r;
+}
+
 // Column strings of column index m.
-template <typename T, int NC>
-__device__ __forceinline__ void decode_cols(const PassArgs<T>& a, int64_t m, int (&ci)[NC], uint64_t (&cb)[NC]) {
+template <typename T, typename W, int NC>
+__device__ __forceinline__ void decode_cols(const PassArgs<T>& a, int64_t m, int (&ci)[NC], W (&cb)[NC]) {
   int64_t r = m;
 #pragma unroll
   for (int c = NC - 1; c >= 0; --c) {
@@ -264,20 +341,20 @@ __device__ __forceinline__ void decode_cols(const PassArgs<T>& a, int64_t m, int
     r /= a.cn[c];
   }
 #pragma unroll
-  for (int c = 0; c < NC; c++) cb[c] = a.sl[a.cs[c]].bits[ci[c]];
+  for (int c = 0; c < NC; c++) cb[c] = (W)a.sl[a.cs[c]].bits[ci[c]];
 }
 
-// Diagonal and every excitation that changes only column strings, gathered from the staged
-// row C(row, :).
-template <typename T, int NC>
+// Diagonal and every excitation that changes only column strings, gathered from the row
+// C(row, :) (crow: staged in LDS or read in place).
+template <typename T, typename W, int NC>
 __device__ __forceinline__ T column_terms(const PassArgs<T>& a, int64_t row, int64_t m, const T* crow, const T* xs,
                                           const uint32_t* rt) {
   T sum = 0;
   if (a.flags & F_DIAG) sum = (T)a.Hd[row * a.M + m] * crow[m];
   if (!(a.flags & F_COLS)) return sum;
   int ci[NC];
-  uint64_t cb[NC];
-  decode_cols<T, NC>(a, m, ci, cb);
+  W cb[NC];
+  decode_cols<T, W, NC>(a, m, ci, cb);
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     const int t = a.cs[c];
@@ -288,9 +365,10 @@ __device__ __forceinline__ T column_terms(const PassArgs<T>& a, int64_t row, int
     const T* xrow = xs + a.x_off[c];
     const int it = ci[c];
     const int64_t cst = a.cstride[c];
+    const T* cbase = crow + m - (int64_t)it * cst;  // crow[m + (jt - it) * cst] = cbase[jt * cst]
     // singles of column string c: s * (A + spectator sums) * C(row, m')
-    for_singles(cb[c], s.n_orb, [&](int e, int, int, int Q, int odd, uint64_t J) {
-      const int jt = str_index(s, tab, J);
+    for_singles_w<W>(cb[c], s.n_orb, [&](int e, int Q, int odd, W J) {
+      const int jt = str_index_w<W>(s, tab, J);
       if (jt < 0) return;
       T v = At[(size_t)e * s.nstr + it] + xrow[Q];
 #pragma unroll
@@ -299,16 +377,16 @@ __device__ __forceinline__ T column_terms(const PassArgs<T>& a, int64_t row, int
         const T* X = a.tb.X[t][a.cs[c2]];
         if (X) v += X[(size_t)ci[c2] * s.npair + Q];
       }
-      const T cv = crow[m + (int64_t)(jt - it) * cst];
+      const T cv = cbase[(int64_t)jt * cst];
       sum += odd ? -v * cv : v * cv;
     });
     // same-string doubles
     if (s.ndouble > 0) {
-      const T* __restrict__ Dt = a.tb.D[t];
-      for_doubles(cb[c], s.n_orb, [&](int e, int, int, int, int, uint64_t J) {
-        const int jt = str_index(s, tab, J);
+      const T* __restrict__ Dt = a.tb.D[t] + it;
+      for_doubles_w<W>(cb[c], s.n_orb, [&](int e, W J) {
+        const int jt = str_index_w<W>(s, tab, J);
         if (jt < 0) return;
-        sum += Dt[(size_t)e * s.nstr + it] * crow[m + (int64_t)(jt - it) * cst];
+        sum += Dt[(size_t)e * s.nstr] * cbase[(int64_t)jt * cst];
       });
     }
     // one single in each of two column strings
@@ -321,17 +399,17 @@ __device__ __forceinline__ T column_terms(const PassArgs<T>& a, int64_t row, int
       const uint32_t* tab2 = rt + a.rt_off[c2];
       const int iu = ci[c2];
       const int64_t cst2 = a.cstride[c2];
-      const uint64_t cb2 = cb[c2];
-      for_singles(cb[c], s.n_orb, [&](int, int, int, int Q, int odd, uint64_t J) {
-        const int jt = str_index(s, tab, J);
+      const W cb2 = cb[c2];
+      for_singles_w<W>(cb[c], s.n_orb, [&](int, int Q, int odd, W J) {
+        const int jt = str_index_w<W>(s, tab, J);
         if (jt < 0) return;
-        const int64_t mj = m + (int64_t)(jt - it) * cst;
+        const T* cb2base = cbase + (int64_t)jt * cst - (int64_t)iu * cst2;
         const T* vrowQ = Vtu + (size_t)Q * s2.npair;
         T part = 0;
-        for_singles(cb2, s2.n_orb, [&](int, int, int, int Q2, int odd2, uint64_t J2) {
-          const int ju = str_index(s2, tab2, J2);
+        for_singles_w<W>(cb2, s2.n_orb, [&](int, int Q2, int odd2, W J2) {
+          const int ju = str_index_w<W>(s2, tab2, J2);
           if (ju < 0) return;
-          const T cv = crow[mj + (int64_t)(ju - iu) * cst2];
+          const T cv = cb2base[(int64_t)ju * cst2];
           const T v = vrowQ[Q2];
           part += odd2 ? -v * cv : v * cv;
         });
@@ -342,14 +420,15 @@ __device__ __forceinline__ T column_terms(const PassArgs<T>& a, int64_t row, int
   return sum;
 }
 
-// For one single excitation of the row string (its coupling row staged in vr and the excited
-// row C(J, :) in crow): sum over single excitations of the column strings.
-template <typename T, int NC>
-__device__ __forceinline__ T mixed_terms(const PassArgs<T>& a, int t0, int64_t m, const T* crow, const T* vr,
-                                         const uint32_t* rt) {
+// For a block of g_n single excitations of the row string (their sign-scaled coupling rows
+// staged in vr, the excited rows C(J_g, :) at crows[g]): sum over single excitations of the
+// column strings.  Each column excitation is enumerated once for all g_n row excitations.
+template <typename T, typename W, int NC>
+__device__ __forceinline__ T mixed_terms(const PassArgs<T>& a, int t0, int64_t m, const T* const (&crows)[kGMax],
+                                         const T* vr, int g_n, const uint32_t* rt) {
   int ci[NC];
-  uint64_t cb[NC];
-  decode_cols<T, NC>(a, m, ci, cb);
+  W cb[NC];
+  decode_cols<T, W, NC>(a, m, ci, cb);
   T part = 0;
 #pragma unroll
   for (int c = 0; c < NC; c++) {
@@ -360,24 +439,31 @@ __device__ __forceinline__ T mixed_terms(const PassArgs<T>& a, int t0, int64_t m
     const T* vrow = vr + a.v_off[c];
     const int it = ci[c];
     const int64_t cst = a.cstride[c];
-    for_singles(cb[c], s.n_orb, [&](int, int, int, int Q, int odd, uint64_t J) {
-      const int jt = str_index(s, tab, J);
+    const int64_t mb = m - (int64_t)it * cst;
+    const int vs = a.vstride;
+    for_singles_w<W>(cb[c], s.n_orb, [&](int, int Q, int odd, W J) {
+      const int jt = str_index_w<W>(s, tab, J);
       if (jt < 0) return;
-      const T cv = crow[m + (int64_t)(jt - it) * cst];
-      const T v = vrow[Q];
-      part += odd ? -v * cv : v * cv;
+      const int64_t mj = mb + (int64_t)jt * cst;
+      T v = 0;
+#pragma unroll
+      for (int g = 0; g < kGMax; g++)
+        if (g < g_n) v += vrow[g * vs + Q] * crows[g][mj];
+      part += odd ? -v : v;
     });
   }
   return part;
 }
 
-template <typename T, int NC, int CPT>
+template <typename T, typename W, int NC, int CPT, bool STAGE>
 __global__ __launch_bounds__(pass_threads<NC>()) void k_sigma_pass(const PassArgs<T> a) {
   extern __shared__ __align__(16) char smem[];
-  T* crow = reinterpret_cast<T*>(smem);
+  T* rows_lds = reinterpret_cast<T*>(smem);
   T* xs = reinterpret_cast<T*>(smem + a.off_x);
   T* vr = reinterpret_cast<T*>(smem + a.off_v);
   uint32_t* rt = reinterpret_cast<uint32_t*>(smem + a.off_rt);
+  int2* rowlist = reinterpret_cast<int2*>(smem + a.off_rl);
+  __shared__ int n_rowlist;
 
   const int root = blockIdx.y;
   const int64_t row = blockIdx.x / a.tiles;
@@ -410,7 +496,27 @@ __global__ __launch_bounds__(pass_threads<NC>()) void k_sigma_pass(const PassArg
       xs[a.x_off[c] + P] = v;
     }
   }
-  for (int64_t m = tid; m < M; m += nth) crow[m] = Cr[row * M + m];
+  const T* crow;
+  if constexpr (STAGE) {
+    for (int64_t m = tid; m < M; m += nth) rows_lds[m] = Cr[row * M + m];
+    crow = rows_lds;
+  } else {
+    crow = Cr + row * M;
+  }
+  const bool mixed = (a.flags & F_MIXED) && a.nr == 1;
+  if (mixed && tid == 0) {
+    // single excitations of the row string: (index offset of the excited row, Q | odd << 30)
+    const int t0 = a.rs[0];
+    const SlotDev& s0 = a.sl[t0];
+    const int i0 = ri[0];
+    int n = 0;
+    for_singles_w<W>((W)s0.bits[i0], s0.n_orb, [&](int, int Q0, int odd0, W J0) {
+      const int j0 = str_index_w<W>(s0, s0.rtab, J0);
+      if (j0 < 0) return;
+      rowlist[n++] = make_int2(j0 - i0, Q0 | (odd0 << 30));
+    });
+    n_rowlist = n;
+  }
   __syncthreads();
 
   const int64_t mbase = (int64_t)tile * a.tile_cols + tid;
@@ -419,36 +525,48 @@ __global__ __launch_bounds__(pass_threads<NC>()) void k_sigma_pass(const PassArg
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
     const int64_t m = mbase + (int64_t)j * nth;
-    acc[j] = (m < mend) ? column_terms<T, NC>(a, row, m, crow, xs, rt) : (T)0;
+    acc[j] = (m < mend) ? column_terms<T, W, NC>(a, row, m, crow, xs, rt) : (T)0;
   }
 
-  if ((a.flags & F_MIXED) && a.nr == 1) {
+  if (mixed) {
     const int t0 = a.rs[0];
-    const SlotDev& s0 = a.sl[t0];
-    const int i0 = ri[0];
-    const uint64_t I0 = s0.bits[i0];
-    for_singles(I0, s0.n_orb, [&](int, int, int, int Q0, int odd0, uint64_t J0) {
-      const int j0 = str_index(s0, s0.rtab, J0);
-      if (j0 < 0) return;
-      __syncthreads();
-      const int64_t rowJ = row + (int64_t)(j0 - i0);
-      for (int64_t m = tid; m < M; m += nth) crow[m] = Cr[rowJ * M + m];
+    const int nrl = n_rowlist;
+    const int G = a.G;
+    for (int a0 = 0; a0 < nrl; a0 += G) {
+      const int g_n = min(G, nrl - a0);
+      __syncthreads();  // previous block's rows and couplings are no longer read
+      const T* crows[kGMax];
 #pragma unroll
-      for (int c = 0; c < NC; c++) {
-        const T* V0t = a.tb.V[t0][a.cs[c]];
-        const int np = a.sl[a.cs[c]].npair;
-        for (int P = tid; P < np; P += nth) vr[a.v_off[c] + P] = V0t ? V0t[(size_t)Q0 * np + P] : (T)0;
+      for (int g = 0; g < kGMax; g++) {
+        const int2 rl = rowlist[a0 + (g < g_n ? g : 0)];
+        const T* src = Cr + (row + rl.x) * M;
+        if constexpr (STAGE) {
+          T* dst = rows_lds + (int64_t)g * M;
+          if (g < g_n)
+            for (int64_t m = tid; m < M; m += nth) dst[m] = src[m];
+          crows[g] = dst;
+        } else {
+          crows[g] = src;
+        }
+        if (g < g_n) {
+          const int Q0 = rl.y & ((1 << 30) - 1);
+          const T sg = (rl.y >> 30) ? (T)-1 : (T)1;
+#pragma unroll
+          for (int c = 0; c < NC; c++) {
+            const T* V0t = a.tb.V[t0][a.cs[c]];
+            const int np = a.sl[a.cs[c]].npair;
+            for (int P = tid; P < np; P += nth)
+              vr[g * a.vstride + a.v_off[c] + P] = V0t ? sg * V0t[(size_t)Q0 * np + P] : (T)0;
+          }
+        }
       }
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < CPT; j++) {
         const int64_t m = mbase + (int64_t)j * nth;
-        if (m < mend) {
-          const T part = mixed_terms<T, NC>(a, t0, m, crow, vr, rt);
-          acc[j] += odd0 ? -part : part;
-        }
+        if (m < mend) acc[j] += mixed_terms<T, W, NC>(a, t0, m, crows, vr, g_n, rt);
       }
-    });
+    }
   }
 
   T* __restrict__ Sr = a.S + (int64_t)root * a.lds;
@@ -1069,49 +1187,85 @@ static void launch_pass(pqg_ci* ci, const Tables<T>& tb, const std::vector<int>&
   a.tiles = (int)((M + tile_cols - 1) / tile_cols);
   a.tile_cols = (int)tile_cols;
   a.cpt = (int)((tile_cols + nth - 1) / nth);
-  // LDS layout
-  size_t off = (size_t)M * sizeof(T);
-  off = (off + 15) / 16 * 16;
-  a.off_x = (int)off;
+  // LDS layout: [rows: G x M][spectator sums][G coupling rows][rank tables][row singles]
   size_t xo = 0;
   for (int c = 0; c < a.nc; c++) {
     a.x_off[c] = (int)xo;
+    a.v_off[c] = (int)xo;
     xo += ci->slot[cols[c]].npair;
   }
-  off += xo * sizeof(T);
-  off = (off + 15) / 16 * 16;
-  a.off_v = (int)off;
-  for (int c = 0; c < a.nc; c++) a.v_off[c] = a.x_off[c];
-  off += xo * sizeof(T);
-  off = (off + 15) / 16 * 16;
-  a.off_rt = (int)off;
+  a.vstride = (int)xo;
   size_t ro = 0;
   for (int c = 0; c < a.nc; c++) {
     a.rt_off[c] = (int)ro;
     const Slot& sl = ci->slot[cols[c]];
     ro += (size_t)sl.nchunk * 256 * (sl.nocc + 1);
   }
-  off += ro * 4;
-  PQG_REQUIRE(off <= 160 * 1024, PQG_ERR_UNSUPPORTED,
-              "sigma row (" + std::to_string(M) + " columns) does not fit in LDS");
+  const Slot& s0 = ci->slot[rows[0]];
+  const size_t nrowsingles = (flags & F_MIXED) && a.nr == 1 ? (size_t)std::max<int64_t>(1, s0.nsingle) : 1;
+  auto r16 = [](size_t x) { return (x + 15) / 16 * 16; };
+  const size_t kLds = 160 * 1024 - 256;  // leaves room for the kernel's static LDS
+  auto layout = [&](int G, bool stage) {
+    size_t off = stage ? r16((size_t)G * M * sizeof(T)) : 0;
+    a.off_x = (int)off;
+    off = r16(off + xo * sizeof(T));
+    a.off_v = (int)off;
+    off = r16(off + (size_t)G * xo * sizeof(T));
+    a.off_rt = (int)off;
+    off = r16(off + ro * 4);
+    a.off_rl = (int)off;
+    off += nrowsingles * sizeof(int2);
+    return off;
+  };
+  // Stage rows in LDS only while the block keeps about 16 waves per CU resident; otherwise read
+  // them in place (they stay in L2 while the block works on them).  Without the row x column
+  // term only the row itself is staged.
+  const int waves = (nth + 63) / 64;
+  const int want_blocks = std::max(1, (16 + waves - 1) / waves);
+  const size_t budget = std::min(kLds, (size_t)(160 * 1024) / (size_t)want_blocks);
+  const bool mixed = (flags & F_MIXED) && a.nr == 1;
+  bool stage = true;
+  int G = mixed ? kGMax : 1;
+  while (G > 1 && layout(G, true) > budget) G--;
+  if (layout(G, true) > budget || (mixed && G < 4)) {
+    stage = false;
+    G = mixed ? kGMax : 1;
+  }
+  const size_t off = layout(G, stage);
+  PQG_REQUIRE(off <= kLds, PQG_ERR_UNSUPPORTED, "sigma pass tables do not fit in LDS");
+  a.G = G;
   dim3 grid((unsigned)(nrows * a.tiles), (unsigned)k);
   ProfScope prof(ci->ctx, st, a.nc == 1 ? (sizeof(T) == 8 ? "sigma_pass_nc1_f64" : "sigma_pass_nc1_f32")
                                         : (sizeof(T) == 8 ? "sigma_pass_nc3_f64" : "sigma_pass_nc3_f32"),
                  k);
+  int max_orb = 0;
+  for (int t = 0; t < ci->nslot; t++) max_orb = std::max(max_orb, ci->slot[t].n_orb);
+  const bool narrow = max_orb <= 32;
   auto go = [&](auto kern) {
     PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)off));
     kern<<<grid, nth, off, st>>>(a);
   };
-  const int cpt = a.cpt;
-  if (a.nc == 1) {
-    if (cpt <= 1) go(k_sigma_pass<T, 1, 1>);
-    else if (cpt <= 2) go(k_sigma_pass<T, 1, 2>);
-    else if (cpt <= 4) go(k_sigma_pass<T, 1, 4>);
-    else if (cpt <= 8) go(k_sigma_pass<T, 1, 8>);
-    else go(k_sigma_pass<T, 1, 16>);
+  auto pick = [&](auto w_tag, auto stage_tag) {
+    using W = decltype(w_tag);
+    constexpr bool S = decltype(stage_tag)::value;
+    const int cpt = a.cpt;
+    if (a.nc == 1) {
+      if (cpt <= 1) go(k_sigma_pass<T, W, 1, 1, S>);
+      else if (cpt <= 2) go(k_sigma_pass<T, W, 1, 2, S>);
+      else if (cpt <= 4) go(k_sigma_pass<T, W, 1, 4, S>);
+      else if (cpt <= 8) go(k_sigma_pass<T, W, 1, 8, S>);
+      else go(k_sigma_pass<T, W, 1, 16, S>);
+    } else {
+      if (cpt <= 1) go(k_sigma_pass<T, W, 3, 1, S>);
+      else go(k_sigma_pass<T, W, 3, 2, S>);
+    }
+  };
+  if (narrow) {
+    if (stage) pick(uint32_t{}, std::true_type{});
+    else pick(uint32_t{}, std::false_type{});
   } else {
-    if (cpt <= 1) go(k_sigma_pass<T, 3, 1>);
-    else go(k_sigma_pass<T, 3, 2>);
+    if (stage) pick(uint64_t{}, std::true_type{});
+    else pick(uint64_t{}, std::false_type{});
   }
   PQG_HIP(hipGetLastError());
 }

@@ -181,6 +181,21 @@ def config3(nroots: int = 4) -> CIProblem:
     return cholesky_problem(SEED_BASE + 3, sp, nroots, name="fci_16o_8a8b")
 
 
+def config4(n: int = 320, nvec: int = 1280, sigma: float = 0.05):
+    """AO->MO transform input: (L, C).
+
+    L: nvec x n_pair symmetric Cholesky-like vectors (idx2-packed), entries N(0, sigma^2), so
+    that (mu nu|la si) = sum_k L[k, idx2(mu,nu)] L[k, idx2(la,si)].  C: Q factor of a Householder
+    QR of an n x n Gaussian with diag(R) > 0 (column-major MO coefficients).
+    """
+    rng = SplitMix64(SEED_BASE + 4)
+    L = packed_cholesky(rng, n, nvec, sigma)
+    G = rng.normal(n * n).reshape(n, n)
+    Q, R = np.linalg.qr(G)
+    Q = Q * np.sign(np.diag(R))[None, :]
+    return L, np.asfortranarray(Q)
+
+
 def small_problem(seed: int, n: int, na: int, nb: int, nroots: int = 1, nspin: int = 1,
                   positron: Optional[tuple] = None, ex_level=None, pos_ex_level=None,
                   max_collective: int = 2**31 - 1) -> CIProblem:
