@@ -170,6 +170,185 @@ def bench_ao2mo(args):
     ctx.close()
 
 
+def bench_fci16(args):
+    """Electronic FCI 16 orbitals 8a8b (165,636,900 determinants), 4 roots: FP64 vs mixed Davidson."""
+    import numpy as np
+    import torch
+
+    from polyquant_amd import workloads as wl
+    from polyquant_amd._lib import PQG_FP64, PQG_MIXED
+    from polyquant_amd.ci import Context, DetSet
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    prob = wl.config3()
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    ds = DetSet(prob, ctx)
+    setup = time.perf_counter() - t0
+    N, nev = ds.N_dets, prob.nroots
+    log(f"[fci16] N={N} roots={nev} setup {setup:.2f}s")
+    res = {}
+    for name, prec in (("fp64", PQG_FP64), ("mixed", PQG_MIXED)):
+        t0 = time.perf_counter()
+        E, _, rep = ds.davidson(nev, nvec_init=args.fci_nvec, nvec_max=args.fci_nmax, tol=1e-6, precision=prec,
+                                maxit=args.fci_maxit, want_vectors=False)
+        dt = time.perf_counter() - t0
+        res[name] = {"seconds": round(dt, 3), "iterations": rep.iterations, "sigma_vectors": rep.sigma_calls,
+                     "sigma_vectors_fp64": rep.sigma_calls_fp64, "energies": [float(x) for x in E],
+                     "residuals": [float(rep.residual[i]) for i in range(nev)]}
+        log(f"[fci16] {name}: {dt:.1f}s {rep.iterations} iterations E={E}")
+    dE = float(np.max(np.abs(np.array(res["mixed"]["energies"]) - np.array(res["fp64"]["energies"]))))
+    out = {
+        "metric": "Davidson time-to-1e-6 Eh, FP64 vs mixed (FP32 sigma bulk + FP64 correction)",
+        "value": res["fp64"]["seconds"], "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "fci_16o_8a8b: electronic FCI, 16 orbitals, 8 alpha + 8 beta electrons, "
+                               f"{N} determinants, {nev} roots, residual <= 1e-6",
+                   "num_subspace_vec": args.fci_nvec, "max_subspace": args.fci_nmax},
+        "fp64": res["fp64"], "mixed": res["mixed"],
+        "mixed_speedup": round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3),
+        "max_abs_dE_mixed_vs_fp64": dE,
+    }
+    print(json.dumps(out), flush=True)
+    ds.close()
+    ctx.close()
+
+
+def bench_serve(args):
+    """Serving loop: 512 CI requests with Poisson arrivals (200/s), mixed precision, vs sequential FP64."""
+    import numpy as np
+    import torch
+
+    from polyquant_amd import workloads as wl
+    from polyquant_amd._lib import PQG_FP64, PQG_MIXED
+    from polyquant_amd.ci import Context, DetSet
+    from polyquant_amd.serve import Server
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    reqs = wl.serving_requests()[: args.serve_n]
+    n = len(reqs)
+    ctx = Context(0, hbm_budget=200 << 30)
+    nvec = 5
+    # sequential plain FP64 solves, one request at a time
+    seq_t, seq_E, seq_fail = [], [], 0
+    t_all = time.perf_counter()
+    for i, (_, prob) in enumerate(reqs):
+        t0 = time.perf_counter()
+        try:
+            ds = DetSet(prob, ctx)
+            E, _, _ = ds.davidson(prob.nroots, nvec_init=nvec, tol=1e-6, precision=PQG_FP64, want_vectors=False)
+            ds.close()
+            seq_E.append(E)
+        except Exception as e:  # a failed request is counted, the loop goes on
+            seq_E.append(None)
+            seq_fail += 1
+            log(f"[serve] sequential request {i} failed: {e}")
+        seq_t.append(time.perf_counter() - t0)
+        if i % 32 == 31:
+            log(f"[serve] sequential {i + 1}/{n} done, {time.perf_counter() - t_all:.1f}s")
+    seq_total = time.perf_counter() - t_all
+    # serving: replay the arrival times, mixed precision with FP64 correction
+    srv = Server(ctx, max_slots=args.serve_slots)
+    got, ids = {}, {}
+    t_start = time.perf_counter()
+    c0 = srv.clock()
+    nxt = 0
+    while len(got) < n:
+        now = time.perf_counter() - t_start
+        while nxt < n and reqs[nxt][0] <= now:
+            rid = srv.submit(reqs[nxt][1], nvec_init=nvec, tol=1e-6, precision=PQG_MIXED)
+            ids[rid] = nxt
+            nxt += 1
+        for r in srv.poll():
+            got[ids[r["id"]]] = r
+            if len(got) % 64 == 0:
+                log(f"[serve] served {len(got)}/{n}, {time.perf_counter() - t_start:.1f}s")
+        time.sleep(0.0005)
+    makespan = time.perf_counter() - t_start
+    srv.close()
+    lat = np.array([got[i]["t_done"] - got[i]["t_submit"] for i in range(n)])
+    ok = [i for i in range(n) if got[i]["status"] == 0 and seq_E[i] is not None]
+    dE = max((float(np.max(np.abs(got[i]["energies"] - seq_E[i]))) for i in ok), default=0.0)
+    seq_lat = np.array(seq_t)
+    out = {
+        "metric": "CI solves/s + p99 latency (serving loop vs sequential plain FP64)",
+        "value": round(n / makespan, 3), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "mixed (FP32 sigma, FP64 correction)", "data": "synthetic",
+        "config": {"workload": f"serving: {n} CI requests, Poisson arrivals 200/s, n_orb 6-14, alpha/beta 2..n/2, "
+                               "positron 0/1 with 6-12 orbitals, 1-4 roots, HBM budget 200 GB",
+                   "max_slots": args.serve_slots, "num_subspace_vec": nvec},
+        "serving": {"makespan_s": round(makespan, 3), "solves_per_s": round(n / makespan, 3),
+                    "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                    "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+                    "failed": int(sum(1 for i in range(n) if got[i]["status"] != 0))},
+        "sequential_fp64": {"total_s": round(seq_total, 3), "solves_per_s": round(n / seq_total, 3),
+                            "p50_solve_s": round(float(np.percentile(seq_lat, 50)), 4),
+                            "p99_solve_s": round(float(np.percentile(seq_lat, 99)), 4), "failed": seq_fail},
+        "speedup_solves_per_s": round(seq_total / makespan, 3),
+        "max_abs_dE_serving_vs_sequential": dE,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_h2o(args):
+    """Water/6-31G full CI (13 orbitals, 5a5b, 1,656,369 determinants), RHF orbitals from the CPU."""
+    import numpy as np
+    import torch
+
+    from polyquant_amd import workloads as wl
+    from polyquant_amd._lib import PQG_FP64, PQG_MIXED
+    from polyquant_amd.ci import Context, DetSet
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    t0 = time.perf_counter()
+    prob = wl.water_problem("6-31g", nroots=1)
+    t_scf = time.perf_counter() - t0
+    ctx = Context(0)
+    ds = DetSet(prob, ctx)
+    N = ds.N_dets
+    log(f"[h2o] RHF {prob.e_rhf:.10f} ({t_scf:.1f}s on the CPU), N={N}")
+    res = {}
+    for name, prec in (("fp64", PQG_FP64), ("mixed", PQG_MIXED)):
+        ds.davidson(1, nvec_init=args.h2o_nvec, tol=1e-6, precision=prec, want_vectors=False)  # warm-up
+        times = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            E, _, rep = ds.davidson(1, nvec_init=args.h2o_nvec, tol=1e-6, precision=prec, want_vectors=False)
+            times.append(time.perf_counter() - t0)
+        res[name] = {"seconds": round(min(times), 4), "iterations": rep.iterations, "sigma_vectors": rep.sigma_calls,
+                     "energy": float(E[0] + prob.e_shift), "residual": float(rep.residual[0])}
+        log(f"[h2o] {name}: {min(times) * 1e3:.1f} ms, {rep.iterations} iterations, E={E[0] + prob.e_shift:.10f}")
+    # one FP64 sigma with kernel events
+    ctx.profile(True)
+    for name in ("sigma_f64",):
+        ctx.profile_read(name)
+    C = np.random.default_rng(0).standard_normal((N, 1))
+    ds.sigma(C)
+    ctx.profile(False)
+    launches, sig_ms, _ = ctx.profile_read("sigma_f64")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(prob, res["fp64"]["sigma_vectors"], 1, args.cpu_threads)
+        cpu.pop("sigma_s", None)
+    out = {
+        "metric": "Davidson time-to-1e-6 Eh (FP64 vs mixed); CI solves/s",
+        "value": round(1.0 / res["fp64"]["seconds"], 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "higher_is_better": True, "dtype": "f64", "data": "water 6-31G, RHF orbitals from the CPU integral code",
+        "config": {"workload": f"h2o_631g_fci: H2O (O-H 0.9572 A, HOH 104.52 deg), 6-31G, 13 orbitals, 5a5b, "
+                               f"{N} determinants, 1 root, residual <= 1e-6", "e_rhf": prob.e_rhf,
+                   "num_subspace_vec": args.h2o_nvec},
+        "fp64": res["fp64"], "mixed": res["mixed"],
+        "mixed_speedup": round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3),
+        "abs_dE_mixed_vs_fp64": abs(res["mixed"]["energy"] - res["fp64"]["energy"]),
+        "sigma_ms_fp64_one_vector": round(sig_ms / max(launches, 1), 4),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ds.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,8 +356,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--workload", default="davidson", choices=["davidson", "ao2mo"],
+    ap.add_argument("--workload", default="davidson", choices=["davidson", "ao2mo", "fci16", "serve", "h2o"],
                     help="davidson (default): ep-FCI 48/48/48 Davidson solves; ao2mo: n = 320 transform")
+    ap.add_argument("--h2o-nvec", type=int, default=5)
+    ap.add_argument("--serve-n", type=int, default=512)
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--fci-nvec", type=int, default=8)
+    ap.add_argument("--fci-nmax", type=int, default=32)
+    ap.add_argument("--fci-maxit", type=int, default=200)
     ap.add_argument("--ao-n", type=int, default=320)
     ap.add_argument("--ao-reps", type=int, default=2)
     ap.add_argument("--ao-budget-gb", type=float, default=120.0)
@@ -199,6 +384,15 @@ def main():
         return
     if args.workload == "ao2mo":
         bench_ao2mo(args)
+        return
+    if args.workload == "fci16":
+        bench_fci16(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
+        return
+    if args.workload == "h2o":
+        bench_h2o(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the CI path is built and "

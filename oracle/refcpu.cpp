@@ -917,6 +917,48 @@ int rc_sigma_slow(void* h, const double* C, int64_t ldc, int k, double* S, int64
   return 0;
 }
 
+// create_sigma_slow (direct.cpp:43-53) restricted to the listed rows: sigma(i) = sum_j SC(i, j)
+// C(j), with j running over the determinants that differ from i by at most two excitations
+// (Slater_Condon is zero beyond that, slater_condon.cpp:4-100), found from the per-string
+// single and double excitation lists.
+int rc_sigma_rows(void* h, const double* C, int64_t ldc, int k, const int64_t* rows, int64_t nrows, double* out) {
+  auto* d = static_cast<DetSet*>(h);
+  const int nslot = 2 * d->nsp;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t r = 0; r < nrows; r++) {
+    const int64_t i = rows[r];
+    const int32_t* I = d->unfold[i].data();
+    // per slot: (string index, excitation degree) candidates
+    std::vector<std::pair<int, int>> cand[4];
+    for (int t = 0; t < nslot; t++) {
+      const int p = t / 2, sp = t % 2;
+      cand[t].push_back({I[t], 0});
+      for (int J : d->singles[p][sp][I[t]]) cand[t].push_back({J, 1});
+      for (int J : d->doubles[p][sp][I[t]]) cand[t].push_back({J, 2});
+    }
+    for (int t = nslot; t < 4; t++) cand[t].push_back({0, 0});
+    std::vector<double> acc(k, 0.0);
+    for (auto& a0 : cand[0])
+      for (auto& a1 : cand[1]) {
+        if (a0.second + a1.second > 2) continue;
+        for (auto& a2 : cand[2]) {
+          if (a0.second + a1.second + a2.second > 2) continue;
+          for (auto& a3 : cand[3]) {
+            if (a0.second + a1.second + a2.second + a3.second > 2) continue;
+            const int32_t J[4] = {a0.first, a1.first, a2.first, a3.first};
+            const int32_t j = d->address(J);
+            if (j < 0) continue;
+            const double e = slater_condon(*d, i, j);
+            if (e == 0.0) continue;
+            for (int s = 0; s < k; s++) acc[s] += e * C[j + s * ldc];
+          }
+        }
+      }
+    for (int s = 0; s < k; s++) out[r + s * nrows] = acc[s];
+  }
+  return 0;
+}
+
 // Davidson, restating the eigensolver the reference drives at epci.cpp:589-608
 // (Spectra DavidsonSymEigsSolver, SmallestAlge): unit-vector start at the nvec_init
 // smallest diagonal entries, Rayleigh-Ritz, residual test on the first nev Ritz pairs,

@@ -61,6 +61,8 @@ def lib():
         L.rc_sigma.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64]
         L.rc_sigma_slow.argtypes = L.rc_sigma.argtypes
         L.rc_sigma_sample.argtypes = L.rc_sigma.argtypes + [ct.c_int64]
+        L.rc_sigma_rows.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
+                                    ct.c_void_p]
         L.rc_davidson.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double,
                                   ct.c_void_p, ct.c_void_p, ct.c_void_p]
         L.rc_get_holes.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p]
@@ -213,6 +215,15 @@ class OracleCI:
         S = np.zeros_like(C, order="F")
         lib().rc_sigma_sample(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(S), S.shape[0], int(stride))
         return S
+
+    def sigma_rows(self, C, rows):
+        """Exact sigma on the listed rows (slow sigma over the connected determinants)."""
+        C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        out = np.zeros((len(rows), C.shape[1]), order="F")
+        lib().rc_sigma_rows(self.h, _ptr(C), C.shape[0], C.shape[1], rows.ctypes.data_as(ct.c_void_p), len(rows),
+                            _ptr(out))
+        return out
 
     def sigma_slow(self, C):
         C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)

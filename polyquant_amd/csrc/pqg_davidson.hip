@@ -203,7 +203,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 
 __global__ __launch_bounds__(1024) void k_jacobi(double* __restrict__ G, int m, int old, const double* __restrict__ gt,
                                                  const double* __restrict__ Yp, int warm, double* __restrict__ theta,
-                                                 double* __restrict__ Y, double* __restrict__ Ynext) {
+                                                 double* __restrict__ Y, double* __restrict__ Ynext, double rel_tol2) {
   __shared__ double A[64][65];
   __shared__ double Vv[64][65];
   __shared__ double Tm[64][65];
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(1024) void k_jacobi(double* __restrict__ G, int m, 
     const double offs = block_sum(off, red);
     __syncthreads();
     const double dias = block_sum(dia, red);
-    if (offs <= 1e-30 * dias || offs < 1e-300) break;
+    if (offs <= rel_tol2 * dias || offs < 1e-300) break;
     for (int step = 0; step < mm - 1; step++) {
       __syncthreads();  // previous step's updates are visible
       if (tid < half) {
@@ -653,8 +653,14 @@ void Davidson::rayleigh_ritz(hipStream_t st) {
     nW_ = m_;
   }
   // Yprev_ (warm start) and Y_ are distinct buffers: the kernel reads one and writes both
+  // Off-diagonal tolerance of the subspace eigensolve: the eigenvectors only need to be accurate
+  // well below the current residual level (1e-3 of the worst residual relative to the largest
+  // Ritz value), and to full precision once the residuals approach the convergence threshold.
+  double rel = 1e-15;
+  if (have_worst_ && theta_abs_max_ > 0.0)
+    rel = std::max(1e-15, std::min(1e-6, 1e-3 * std::max(last_worst_, p_.tol) / theta_abs_max_));
   k_jacobi<<<1, 1024, 0, st>>>(G_.get(), m_, old, gt, Yprev_.get(), std::min(warm_, m_), theta_.get(), Y_.get(),
-                               Ynext_.get());
+                               Ynext_.get(), rel * rel);
   PQG_HIP(hipGetLastError());
   std::swap(Yprev_, Ynext_);
   warm_ = m_;
@@ -769,6 +775,10 @@ void Davidson::post_sigma(hipStream_t st) {
   rn_h_.assign(status_h_, status_h_ + nev_);
   double worst = 0.0;
   for (int k = 0; k < nev_; k++) worst = std::max(worst, rn_h_[k]);
+  last_worst_ = worst;
+  have_worst_ = true;
+  theta_abs_max_ = 0.0;
+  for (int k = 0; k < nev_; k++) theta_abs_max_ = std::max(theta_abs_max_, std::fabs(theta_h_[k]));
   dav_log(mixed_ ? "fp32" : "fp64", iter_, m_, p_.tol, theta_h_, rn_h_);
   since_check_++;
   const double thr = std::max(p_.tol, std::min(1e-3, 0.05 * last_true_));

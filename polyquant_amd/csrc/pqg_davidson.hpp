@@ -80,6 +80,8 @@ class Davidson {
   bool done_ = false, checking_ = false;
   pqg_status status_ = PQG_OK;
   double last_true_ = 1e300;
+  double last_worst_ = 1e300, theta_abs_max_ = 0.0;  // from the last residual check
+  bool have_worst_ = false;
   std::vector<double> theta_h_, rn_h_;
   SigmaWork work_;
   std::chrono::steady_clock::time_point t0_;

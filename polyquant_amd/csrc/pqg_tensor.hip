@@ -145,13 +145,19 @@ __global__ __launch_bounds__(256) void k_pair_gemm(PairGemmArgs a, const T* __re
     }
     __syncthreads();
     if (k0 + PG_BK < kend) load(k0 + PG_BK);
+    // all fragments of the tile first, so one LDS wait covers the 4 * NF MFMAs that follow
+    T av[PG_BK / 4], bv[PG_BK / 4][NF];
 #pragma unroll
-    for (int ks = 0; ks < PG_BK; ks += 4) {
-      const int kk = ks + (lane >> 4);
-      const T av = As[kk][wv * 16 + (lane & 15)];
+    for (int q = 0; q < PG_BK / 4; q++) {
+      const int kk = 4 * q + (lane >> 4);
+      av[q] = As[kk][wv * 16 + (lane & 15)];
 #pragma unroll
-      for (int f = 0; f < NF; f++) acc[f] = Mfma<T>::op(av, Bs[kk][f * 16 + (lane & 15)], acc[f]);
+      for (int f = 0; f < NF; f++) bv[q][f] = Bs[kk][f * 16 + (lane & 15)];
     }
+#pragma unroll
+    for (int q = 0; q < PG_BK / 4; q++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) acc[f] = Mfma<T>::op(av[q], bv[q][f], acc[f]);
   }
   T* __restrict__ out = P + (int64_t)(pi * a.split + sp) * a.N * a.k;
 #pragma unroll

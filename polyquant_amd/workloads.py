@@ -196,6 +196,30 @@ def config4(n: int = 320, nvec: int = 1280, sigma: float = 0.05):
     return L, np.asfortranarray(Q)
 
 
+def water_problem(basis: str = "6-31g", nroots: int = 1, ex_level=None) -> CIProblem:
+    """Water (O-H 0.9572 A, HOH 104.52 deg), RHF orbitals on the CPU, all orbitals active.
+
+    basis "6-31g" with ex_level None is the full CI of 13 orbitals, 5a5b (1,656,369 dets);
+    "sto-3g" with ex_level (2, 2, 2) is the reference's H2O/STO-3G CISD case
+    (tests/data/h2o_sto3glibrary_cisd/h2o.json).  e_shift = nuclear repulsion.
+    """
+    from . import molint
+
+    w = molint.water(basis)
+    n = w["n_orb"]
+    P = pair_index_matrix(n)
+    ii, jj = np.tril_indices(n)
+    pairs = P[ii, jj]
+    order = np.argsort(pairs)
+    ii, jj = ii[order], jj[order]
+    v = w["eri_mo"][ii[:, None], jj[:, None], ii[None, :], jj[None, :]]
+    sp = [Species(n, w["n_alpha"], w["n_beta"], -1.0, 1, ex_level)]
+    prob = CIProblem(species=sp, h=[[w["h_mo"].copy()]], eri={(0, 0, 0, 0): np.ascontiguousarray(v)},
+                     nroots=nroots, e_shift=w["E_nuc"], name=f"h2o_{basis}")
+    prob.e_rhf = w["E_rhf"]
+    return prob
+
+
 def small_problem(seed: int, n: int, na: int, nb: int, nroots: int = 1, nspin: int = 1,
                   positron: Optional[tuple] = None, ex_level=None, pos_ex_level=None,
                   max_collective: int = 2**31 - 1) -> CIProblem:
