@@ -148,6 +148,7 @@ typedef struct {
   double energies[64];
   double t_submit, t_admit, t_done; /* seconds on the server clock                       */
   int iterations;
+  char message[128];               /* error text of a failed request ("" on success)      */
 } pqg_result;
 
 /* max_slots: solves resident on the device at once; budget from the context.            */

@@ -67,6 +67,7 @@ class Result(ct.Structure):
         ("t_admit", ct.c_double),
         ("t_done", ct.c_double),
         ("iterations", ct.c_int),
+        ("message", ct.c_char * 128),
     ]
 
 

@@ -232,7 +232,7 @@ __global__ void k_hdiag(DiagArgs a, double* __restrict__ Hd) {
 constexpr int kCPT = 16;   // max columns per thread
 constexpr int kGMax = 8;   // excited rows of the row string staged per block (row x column doubles)
 
-enum PassFlags { F_DIAG = 1, F_COLS = 2, F_MIXED = 4, F_ACCUM = 8 };
+enum PassFlags { F_DIAG = 1, F_COLS = 2, F_MIXED = 4, F_ACCUM = 8, F_NODBL = 16 };
 
 template <typename T>
 struct PassArgs {
@@ -380,8 +380,8 @@ __device__ __forceinline__ T column_terms(const PassArgs<T>& a, int64_t row, int
       const T cv = cbase[(int64_t)jt * cst];
       sum += odd ? -v * cv : v * cv;
     });
-    // same-string doubles
-    if (s.ndouble > 0) {
+    // same-string doubles (unless the link GEMM forms them)
+    if (s.ndouble > 0 && !(a.flags & F_NODBL)) {
       const T* __restrict__ Dt = a.tb.D[t] + it;
       for_doubles_w<W>(cb[c], s.n_orb, [&](int e, W J) {
         const int jt = str_index_w<W>(s, tab, J);
@@ -987,6 +987,8 @@ void ensure_mixed(pqg_ci* ci) {
       if (t != u) to_f32(ctx, ci->stream, ci->V64[t][u], ci->V32[t][u]);
     }
   }
+  if (ci->lg.enabled)
+    for (int r = 0; r < 2; r++) to_f32(ctx, ci->stream, ci->V64[r][r], ci->lg.Vrr32[r]);
   PQG_HIP(hipStreamSynchronize(ci->stream));
   ci->mixed_ready = true;
 }
@@ -1124,6 +1126,7 @@ pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* d, hipStream_t stream) {
     for (int t = 0; t < ci->nslot; t++) h1[t] = hdev[t].get();
     tensor_setup(ci.get(), h1);
   }
+  if (!ci->tensor.enabled) link_gemm_setup(ci.get());
   // ---- diagonal ----------------------------------------------------------------------------
   DiagArgs da{};
   da.nslot = ci->nslot;
@@ -1293,12 +1296,28 @@ static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k,
     colsA.push_back(t);
     rowsB.push_back(t);
   }
+  if (ci->lg.enabled) {
+    // diagonal and beta singles (with their alpha spectator sums)
+    launch_pass<T>(ci, tb, rowsA, colsA, C, ld, S, ld, k, F_DIAG | F_COLS | F_NODBL, st);
+    const size_t need = 2 * (size_t)N * k * sizeof(T);
+    grow(ci->scratch, ci->ctx, need, st);
+    T* CT = reinterpret_cast<T*>(ci->scratch.get());
+    T* ST = CT + (size_t)N * k;
+    transpose<T>(C, ld, CT, N, n0, M, k, false, st);
+    // alpha singles (with their beta spectator sums), then beta doubles, on the transpose
+    launch_pass<T>(ci, tb, rowsB, {0}, CT, N, ST, N, k, F_COLS | F_NODBL, st);
+    for (int v = 0; v < k; v++) link_gemm_pass(ci, 1, CT + (size_t)v * N, ST + (size_t)v * N, st);
+    transpose<T>(ST, N, S, ld, M, n0, k, true, st);
+    // alpha-beta single-single couplings and alpha doubles
+    for (int v = 0; v < k; v++) link_gemm_pass(ci, 0, C + (size_t)v * ld, S + (size_t)v * ld, st);
+    return;
+  }
   const bool need_b = ci->slot[0].nsingle > 0;
   // pass A: diagonal, column-side excitations, row x column doubles
   launch_pass<T>(ci, tb, rowsA, colsA, C, ld, S, ld, k, F_DIAG | F_COLS | F_MIXED, st);
   if (need_b) {
     const size_t need = 2 * (size_t)N * k * sizeof(T);
-    if (ci->scratch.n < need) ci->scratch.reset(ci->ctx, need);
+    grow(ci->scratch, ci->ctx, need, st);
     T* CT = reinterpret_cast<T*>(ci->scratch.get());
     T* ST = CT + (size_t)N * k;
     transpose<T>(C, ld, CT, N, n0, M, k, false, st);
@@ -1343,6 +1362,7 @@ uint64_t estimate_ci_bytes(const pqg_ci_desc* d, int nvec_max, int precision) {
   uint64_t vec = N * w * (2ull * nvec_max + 8);  // V, W, work vectors
   vec += N * 8;                                  // diagonal
   vec += N * (precision == PQG_MIXED ? 4 : 8) * 4ull * 2;  // sigma transposes
+  if (d->nspecies == 1) vec += lg_budget_bytes((int64_t)N, 8);  // link-GEMM chunk
   return tables + vec + (64ull << 20);
 }
 

@@ -47,6 +47,19 @@ struct Tables {
   const T* V[kSlots][kSlots] = {};  // coupling blocks [npair_t][npair_u], charge factor included
 };
 
+// One-species full CI: link tables for the sigma terms computed as GEMMs on MFMA.
+struct LinkTab {
+  int L = 0, nv = 0, nb = 0;  // links per string, virtuals, off-diagonal orbital pairs
+  DevBuf<int2> links;         // [nstr][L]: {J | odd << 31, Q << 16 | rev}
+  DevBuf<int32_t> tbm;        // [nb][nstr]: (excited string << 1 | odd) or -1 (column set only)
+};
+struct LinkGemm {
+  bool enabled = false;
+  LinkTab tab[2];
+  DevBuf<char> U;             // GEMM output for one column chunk
+  DevBuf<float> Vrr32[2];     // FP32 copies of the same-set coupling blocks
+};
+
 }  // namespace pqg
 
 namespace pqg {
@@ -90,6 +103,7 @@ struct pqg_ci {
   pqg::DevBuf<double> DW[pqg::kSlots][pqg::kSlots];
   bool mixed_ready = false;
   pqg::TensorPath tensor;          // single-particle strings: dense pair-GEMM sigma
+  pqg::LinkGemm lg;                // one-species full CI: link GEMM sigma terms
   // scratch for sigma (transposed copies), grown on demand
   pqg::DevBuf<char> scratch;
   // boundary staging
@@ -115,5 +129,12 @@ bool tensor_eligible(const pqg_ci* ci);
 void tensor_setup(pqg_ci* ci, const std::vector<const double*>& h1);
 void tensor_sigma(pqg_ci* ci, const double* C, double* S, int k, int64_t ld, hipStream_t st);
 void tensor_sigma(pqg_ci* ci, const float* C, float* S, int k, int64_t ld, hipStream_t st);
+bool link_gemm_eligible(const pqg_ci* ci);
+void link_gemm_setup(pqg_ci* ci);
+size_t lg_budget_bytes(int64_t N, size_t w);
+// r = 0: alpha-beta single-single couplings + alpha doubles on the natural layout;
+// r = 1: beta doubles on the transposed vector.  Accumulates into S (one vector).
+void link_gemm_pass(pqg_ci* ci, int r, const double* C, double* S, hipStream_t st);
+void link_gemm_pass(pqg_ci* ci, int r, const float* C, float* S, hipStream_t st);
 
 }  // namespace pqg

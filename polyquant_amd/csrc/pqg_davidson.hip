@@ -546,10 +546,8 @@ void Davidson::run_sigma(hipStream_t st) {
     sigma_calls64_ += k;
   } else {
     const int64_t n = (int64_t)N_ * k;
-    if (c32_.n < (size_t)n) {
-      c32_.reset(ci_->ctx, n);
-      s32_.reset(ci_->ctx, n);
-    }
+    grow(c32_, ci_->ctx, n, st);
+    grow(s32_, ci_->ctx, n, st);
     for (int j = 0; j < k; j++)
       k_d2f_blk<<<g1(N_, 256), 256, 0, st>>>(work_.C + j * work_.ld, c32_.get() + (int64_t)j * N_, N_);
     sigma_internal(ci_, c32_.get(), s32_.get(), k, N_, st);

@@ -169,4 +169,14 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Grows a scratch buffer used by work queued on `st`.  The old block goes back to the shared
+// arena only after that work has finished: another stream (a serving slot) may be handed the
+// memory at once.
+template <class T>
+void grow(DevBuf<T>& b, pqg_ctx* c, size_t count, hipStream_t st) {
+  if (b.n >= count) return;
+  if (b.p) PQG_HIP(hipStreamSynchronize(st));
+  b.reset(c, count);
+}
+
 }  // namespace pqg

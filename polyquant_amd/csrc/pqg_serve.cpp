@@ -11,6 +11,7 @@
 // is retired with that status and the loop keeps serving.  Every solve's arithmetic depends only
 // on its own request (no shared reductions, no atomics), so a result is bit-identical alone
 // and in the mix.
+#include <cstdio>
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -96,6 +97,7 @@ struct pqg_server {
           pqg_result res{};
           res.request_id = r->id;
           res.status = PQG_ERR_OOM;
+          std::snprintf(res.message, sizeof(res.message), "request needs more than the whole HBM budget");
           res.t_submit = r->t_submit;
           res.t_admit = res.t_done = now();
           done.push_back(res);
@@ -144,6 +146,7 @@ struct pqg_server {
         if (dav.status() != PQG_OK) throw Error{dav.status(), "CI Calculation did not converge!"};
       });
       res.status = status;
+      if (status != PQG_OK) std::snprintf(res.message, sizeof(res.message), "%s", pqg_last_error());
       res.t_done = now();
       std::lock_guard<std::mutex> lk(mu);
       done.push_back(res);

@@ -17,11 +17,9 @@
 #include <cstring>
 
 #include "pqg_ci.hpp"
+#include "pqg_mfma.hpp"
 
 namespace pqg {
-
-typedef double d4v __attribute__((ext_vector_type(4)));
-typedef float f4v __attribute__((ext_vector_type(4)));
 
 // One launch computes every pair term of sigma.  Pair p's matrix W_p (M x M, row-major) is
 // applied to C viewed as an M x (rest * k) matrix without moving C: row m = (x1, x2) of the
@@ -44,27 +42,6 @@ struct PairGemmArgs {
   PairGemm p[kMaxPairs];
   int npairs, split, k;
   int64_t N, ldc;
-};
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  typedef d4v acc_t;
-  // lane l: A[l%16][l/16], B[l/16][l%16]; register r holds D[4*r + l/16][l%16]
-  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int r, int lane) { return 4 * r + (lane >> 4); }
-};
-template <>
-struct Mfma<float> {
-  typedef f4v acc_t;
-  // lane l: A[l%16][l/16], B[l/16][l%16]; register r holds D[4*(l/16) + r][l%16]
-  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int r, int lane) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int PG_BM = 64, PG_BK = 16;
@@ -366,7 +343,7 @@ static void tensor_sigma_impl(pqg_ci* ci, const T* C, T* S, int k, int64_t ld, h
   }
   const int nparts = npairs * split;
   const size_t need = (size_t)nparts * N * k * sizeof(T);
-  if (ci->scratch.n < need) ci->scratch.reset(ci->ctx, need);
+  grow(ci->scratch, ci->ctx, need, st);
   T* P = reinterpret_cast<T*>(ci->scratch.get());
   int64_t flops = 0;
   for (int q = 0; q < npairs; q++) flops += 2ll * a.p[q].M * (int64_t)a.p[q].M * a.p[q].rest * k;

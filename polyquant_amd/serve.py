@@ -37,7 +37,8 @@ class Server:
         for i in range(n.value):
             r = self._buf[i]
             out.append(dict(id=r.request_id, status=r.status, energies=np.array(r.energies[: r.nev]),
-                            t_submit=r.t_submit, t_admit=r.t_admit, t_done=r.t_done, iterations=r.iterations))
+                            t_submit=r.t_submit, t_admit=r.t_admit, t_done=r.t_done, iterations=r.iterations,
+                            message=r.message.decode(errors="replace")))
         return out
 
     def stats(self):

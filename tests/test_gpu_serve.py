@@ -53,7 +53,7 @@ def test_admission_by_budget_and_failure_isolation():
     bad_mem = srv.submit(huge)
     bad_conv = srv.submit(wl.small_problem(77, 8, 3, 3), maxit=2, tol=1e-12)
     got = srv.drain(len(ids) + 2)
-    assert all(got[i]["status"] == 0 for i in ids)
+    assert all(got[i]["status"] == 0 for i in ids), [(i, got[i]["status"], got[i]["message"]) for i in ids]
     assert got[bad_mem]["status"] == 3  # PQG_ERR_OOM
     assert got[bad_conv]["status"] == 4  # PQG_ERR_NOCONV
     q, res, b = srv.stats()
