@@ -244,8 +244,8 @@ def bench_serve(args):
             seq_fail += 1
             log(f"[serve] sequential request {i} failed: {e}")
         seq_t.append(time.perf_counter() - t0)
-        if i % 32 == 31:
-            log(f"[serve] sequential {i + 1}/{n} done, {time.perf_counter() - t_all:.1f}s")
+        log(f"[serve] sequential {i + 1}/{n}: {prob.n_dets_fci()} determinants, {prob.nroots} roots, "
+            f"{seq_t[-1]:.3f}s (total {time.perf_counter() - t_all:.1f}s)")
     seq_total = time.perf_counter() - t_all
     # serving: replay the arrival times, mixed precision with FP64 correction
     srv = Server(ctx, max_slots=args.serve_slots)
@@ -261,8 +261,8 @@ def bench_serve(args):
             nxt += 1
         for r in srv.poll():
             got[ids[r["id"]]] = r
-            if len(got) % 64 == 0:
-                log(f"[serve] served {len(got)}/{n}, {time.perf_counter() - t_start:.1f}s")
+            log(f"[serve] served {len(got)}/{n}: request {ids[r['id']]} status {r['status']}, "
+                f"latency {r['t_done'] - r['t_submit']:.3f}s ({time.perf_counter() - t_start:.1f}s)")
         time.sleep(0.0005)
     makespan = time.perf_counter() - t_start
     srv.close()

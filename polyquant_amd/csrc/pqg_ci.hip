@@ -232,7 +232,7 @@ __global__ void k_hdiag(DiagArgs a, double* __restrict__ Hd) {
 constexpr int kCPT = 16;   // max columns per thread
 constexpr int kGMax = 8;   // excited rows of the row string staged per block (row x column doubles)
 
-enum PassFlags { F_DIAG = 1, F_COLS = 2, F_MIXED = 4, F_ACCUM = 8, F_NODBL = 16 };
+enum PassFlags { F_DIAG = 1, F_COLS = 2, F_MIXED = 4, F_ACCUM = 8, F_NODBL = 16, F_NOPAIR = 32 };
 
 template <typename T>
 struct PassArgs {
@@ -389,7 +389,8 @@ __device__ __forceinline__ T column_terms(const PassArgs<T>& a, int64_t row, int
         sum += Dt[(size_t)e * s.nstr] * cbase[(int64_t)jt * cst];
       });
     }
-    // one single in each of two column strings
+    // one single in each of two column strings (unless the link GEMM forms them)
+    if (a.flags & F_NOPAIR) continue;
 #pragma unroll
     for (int c2 = c + 1; c2 < NC; c2++) {
       const int u = a.cs[c2];
@@ -988,7 +989,7 @@ void ensure_mixed(pqg_ci* ci) {
     }
   }
   if (ci->lg.enabled)
-    for (int r = 0; r < 2; r++) to_f32(ctx, ci->stream, ci->V64[r][r], ci->lg.Vrr32[r]);
+    for (int t = 0; t < ci->nslot; t++) to_f32(ctx, ci->stream, ci->V64[t][t], ci->lg.V32s[t]);
   PQG_HIP(hipStreamSynchronize(ci->stream));
   ci->mixed_ready = true;
 }
@@ -1297,19 +1298,50 @@ static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k,
     rowsB.push_back(t);
   }
   if (ci->lg.enabled) {
-    // diagonal and beta singles (with their alpha spectator sums)
-    launch_pass<T>(ci, tb, rowsA, colsA, C, ld, S, ld, k, F_DIAG | F_COLS | F_NODBL, st);
-    const size_t need = 2 * (size_t)N * k * sizeof(T);
-    grow(ci->scratch, ci->ctx, need, st);
+    // at most four vectors at a time, so the work copies stay bounded for large N
+    constexpr int kGroup = 4;
+    const int64_t n1 = ci->slot[1].nstr;
+    const int64_t nP = M / n1;  // positron strings per (alpha, beta) pair
+    const int kmax = std::min(k, kGroup);
+    grow(ci->scratch, ci->ctx, (nP > 1 ? 4 : 2) * (size_t)N * kmax * sizeof(T), st);
     T* CT = reinterpret_cast<T*>(ci->scratch.get());
-    T* ST = CT + (size_t)N * k;
-    transpose<T>(C, ld, CT, N, n0, M, k, false, st);
-    // alpha singles (with their beta spectator sums), then beta doubles, on the transpose
-    launch_pass<T>(ci, tb, rowsB, {0}, CT, N, ST, N, k, F_COLS | F_NODBL, st);
-    for (int v = 0; v < k; v++) link_gemm_pass(ci, 1, CT + (size_t)v * N, ST + (size_t)v * N, st);
-    transpose<T>(ST, N, S, ld, M, n0, k, true, st);
-    // alpha-beta single-single couplings and alpha doubles
-    for (int v = 0; v < k; v++) link_gemm_pass(ci, 0, C + (size_t)v * ld, S + (size_t)v * ld, st);
+    T* ST = CT + (size_t)N * kmax;
+    T* CP = ST + (size_t)N * kmax;
+    T* SP = CP + (size_t)N * kmax;
+    for (int v0 = 0; v0 < k; v0 += kGroup) {
+      const int kg = std::min(kGroup, k - v0);
+      const T* Cg = C + (size_t)v0 * ld;
+      T* Sg = S + (size_t)v0 * ld;
+      // diagonal and singles of every column set (with their spectator sums)
+      launch_pass<T>(ci, tb, rowsA, colsA, Cg, ld, Sg, ld, kg, F_DIAG | F_COLS | F_NODBL | F_NOPAIR, st);
+      transpose<T>(Cg, ld, CT, N, n0, M, kg, false, st);
+      // alpha singles (with their spectator sums), then beta doubles, on the transpose
+      launch_pass<T>(ci, tb, rowsB, {0}, CT, N, ST, N, kg, F_COLS | F_NODBL, st);
+      for (int v = 0; v < kg; v++) link_gemm_pass(ci, 1, CT + (size_t)v * N, ST + (size_t)v * N, st);
+      transpose<T>(ST, N, Sg, ld, M, n0, kg, true, st);
+      // couplings of beta singles with alpha / positron singles, and everything among alpha and
+      // positron strings; with positrons on C permuted to (alpha, positron strings, beta)
+      for (int v = 0; v < kg; v++) {
+        const T* Cv = Cg + (size_t)v * ld;
+        T* Sv = Sg + (size_t)v * ld;
+        if (nP == 1) {
+          link_gemm_pass(ci, 0, Cv, Sv, st);
+          continue;
+        }
+        T* Cp = CP + (size_t)v * N;
+        T* Sp = SP + (size_t)v * N;
+        for (int64_t z0 = 0; z0 < n0; z0 += 65535) {
+          const int64_t z = std::min<int64_t>(65535, n0 - z0);
+          transpose<T>(Cv + z0 * M, M, Cp + z0 * M, M, n1, nP, (int)z, false, st);
+        }
+        PQG_HIP(hipMemsetAsync(Sp, 0, (size_t)N * sizeof(T), st));
+        link_gemm_pass(ci, 0, Cp, Sp, st);
+        for (int64_t z0 = 0; z0 < n0; z0 += 65535) {
+          const int64_t z = std::min<int64_t>(65535, n0 - z0);
+          transpose<T>(Sp + z0 * M, M, Sv + z0 * M, M, nP, n1, (int)z, true, st);
+        }
+      }
+    }
     return;
   }
   const bool need_b = ci->slot[0].nsingle > 0;
@@ -1361,8 +1393,8 @@ uint64_t estimate_ci_bytes(const pqg_ci_desc* d, int nvec_max, int precision) {
   const uint64_t w = 8;
   uint64_t vec = N * w * (2ull * nvec_max + 8);  // V, W, work vectors
   vec += N * 8;                                  // diagonal
-  vec += N * (precision == PQG_MIXED ? 4 : 8) * 4ull * 2;  // sigma transposes
-  if (d->nspecies == 1) vec += lg_budget_bytes((int64_t)N, 8);  // link-GEMM chunk
+  vec += N * (precision == PQG_MIXED ? 4 : 8) * 4ull * 4;  // sigma transposes and permutations
+  vec += lg_budget_bytes((int64_t)N, 8);  // link-GEMM chunk
   return tables + vec + (64ull << 20);
 }
 

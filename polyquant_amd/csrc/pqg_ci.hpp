@@ -55,9 +55,9 @@ struct LinkTab {
 };
 struct LinkGemm {
   bool enabled = false;
-  LinkTab tab[2];
+  LinkTab tab[kSlots];
   DevBuf<char> U;             // GEMM output for one column chunk
-  DevBuf<float> Vrr32[2];     // FP32 copies of the same-set coupling blocks
+  DevBuf<float> V32s[kSlots]; // FP32 copies of the same-set coupling blocks
 };
 
 }  // namespace pqg
@@ -132,8 +132,10 @@ void tensor_sigma(pqg_ci* ci, const float* C, float* S, int k, int64_t ld, hipSt
 bool link_gemm_eligible(const pqg_ci* ci);
 void link_gemm_setup(pqg_ci* ci);
 size_t lg_budget_bytes(int64_t N, size_t w);
-// r = 0: alpha-beta single-single couplings + alpha doubles on the natural layout;
-// r = 1: beta doubles on the transposed vector.  Accumulates into S (one vector).
+// pass 0: couplings of e-beta singles with every other set's singles, and the doubles and
+// single-single couplings among e-alpha and positron sets (C with positron strings permuted
+// in front of e-beta); pass 1: e-beta doubles on the transposed vector.  Accumulates into S
+// (one vector).
 void link_gemm_pass(pqg_ci* ci, int r, const double* C, double* S, hipStream_t st);
 void link_gemm_pass(pqg_ci* ci, int r, const float* C, float* S, hipStream_t st);
 

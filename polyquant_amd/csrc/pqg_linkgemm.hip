@@ -38,19 +38,57 @@ constexpr int LG_MAXL = 128;
 // (L * LG_NT / 256); the kernel needs nb <= 128 and L <= 64
 constexpr int LG_TBMAX = 16, LG_TAMAX = 8;
 
+constexpr int LG_MAXSETS = 3;
+
+// Row strings are tuples over up to three string sets (e.g. electron alpha x positron alpha x
+// positron beta); the row index is sum_s i_s * stride_s.  Links of a row are the links of its
+// strings in set order, numbered from loff[s].
 template <typename T>
 struct LgArgs {
   const T* C;          // this vector, n_r x n_c row-major
   T* U;                // chunk scratch [n_r][L][cw]
-  const int2* links;   // links of the row set [n_r][L]: {J | odd << 31, Q << 16 | rev}
+  int nsets;
+  const int2* links[LG_MAXSETS];  // per set [n_s][L_s]: {J | odd << 31, Q << 16 | rev}
+  int Ls[LG_MAXSETS], nvs[LG_MAXSETS], ns[LG_MAXSETS], loff[LG_MAXSETS], np[LG_MAXSETS];
+  int64_t strides[LG_MAXSETS];
+  const T* Vrc[LG_MAXSETS];              // (P|Q): P a pair of row set s, Q a pair of the column set
+  const T* Vrr[LG_MAXSETS][LG_MAXSETS];  // (P|Q): P of row set s, Q of row set t (charges folded in)
   const int32_t* tbm;  // column-set pair map [nb][n_c]: (index << 1 | odd) or -1
-  const T* Vrc;        // (P|Q), P a pair of r, Q a pair of c, row-major
-  const T* Vrr;        // (P|Q), both pairs of r
-  int np_c, np_r;
-  int n_r, n_c, L, nv, nb, K, kq;
+  int np_c;
+  int n_r, n_c, L, nb, K;
   int c0, cw, gcols;
   int lab_off, kbf_off, crow_off, tile_off;  // LDS byte offsets (links at 0)
 };
+
+// kernel-argument arrays indexed by a run-time set number, without a copy to scratch
+template <class X>
+__device__ __forceinline__ X pick3(const X (&v)[LG_MAXSETS], int s) {
+  return s == 0 ? v[0] : (s == 1 ? v[1] : v[2]);
+}
+template <class X>
+__device__ __forceinline__ X pick33(const X (&v)[LG_MAXSETS][LG_MAXSETS], int s, int t) {
+  return s == 0 ? pick3(v[0], t) : (s == 1 ? pick3(v[1], t) : pick3(v[2], t));
+}
+
+// Links of row J into LDS: lq[e] = {target row | odd << 31, Q << 16 | rev}, lab[e] =
+// set << 16 | hole number << 8 | particle number.
+template <typename T>
+__device__ void lg_row_links(const LgArgs<T>& a, int64_t J, int2* lq, int* lab, int tid) {
+  for (int e = tid; e < a.L; e += 256) {
+    int s = 0;
+    if (a.nsets > 1 && e >= a.loff[1]) s = 1;
+    if (a.nsets > 2 && e >= a.loff[2]) s = 2;
+    const int el = e - pick3(a.loff, s);
+    const int64_t stride = pick3(a.strides, s);
+    const int64_t i = (J / stride) % pick3(a.ns, s);
+    const int2 l = pick3(a.links, s)[(size_t)i * pick3(a.Ls, s) + el];
+    const int64_t tgt = J + ((int64_t)(l.x & 0x7fffffff) - i) * stride;
+    lq[e] = make_int2((int)tgt | (l.x & (int)0x80000000), (l.y & ~0xffff) | (pick3(a.loff, s) + (l.y & 0xffff)));
+    const int nv = pick3(a.nvs, s);
+    const int ae = el / nv;
+    lab[e] = (s << 16) | (ae << 8) | (el - ae * nv);
+  }
+}
 
 // p of the off-diagonal pair number k = p (p - 1) / 2 + q, q < p
 __device__ __forceinline__ int offpair_hi(int k) {
@@ -76,15 +114,11 @@ __global__ __launch_bounds__(256) void k_lg_u(const LgArgs<T> a) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int L = a.L, K = a.K, nb = a.nb;
   constexpr int KP = 4 * KQ;  // rows of the LDS tile; rows K..KP-1 stay zero
-  // LDS: links of J, hole/particle numbers of each link, full pair index of each
+  // LDS: links of J, set/hole/particle numbers of each link, full pair index of each
   // off-diagonal pair of the column set, the staged row C(J, :), the B tile
   int* lab = reinterpret_cast<int*>(smem + a.lab_off);
   int* kbf = reinterpret_cast<int*>(smem + a.kbf_off);
-  for (int e = tid; e < L; e += 256) {
-    lq[e] = a.links[(size_t)J * L + e];
-    const int ae = e / a.nv;
-    lab[e] = (ae << 8) | (e - ae * a.nv);
-  }
+  lg_row_links(a, J, lq, lab, tid);
   for (int k = tid; k < nb; k += 256) kbf[k] = k + offpair_hi(k);
   if (nb > 0)
     for (int m = tid; m < a.n_c; m += 256) crow[m] = a.C[(size_t)J * a.n_c + m];
@@ -101,20 +135,27 @@ __global__ __launch_bounds__(256) void k_lg_u(const LgArgs<T> a) {
     const bool rowok = ep < L;
     const int Qp = rowok ? (lq[ep].y >> 16) : 0;
     const int abp = rowok ? lab[ep] : 0;
+    const int sp = abp >> 16;
 #pragma unroll
     for (int q = 0; q < KQ; q++) {
       const int k = 4 * q + (lane >> 4);
       T v = T(0);
       if (rowok && k < K) {
         if (k < nb) {
-          v = a.Vrc[(size_t)Qp * a.np_c + kbf[k]];
+          v = pick3(a.Vrc, sp)[(size_t)Qp * a.np_c + kbf[k]];
         } else {
           const int e = k - nb;
           const int abe = lab[e];
-          // same-set doubles: the two links share neither hole nor particle
-          if (((abe ^ abp) & 0xff00) && ((abe ^ abp) & 0xff)) {
-            const int2 l = lq[e];
-            v = (l.x < 0 ? T(-0.5) : T(0.5)) * a.Vrr[(size_t)Qp * a.np_r + (l.y >> 16)];
+          const int se = abe >> 16;
+          const int2 l = lq[e];
+          if (se == sp) {
+            // same-set doubles: the two links share neither hole nor particle
+            if (((abe ^ abp) & 0xff00) && ((abe ^ abp) & 0xff))
+              v = (l.x < 0 ? T(-0.5) : T(0.5)) * pick33(a.Vrr, sp, sp)[(size_t)Qp * pick3(a.np, sp) + (l.y >> 16)];
+          } else if (sp < se) {
+            // one excitation in each of two row sets: only the path whose first link lies in
+            // the lower set carries the coupling
+            v = (l.x < 0 ? T(-1) : T(1)) * pick33(a.Vrr, sp, se)[(size_t)Qp * pick3(a.np, se) + (l.y >> 16)];
           }
         }
       }
@@ -209,12 +250,14 @@ __global__ __launch_bounds__(256) void k_lg_u(const LgArgs<T> a) {
 
 // sigma(I, c0 + j) += sum_e s(I,e) U[J(I,e)][rev(I,e)][j] for the columns j of a chunk
 template <typename T, int CPT>
-__global__ __launch_bounds__(256) void k_lg_gather(const T* __restrict__ U, const int2* __restrict__ links, int L,
-                                                   int n_c, int c0, int cw, T* __restrict__ S) {
+__global__ __launch_bounds__(256) void k_lg_gather(const LgArgs<T> a, T* __restrict__ S) {
   __shared__ int2 lq[LG_MAXL];
-  const int I = blockIdx.x;
-  for (int e = threadIdx.x; e < L; e += 256) lq[e] = links[(size_t)I * L + e];
+  __shared__ int lab[LG_MAXL];
+  const int64_t I = blockIdx.x;
+  lg_row_links(a, I, lq, lab, threadIdx.x);
   __syncthreads();
+  const int L = a.L, cw = a.cw;
+  const T* __restrict__ U = a.U;
   const int j0 = blockIdx.y * 256 * CPT + threadIdx.x;
   T acc[CPT];
 #pragma unroll
@@ -232,7 +275,7 @@ __global__ __launch_bounds__(256) void k_lg_gather(const T* __restrict__ U, cons
       }
     }
   }
-  T* dst = S + (size_t)I * n_c + c0;
+  T* dst = S + (size_t)I * a.n_c + a.c0;
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
     const int col = j0 + 256 * j;
@@ -270,26 +313,80 @@ size_t lg_budget_bytes(int64_t N, size_t w) {
 // K steps are a compile-time bound (multiple of 8) so the MFMA loop has no branches
 static int lg_kq_bound(int kq) { return kq <= 48 ? (kq + 7) / 8 * 8 : 0; }
 
-bool link_gemm_eligible(const pqg_ci* ci) {
-  if (ci->nsp != 1 || !ci->fci || ci->nslot != 2) return false;
-  if (std::getenv("PQG_NO_LINK_GEMM")) return false;
-  for (int t = 0; t < 2; t++) {
-    const Slot& s = ci->slot[t];
-    if (!s.complete || s.nocc < 1 || s.nvirt < 1) return false;
-    const int L = s.nocc * s.nvirt;
-    if (L * LG_NT > 256 * LG_TAMAX) return false;
-    if (s.n_orb * (s.n_orb - 1) / 2 * LG_NT > 256 * LG_TBMAX) return false;
+// Row sets of the two passes.  Pass A (natural layout; with positrons, on C permuted to
+// (e-alpha, positron strings, e-beta)): rows = e-alpha x positron strings, with the links of
+// e-alpha and of every positron set that holds a particle; columns = e-beta.  Pass B
+// (transposed vector): rows = e-beta x positron strings with the e-beta links only; columns =
+// e-alpha.
+struct LgPass {
+  int nsets = 0;
+  int slot[LG_MAXSETS] = {0, 0, 0};
+  int64_t stride[LG_MAXSETS] = {1, 1, 1};
+  int col = 1;        // column slot
+  bool pairs = true;  // in-row block over the column set's off-diagonal pairs
+  int64_t n_r = 0;
+  int L = 0;
+};
+
+static int64_t positron_strings(const pqg_ci* ci) {
+  return ci->nslot == 4 ? (int64_t)ci->slot[2].nstr * ci->slot[3].nstr : 1;
+}
+
+static LgPass lg_pass_desc(const pqg_ci* ci, int pass) {
+  LgPass d;
+  const int64_t nP = positron_strings(ci);
+  if (pass == 0) {
+    d.col = 1;
+    d.pairs = true;
+    d.n_r = (int64_t)ci->slot[0].nstr * nP;
+    d.slot[d.nsets] = 0;
+    d.stride[d.nsets++] = nP;
+    if (ci->nslot == 4) {
+      if (ci->lg.tab[2].L > 0) {
+        d.slot[d.nsets] = 2;
+        d.stride[d.nsets++] = ci->slot[3].nstr;
+      }
+      if (ci->lg.tab[3].L > 0) {
+        d.slot[d.nsets] = 3;
+        d.stride[d.nsets++] = 1;
+      }
+    }
+  } else {
+    d.col = 0;
+    d.pairs = false;
+    d.n_r = (int64_t)ci->slot[1].nstr * nP;
+    d.slot[d.nsets] = 1;
+    d.stride[d.nsets++] = nP;
   }
-  // pass on the natural layout (r = alpha, with the alpha-beta term) and on the transpose
-  for (int r = 0; r < 2; r++) {
-    const Slot &sr = ci->slot[r], &sc = ci->slot[1 - r];
-    const int L = sr.nocc * sr.nvirt;
-    const int nb = r == 0 ? sc.n_orb * (sc.n_orb - 1) / 2 : 0;
-    const int kq = (nb + L + 3) / 4;
-    const int KQ = lg_kq_bound(kq);
+  for (int q = 0; q < d.nsets; q++) d.L += ci->lg.tab[d.slot[q]].L;
+  return d;
+}
+
+bool link_gemm_eligible(const pqg_ci* ci) {
+  if (!ci->fci || ci->nslot < 2) return false;
+  if (std::getenv("PQG_NO_LINK_GEMM")) return false;
+  for (int t = 0; t < ci->nslot; t++)
+    if (!ci->slot[t].complete) return false;
+  for (int t = 0; t < 2; t++)
+    if (ci->slot[t].nocc < 1 || ci->slot[t].nvirt < 1) return false;
+  const Slot& sb = ci->slot[1];
+  if (sb.n_orb * (sb.n_orb - 1) / 2 * LG_NT > 256 * LG_TBMAX) return false;
+  for (int pass = 0; pass < 2; pass++) {
+    // link counts as the pass would see them
+    int L = 0;
+    for (int t = 0; t < ci->nslot; t++) {
+      const bool in = pass == 0 ? (t != 1) : (t == 1);
+      if (in) L += ci->slot[t].nocc * ci->slot[t].nvirt;
+    }
+    if (L * LG_NT > 256 * LG_TAMAX) return false;
+    const Slot& sc = ci->slot[pass == 0 ? 1 : 0];
+    const int nb = pass == 0 ? sc.n_orb * (sc.n_orb - 1) / 2 : 0;
+    const int KQ = lg_kq_bound((nb + L + 3) / 4);
     if (!KQ) return false;
     const size_t lds = 2048 + (nb > 0 ? (size_t)sc.nstr * 8 : 0) + (size_t)4 * KQ * (LG_NT + 2) * 8;
     if (lds > 160 * 1024 - 1024) return false;
+    const int64_t nr = (int64_t)ci->slot[pass == 0 ? 0 : 1].nstr * positron_strings(ci);
+    if (nr >= (1ll << 31) - 1) return false;
   }
   return true;
 }
@@ -299,17 +396,18 @@ void link_gemm_setup(pqg_ci* ci) {
   lg.enabled = link_gemm_eligible(ci);
   if (!lg.enabled) return;
   int nmax = 0;
-  for (int t = 0; t < 2; t++) nmax = std::max(nmax, ci->slot[t].n_orb);
+  for (int t = 0; t < ci->nslot; t++) nmax = std::max(nmax, ci->slot[t].n_orb);
   std::vector<std::vector<uint64_t>> bn(nmax + 1, std::vector<uint64_t>(nmax + 2, 0));
   for (int n = 0; n <= nmax; n++)
     for (int k = 0; k <= nmax + 1; k++) bn[n][k] = lg_binom(n, k);
-  for (int t = 0; t < 2; t++) {
+  for (int t = 0; t < ci->nslot; t++) {
     const Slot& s = ci->slot[t];
     LinkTab& lt = lg.tab[t];
     const int n = s.n_orb, nv = s.nvirt;
     lt.L = s.nocc * nv;
     lt.nv = nv;
     lt.nb = n * (n - 1) / 2;
+    if (lt.L == 0) continue;
     std::vector<int2> links((size_t)s.nstr * lt.L);
     for (int i = 0; i < s.nstr; i++) {
       const uint64_t S = s.sorted[i];
@@ -335,7 +433,7 @@ void link_gemm_setup(pqg_ci* ci) {
     }
     lt.links.reset(ci->ctx, links.size());
     PQG_HIP(hipMemcpy(lt.links.get(), links.data(), links.size() * sizeof(int2), hipMemcpyHostToDevice));
-    if (t == 1) {  // only the beta set serves as the column set of the single-single term
+    if (t == 1) {  // only the electron beta set serves as the column set of the pair block
       std::vector<int32_t> map((size_t)lt.nb * s.nstr, -1);
       for (int p = 1; p < n; p++)
         for (int q = 0; q < p; q++) {
@@ -374,28 +472,54 @@ static void lg_go_kq(const LgArgs<T>& a, int KQ, dim3 grid, size_t lds, hipStrea
   }
 }
 
-// One pass of the link GEMM for one vector: rows = set r, columns = set c.
+static const double* vblock(pqg_ci* ci, int t, int u, double) { return ci->V64[t][u].get(); }
+static const float* vblock(pqg_ci* ci, int t, int u, float) {
+  return t == u ? ci->lg.V32s[t].get() : ci->V32[t][u].get();
+}
+
+// U bytes per column of a pass, and the chunk width that keeps U within its budget
+static size_t lg_per_col(const pqg_ci* ci, const LgPass& d, size_t w) { return (size_t)d.n_r * d.L * w; }
+static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w) {
+  const size_t per_col = lg_per_col(ci, d, w);
+  const size_t want = std::min<size_t>(lg_budget_bytes(ci->n_int, w), per_col * (size_t)n_c);
+  const size_t budget = std::max<size_t>(per_col * LG_NT, want);
+  int cw = (int)std::min<size_t>((size_t)n_c, budget / per_col);
+  if (cw < n_c) cw = std::max(LG_NT, cw / LG_NT * LG_NT);
+  return cw;
+}
+
+// One pass of the link GEMM for one vector.
 template <typename T>
-static void lg_pass(pqg_ci* ci, int r, const T* C, T* S, const T* Vrc, const T* Vrr, hipStream_t st) {
+static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, hipStream_t st) {
   LinkGemm& lg = ci->lg;
-  const int c = 1 - r;
-  const Slot &sr = ci->slot[r], &sc = ci->slot[c];
+  const LgPass d = lg_pass_desc(ci, pass);
+  if (d.L == 0) return;
+  const Slot& sc = ci->slot[d.col];
   LgArgs<T> a{};
   a.C = C;
-  a.links = lg.tab[r].links.get();
-  a.tbm = r == 0 ? lg.tab[c].tbm.get() : nullptr;
-  a.Vrc = Vrc;
-  a.Vrr = Vrr;
+  a.nsets = d.nsets;
+  int off = 0;
+  for (int q = 0; q < d.nsets; q++) {
+    const int t = d.slot[q];
+    a.links[q] = lg.tab[t].links.get();
+    a.Ls[q] = lg.tab[t].L;
+    a.nvs[q] = lg.tab[t].nv;
+    a.ns[q] = ci->slot[t].nstr;
+    a.loff[q] = off;
+    a.np[q] = ci->slot[t].npair;
+    a.strides[q] = d.stride[q];
+    a.Vrc[q] = vblock(ci, t, d.col, T{});
+    for (int q2 = 0; q2 < d.nsets; q2++) a.Vrr[q][q2] = vblock(ci, t, d.slot[q2], T{});
+    off += lg.tab[t].L;
+  }
+  a.tbm = d.pairs ? lg.tab[d.col].tbm.get() : nullptr;
   a.np_c = sc.npair;
-  a.np_r = sr.npair;
-  a.n_r = sr.nstr;
+  a.n_r = (int)d.n_r;
   a.n_c = sc.nstr;
-  a.L = lg.tab[r].L;
-  a.nv = lg.tab[r].nv;
-  a.nb = r == 0 ? lg.tab[c].nb : 0;
+  a.L = d.L;
+  a.nb = d.pairs ? lg.tab[d.col].nb : 0;
   a.K = a.nb + a.L;
-  a.kq = (a.K + 3) / 4;
-  const int KQ = lg_kq_bound(a.kq);
+  const int KQ = lg_kq_bound((a.K + 3) / 4);
   constexpr int NTP = LG_NT + 16 / (int)sizeof(T);
   auto r16 = [](size_t x) { return (x + 15) / 16 * 16; };
   a.lab_off = (int)r16((size_t)a.L * sizeof(int2));
@@ -403,19 +527,13 @@ static void lg_pass(pqg_ci* ci, int r, const T* C, T* S, const T* Vrc, const T* 
   a.crow_off = (int)r16(a.kbf_off + (size_t)a.nb * 4);
   a.tile_off = (int)r16(a.crow_off + (a.nb > 0 ? (size_t)a.n_c * sizeof(T) : 0));
   const size_t lds = a.tile_off + (size_t)4 * KQ * NTP * sizeof(T);
-  // column chunk: U holds n_r * L * cw values; the buffer is sized for the larger of the two
-  // passes so that it grows at most once
-  const size_t per_col = (size_t)a.n_r * a.L * sizeof(T);
-  const size_t want = std::min<size_t>(lg_budget_bytes(ci->n_int, sizeof(T)), per_col * (size_t)a.n_c);
-  const size_t budget = std::max<size_t>(per_col * LG_NT, want);
-  int cw = (int)std::min<size_t>((size_t)a.n_c, budget / per_col);
-  if (cw < a.n_c) cw = std::max(LG_NT, cw / LG_NT * LG_NT);
-  size_t need = per_col * (size_t)cw;
+  // U holds n_r * L * cw values; sized for the larger of the two passes so it grows once
+  const int cw = lg_chunk(ci, d, a.n_c, sizeof(T));
+  size_t need = lg_per_col(ci, d, sizeof(T)) * (size_t)cw;
   {
-    const Slot& so = ci->slot[c];
-    const size_t per_col_o = (size_t)so.nstr * lg.tab[c].L * sizeof(T);
-    const size_t want_o = std::min<size_t>(lg_budget_bytes(ci->n_int, sizeof(T)), per_col_o * (size_t)sr.nstr);
-    need = std::max(need, std::max<size_t>(per_col_o * LG_NT, want_o) + per_col_o * LG_NT);
+    const LgPass o = lg_pass_desc(ci, 1 - pass);
+    const int n_co = ci->slot[o.col].nstr;
+    need = std::max(need, lg_per_col(ci, o, sizeof(T)) * (size_t)lg_chunk(ci, o, n_co, sizeof(T)));
   }
   grow(lg.U, ci->ctx, need, st);
   a.U = reinterpret_cast<T*>(lg.U.get());
@@ -430,8 +548,7 @@ static void lg_pass(pqg_ci* ci, int r, const T* C, T* S, const T* Vrc, const T* 
     dim3 grid((unsigned)a.n_r, (unsigned)lg_grid(a.cw, a.gcols));
     {
       ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "lg_gemm_f64" : "lg_gemm_f32", flops_per_col * a.cw);
-      if (mb <= 1) lg_go_kq<T, 1, 2>(a, KQ, grid, lds, st);
-      else if (mb <= 2) lg_go_kq<T, 1, 2>(a, KQ, grid, lds, st);
+      if (mb <= 2) lg_go_kq<T, 1, 2>(a, KQ, grid, lds, st);
       else lg_go_kq<T, 1, 1>(a, KQ, grid, lds, st);
       PQG_HIP(hipGetLastError());
     }
@@ -440,17 +557,13 @@ static void lg_pass(pqg_ci* ci, int r, const T* C, T* S, const T* Vrc, const T* 
                      (int64_t)a.n_r * a.cw * (a.L + 2) * (int64_t)sizeof(T));
       constexpr int CPT = 4;
       dim3 g2((unsigned)a.n_r, (unsigned)lg_grid(a.cw, 256 * CPT));
-      k_lg_gather<T, CPT><<<g2, 256, 0, st>>>(a.U, a.links, a.L, a.n_c, a.c0, a.cw, S);
+      k_lg_gather<T, CPT><<<g2, 256, 0, st>>>(a, S);
       PQG_HIP(hipGetLastError());
     }
   }
 }
 
-void link_gemm_pass(pqg_ci* ci, int r, const double* C, double* S, hipStream_t st) {
-  lg_pass<double>(ci, r, C, S, ci->V64[0][1].get(), ci->V64[r][r].get(), st);
-}
-void link_gemm_pass(pqg_ci* ci, int r, const float* C, float* S, hipStream_t st) {
-  lg_pass<float>(ci, r, C, S, ci->V32[0][1].get(), ci->lg.Vrr32[r].get(), st);
-}
+void link_gemm_pass(pqg_ci* ci, int pass, const double* C, double* S, hipStream_t st) { lg_pass<double>(ci, pass, C, S, st); }
+void link_gemm_pass(pqg_ci* ci, int pass, const float* C, float* S, hipStream_t st) { lg_pass<float>(ci, pass, C, S, st); }
 
 }  // namespace pqg

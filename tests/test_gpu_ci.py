@@ -200,6 +200,11 @@ LINK_GEMM_CASES = {
     "fci_unrestricted_6o_3a2b": lambda: wl.small_problem(103, 6, 3, 2, nspin=2),
     "fci_unrestricted_9o_2a4b": lambda: wl.small_problem(114, 9, 2, 4, nspin=2),
     "fci_12o_6a5b": lambda: wl.small_problem(115, 12, 6, 5),
+    # with positrons: rows are (electron alpha, positron) strings, positron links in the GEMM
+    "ep_fci_6e_2a1b_5p_1": lambda: wl.small_problem(106, 6, 2, 1, positron=(5, 1, 0)),
+    "ep_fci_5e_2a2b_4p_2a1b": lambda: wl.small_problem(107, 5, 2, 2, positron=(4, 2, 1)),
+    "ep_fci_9e_3a3b_7p_1": lambda: wl.small_problem(116, 9, 3, 3, positron=(7, 1, 0)),
+    "ep_fci_unrestricted_7e_3a2b_6p_2a0b": lambda: wl.small_problem(117, 7, 3, 2, nspin=2, positron=(6, 2, 0)),
 }
 
 
