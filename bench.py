@@ -214,31 +214,22 @@ def bench_fci16(args):
     ctx.close()
 
 
-def bench_serve(args):
-    """Serving loop: 512 CI requests with Poisson arrivals (200/s), mixed precision, vs sequential FP64."""
-    import numpy as np
-    import torch
+def _serve_sequential(args, reqs, ctx, nvec):
+    """Sequential plain FP64 solves, one request at a time: per-request seconds and energies."""
+    from polyquant_amd._lib import PQG_FP64
+    from polyquant_amd.ci import DetSet
 
-    from polyquant_amd import workloads as wl
-    from polyquant_amd._lib import PQG_FP64, PQG_MIXED
-    from polyquant_amd.ci import Context, DetSet
-    from polyquant_amd.serve import Server
-
-    assert torch.cuda.is_available(), "no GPU visible"
-    reqs = wl.serving_requests()[: args.serve_n]
     n = len(reqs)
-    ctx = Context(0, hbm_budget=200 << 30)
-    nvec = 5
-    # sequential plain FP64 solves, one request at a time
     seq_t, seq_E, seq_fail = [], [], 0
     t_all = time.perf_counter()
     for i, (_, prob) in enumerate(reqs):
         t0 = time.perf_counter()
         try:
             ds = DetSet(prob, ctx)
-            E, _, _ = ds.davidson(prob.nroots, nvec_init=nvec, tol=1e-6, precision=PQG_FP64, want_vectors=False)
+            E, _, _ = ds.davidson(prob.nroots, nvec_init=nvec, tol=1e-6, precision=PQG_FP64, want_vectors=False,
+                                  perturb=args.serve_perturb)
             ds.close()
-            seq_E.append(E)
+            seq_E.append([float(x) for x in E])
         except Exception as e:  # a failed request is counted, the loop goes on
             seq_E.append(None)
             seq_fail += 1
@@ -246,17 +237,23 @@ def bench_serve(args):
         seq_t.append(time.perf_counter() - t0)
         log(f"[serve] sequential {i + 1}/{n}: {prob.n_dets_fci()} determinants, {prob.nroots} roots, "
             f"{seq_t[-1]:.3f}s (total {time.perf_counter() - t_all:.1f}s)")
-    seq_total = time.perf_counter() - t_all
-    # serving: replay the arrival times, mixed precision with FP64 correction
+    return {"total_s": time.perf_counter() - t_all, "seconds": seq_t, "energies": seq_E, "failed": seq_fail}
+
+
+def _serve_loop(args, reqs, ctx, nvec):
+    """Replays the arrival times into the serving loop (mixed precision, FP64 correction)."""
+    from polyquant_amd._lib import PQG_MIXED
+    from polyquant_amd.serve import Server
+
+    n = len(reqs)
     srv = Server(ctx, max_slots=args.serve_slots)
     got, ids = {}, {}
     t_start = time.perf_counter()
-    c0 = srv.clock()
     nxt = 0
     while len(got) < n:
         now = time.perf_counter() - t_start
         while nxt < n and reqs[nxt][0] <= now:
-            rid = srv.submit(reqs[nxt][1], nvec_init=nvec, tol=1e-6, precision=PQG_MIXED)
+            rid = srv.submit(reqs[nxt][1], nvec_init=nvec, tol=1e-6, precision=PQG_MIXED, perturb=args.serve_perturb)
             ids[rid] = nxt
             nxt += 1
         for r in srv.poll():
@@ -266,29 +263,90 @@ def bench_serve(args):
         time.sleep(0.0005)
     makespan = time.perf_counter() - t_start
     srv.close()
-    lat = np.array([got[i]["t_done"] - got[i]["t_submit"] for i in range(n)])
-    ok = [i for i in range(n) if got[i]["status"] == 0 and seq_E[i] is not None]
-    dE = max((float(np.max(np.abs(got[i]["energies"] - seq_E[i]))) for i in ok), default=0.0)
-    seq_lat = np.array(seq_t)
-    out = {
+    return {"makespan_s": makespan,
+            "latency_s": [got[i]["t_done"] - got[i]["t_submit"] for i in range(n)],
+            "status": [int(got[i]["status"]) for i in range(n)],
+            "energies": [[float(x) for x in got[i]["energies"]] for i in range(n)]}
+
+
+def _merge_legs(parts):
+    """Sequential legs run over consecutive request ranges, concatenated."""
+    out = {"total_s": 0.0, "seconds": [], "energies": [], "failed": 0}
+    for p in parts:
+        out["total_s"] += p["total_s"]
+        out["seconds"] += p["seconds"]
+        out["energies"] += p["energies"]
+        out["failed"] += p["failed"]
+    return out
+
+
+def _serve_report(args, n, seq, srv):
+    import numpy as np
+
+    lat = np.array(srv["latency_s"])
+    solved = sum(1 for i in range(n) if srv["status"][i] == 0)
+    ok = [i for i in range(n) if srv["status"][i] == 0 and seq["energies"][i] is not None]
+    dE = max((float(np.max(np.abs(np.array(srv["energies"][i]) - np.array(seq["energies"][i])))) for i in ok),
+             default=0.0)
+    seq_lat = np.array(seq["seconds"])
+    makespan = srv["makespan_s"]
+    return {
         "metric": "CI solves/s + p99 latency (serving loop vs sequential plain FP64)",
-        "value": round(n / makespan, 3), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "value": round(solved / makespan, 3), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "mixed (FP32 sigma, FP64 correction)", "data": "synthetic",
         "config": {"workload": f"serving: {n} CI requests, Poisson arrivals 200/s, n_orb 6-14, alpha/beta 2..n/2, "
                                "positron 0/1 with 6-12 orbitals, 1-4 roots, HBM budget 200 GB",
-                   "max_slots": args.serve_slots, "num_subspace_vec": nvec},
-        "serving": {"makespan_s": round(makespan, 3), "solves_per_s": round(n / makespan, 3),
+                   "max_slots": args.serve_slots, "num_subspace_vec": args.serve_nvec,
+                   "start_perturbation": args.serve_perturb, "admission": "smallest work first, oldest first after 30 s"},
+        "serving": {"makespan_s": round(makespan, 3), "solves_per_s": round(solved / makespan, 3),
                     "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
                     "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-                    "failed": int(sum(1 for i in range(n) if got[i]["status"] != 0))},
-        "sequential_fp64": {"total_s": round(seq_total, 3), "solves_per_s": round(n / seq_total, 3),
+                    "failed": int(sum(1 for i in range(n) if srv["status"][i] != 0))},
+        "sequential_fp64": {"total_s": round(seq["total_s"], 3), "solves_per_s": round(n / seq["total_s"], 3),
                             "p50_solve_s": round(float(np.percentile(seq_lat, 50)), 4),
-                            "p99_solve_s": round(float(np.percentile(seq_lat, 99)), 4), "failed": seq_fail},
-        "speedup_solves_per_s": round(seq_total / makespan, 3),
+                            "p99_solve_s": round(float(np.percentile(seq_lat, 99)), 4), "failed": seq["failed"]},
+        "speedup_solves_per_s": round((solved / makespan) / ((n - seq["failed"]) / seq["total_s"]), 3),
         "max_abs_dE_serving_vs_sequential": dE,
     }
-    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Serving loop: 512 CI requests with Poisson arrivals (200/s), mixed precision, vs sequential FP64.
+
+    --serve-leg sequential|serving runs one leg and writes it to --serve-save (JSON); --serve-combine
+    SEQ SRV prints the report of two saved legs.  The default runs both legs in one process."""
+    import torch
+
+    from polyquant_amd import workloads as wl
+    from polyquant_amd.ci import Context
+
+    reqs = wl.serving_requests()[: args.serve_n]
+    n = len(reqs)
+    # num_subspace_vec of every request (both legs); the largest subspace is the reference's
+    # max(2 * num_subspace_vec, 10 * roots) (epci.cpp:589-608)
+    nvec = args.serve_nvec
+    if args.serve_combine:
+        seq = _merge_legs([json.load(open(f)) for f in args.serve_combine[0].split(",")])
+        srv = json.load(open(args.serve_combine[1]))
+        print(json.dumps(_serve_report(args, n, seq, srv)), flush=True)
+        return
+    assert torch.cuda.is_available(), "no GPU visible"
+    ctx = Context(0, hbm_budget=200 << 30)
+    seq = srv = None
+    if args.serve_leg in ("both", "sequential"):
+        lo, hi = args.serve_range if args.serve_range else (0, n)
+        seq = _serve_sequential(args, reqs[lo:hi], ctx, nvec)
+    if args.serve_leg in ("both", "serving"):
+        srv = _serve_loop(args, reqs, ctx, nvec)
     ctx.close()
+    if args.serve_save:
+        json.dump(seq if args.serve_leg == "sequential" else srv, open(args.serve_save, "w"))
+    if seq is not None and srv is not None:
+        print(json.dumps(_serve_report(args, n, seq, srv)), flush=True)
+    else:
+        leg = seq if seq is not None else srv
+        total = leg.get("total_s", leg.get("makespan_s"))
+        print(json.dumps({"serve_leg": args.serve_leg, "requests": n, "seconds": round(total, 3)}), flush=True)
 
 
 def bench_h2o(args):
@@ -361,6 +419,15 @@ def main():
     ap.add_argument("--h2o-nvec", type=int, default=5)
     ap.add_argument("--serve-n", type=int, default=512)
     ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-leg", default="both", choices=["both", "sequential", "serving"])
+    ap.add_argument("--serve-range", nargs=2, type=int, default=None, metavar=("LO", "HI"),
+                    help="sequential leg over requests [LO, HI) only")
+    ap.add_argument("--serve-nvec", type=int, default=8)
+    ap.add_argument("--serve-perturb", type=float, default=1e-2,
+                    help="start-vector perturbation of every request (both legs); 0 = unit vectors")
+    ap.add_argument("--serve-save", default=None, help="write the leg's per-request results (JSON)")
+    ap.add_argument("--serve-combine", nargs=2, default=None, metavar=("SEQ", "SRV"),
+                    help="print the serving report from saved legs (SEQ: comma-separated sequential parts)")
     ap.add_argument("--fci-nvec", type=int, default=8)
     ap.add_argument("--fci-nmax", type=int, default=32)
     ap.add_argument("--fci-maxit", type=int, default=200)

@@ -67,6 +67,12 @@ typedef struct {
   int maxit;
   double tol;                      /* residual-norm tolerance                            */
   int precision;
+  /* 0: start from unit vectors at the smallest diagonal entries (the reference's start).
+   * > 0 (no counterpart in the reference): add a deterministic pseudo-random admixture of
+   * norm ~guess_perturb to each start vector, so the start overlaps every eigenvector (a
+   * unit-vector start can converge to an excited root when the ground state has no weight
+   * on the lowest-diagonal determinants).                                                  */
+  double guess_perturb;
 } pqg_dav_params;
 
 typedef struct {

@@ -42,6 +42,7 @@ class DavParams(ct.Structure):
         ("maxit", ct.c_int),
         ("tol", ct.c_double),
         ("precision", ct.c_int),
+        ("guess_perturb", ct.c_double),
     ]
 
 

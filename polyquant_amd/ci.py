@@ -205,8 +205,8 @@ class DetSet:
 
     # --- solver -------------------------------------------------------------------------
     def davidson(self, nev: int, nvec_init: int = 5, nvec_max: int = 0, maxit: int = 500, tol: float = 1e-6,
-                 precision: int = _lib.PQG_FP64, want_vectors: bool = True):
-        p = DavParams(nev, nvec_init, nvec_max, maxit, tol, precision)
+                 precision: int = _lib.PQG_FP64, want_vectors: bool = True, perturb: float = 0.0):
+        p = DavParams(nev, nvec_init, nvec_max, maxit, tol, precision, perturb)
         E = np.zeros(nev)
         X = np.zeros((self.N_dets, nev), order="F") if want_vectors else None
         rep = DavReport()

@@ -1373,29 +1373,42 @@ __global__ void k_f2d(const float* __restrict__ x, double* __restrict__ y, int64
   if (i < n) y[i] = (double)x[i];
 }
 
-uint64_t estimate_ci_bytes(const pqg_ci_desc* d, int nvec_max, int precision) {
+// Device bytes one solve allocates (admission control of the serving loop): the string
+// tables and integrals, the Davidson subspace and work blocks (as Davidson::Davidson sizes them),
+// the sigma work copies of the largest block and the link-GEMM chunk.
+uint64_t estimate_ci_bytes(const pqg_ci_desc* d, const pqg_dav_params* p) {
   uint64_t N = 1, tables = 0;
-  for (int p = 0; p < d->nspecies; p++) {
-    const int n = d->n_orb[p];
+  int one_particle_sets = 0;
+  bool positrons = d->nspecies == 2;
+  for (int sp = 0; sp < d->nspecies; sp++) {
+    const int n = d->n_orb[sp];
     const uint64_t np = (uint64_t)n * (n + 1) / 2;
     for (int s = 0; s < 2; s++) {
-      const int k = s == 0 ? d->n_alpha[p] : d->n_beta[p];
+      const int k = s == 0 ? d->n_alpha[sp] : d->n_beta[sp];
       const uint64_t ns = binom(n, k);
       N *= ns;
       const uint64_t nv = n - k;
+      one_particle_sets += k == 1;
       tables += ns * (8 + 4 + 8) + (uint64_t)((n + 7) / 8) * 256 * (k + 1) * 4;
-      tables += ns * (uint64_t)k * nv * 12;                                  // A (+FP32)
+      tables += ns * (uint64_t)k * nv * (12 + 8);                            // A (+FP32), links
       tables += ns * (uint64_t)(k * (k - 1) / 2) * (nv * (nv - 1) / 2) * 12;  // D (+FP32)
-      tables += ns * np * 12 * 2;                                            // spectator sums
+      tables += ns * np * 12 * 2 + ns * (uint64_t)n * (n - 1) / 2 * 4;        // spectator sums, pair map
     }
     tables += np * np * 12 * 8;  // coupling blocks
   }
-  const uint64_t w = 8;
-  uint64_t vec = N * w * (2ull * nvec_max + 8);  // V, W, work vectors
-  vec += N * 8;                                  // diagonal
-  vec += N * (precision == PQG_MIXED ? 4 : 8) * 4ull * 4;  // sigma transposes and permutations
-  vec += lg_budget_bytes((int64_t)N, 8);  // link-GEMM chunk
-  return tables + vec + (64ull << 20);
+  const uint64_t nev = (uint64_t)std::max(1, p->nev);
+  const uint64_t ninit = std::min<uint64_t>(std::max<uint64_t>(p->nvec_init, nev), N);
+  uint64_t nmax = p->nvec_max > 0 ? (uint64_t)p->nvec_max : std::max<uint64_t>(2ull * p->nvec_init, 10 * nev);
+  nmax = std::min<uint64_t>(std::max(nmax, ninit + nev), std::min<uint64_t>(N, 64));
+  uint64_t dav = N * 8 * (2 * nmax + 4 * nev + ninit);  // V, W, X, AX, [R T], restart block
+  if (p->precision == PQG_MIXED) dav += N * 8 * nev + 2 * N * 4 * std::max(nev, ninit);
+  const uint64_t kb = std::max(ninit, nev);
+  // sigma work copies (FP64 is the larger): transposes (+ positron permutations) of up to four
+  // vectors, the enumeration kernel's transposes of the whole block, or the pair-GEMM partials
+  uint64_t sig = std::max<uint64_t>((positrons ? 4 : 2) * N * std::min<uint64_t>(kb, 4) * 8, 2 * N * kb * 8);
+  if (one_particle_sets >= 2) sig = std::max<uint64_t>(sig, 16ull * 6 * N * kb * 8);
+  const uint64_t other = N * 8 /* diagonal */ + lg_budget_bytes((int64_t)N, 8);
+  return tables + dav + sig + other + (64ull << 20);
 }
 
 }  // namespace pqg

@@ -123,7 +123,7 @@ void to_reference(pqg_ci* ci, const double* in, int64_t ldi, double* ref, int64_
 template <typename T>
 void mask_invalid(pqg_ci* ci, T* v, int64_t ld, int k, hipStream_t st);
 // bytes of device memory a CI of this description needs (admission control)
-uint64_t estimate_ci_bytes(const pqg_ci_desc* d, int nvec_max, int precision);
+uint64_t estimate_ci_bytes(const pqg_ci_desc* d, const pqg_dav_params* p);
 pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* desc, hipStream_t stream);
 bool tensor_eligible(const pqg_ci* ci);
 void tensor_setup(pqg_ci* ci, const std::vector<const double*>& h1);

@@ -17,53 +17,77 @@ void set_error(const std::string& msg) { t_err = msg; }
 
 static constexpr uint64_t kAlign = 256;
 static constexpr uint64_t kMinChunk = 256ull << 20;
+// Blocks at least this large get a chunk of their own that goes back to the device when the
+// block is freed, so long-lived small blocks cannot fragment the budget for large solves.
+static constexpr uint64_t kDedicated = 1ull << 30;
 
 static uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 Arena::~Arena() {
-  for (auto& c : chunks_) (void)hipFree(c.base);
+  for (auto& c : chunks_)
+    if (c.base) (void)hipFree(c.base);
 }
 
 bool Arena::fits(uint64_t bytes) const {
   std::lock_guard<std::mutex> lk(mu_);
   bytes = round_up(std::max<uint64_t>(bytes, 1), kAlign);
   for (const auto& c : chunks_)
-    for (const auto& [off, sz] : c.free_)
-      if (sz >= bytes) return true;
+    if (c.base && !c.dedicated && bytes < kDedicated)
+      for (const auto& [off, sz] : c.free_)
+        if (sz >= bytes) return true;
   return reserved_ + std::max(bytes, std::min(kMinChunk, budget_ - std::min(budget_, reserved_))) <= budget_ ||
          reserved_ + bytes <= budget_;
+}
+
+bool Arena::release_empty_locked() {
+  bool any = false;
+  for (auto& c : chunks_) {
+    if (!c.base || c.live > 0) continue;
+    (void)hipFree(c.base);
+    reserved_ -= c.size;
+    c.base = nullptr;
+    c.free_.clear();
+    any = true;
+  }
+  return any;
 }
 
 void* Arena::alloc(uint64_t bytes) {
   std::lock_guard<std::mutex> lk(mu_);
   bytes = round_up(std::max<uint64_t>(bytes, 1), kAlign);
-  for (size_t ci = 0; ci < chunks_.size(); ci++) {
-    auto& c = chunks_[ci];
-    for (auto it = c.free_.begin(); it != c.free_.end(); ++it) {
-      if (it->second >= bytes) {
-        uint64_t off = it->first, sz = it->second;
-        c.free_.erase(it);
-        if (sz > bytes) c.free_[off + bytes] = sz - bytes;
-        char* p = c.base + off;
-        live_[p] = {(int)ci, bytes};
-        used_ += bytes;
-        peak_ = std::max(peak_, used_);
-        return p;
+  if (bytes < kDedicated) {
+    for (size_t ci = 0; ci < chunks_.size(); ci++) {
+      auto& c = chunks_[ci];
+      if (!c.base || c.dedicated) continue;
+      for (auto it = c.free_.begin(); it != c.free_.end(); ++it) {
+        if (it->second >= bytes) {
+          uint64_t off = it->first, sz = it->second;
+          c.free_.erase(it);
+          if (sz > bytes) c.free_[off + bytes] = sz - bytes;
+          char* p = c.base + off;
+          live_[p] = {(int)ci, bytes};
+          c.live++;
+          used_ += bytes;
+          peak_ = std::max(peak_, used_);
+          return p;
+        }
       }
     }
   }
-  // new chunk
+  // new chunk (empty chunks are given back first when the budget would be exceeded)
+  if (reserved_ + bytes > budget_) release_empty_locked();
   if (reserved_ + bytes > budget_)
     throw Error{PQG_ERR_OOM, "HBM budget exceeded: need " + std::to_string(bytes) + " B, " +
                                  std::to_string(budget_ - std::min(budget_, used_)) + " B free of budget"};
-  uint64_t csize = std::max(bytes, kMinChunk);
+  const bool dedicated = bytes >= kDedicated;
+  uint64_t csize = dedicated ? bytes : std::max(bytes, kMinChunk);
   csize = std::min(csize, budget_ - reserved_);
   csize = std::max(csize, bytes);
   char* base = nullptr;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&base), csize);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    if (csize > bytes) {  // retry with the exact size
+    if (release_empty_locked() || csize > bytes) {  // retry with the exact size
       csize = bytes;
       e = hipMalloc(reinterpret_cast<void**>(&base), csize);
     }
@@ -73,10 +97,21 @@ void* Arena::alloc(uint64_t bytes) {
     }
   }
   reserved_ += csize;
-  Chunk c{base, csize, {}};
+  Chunk c{base, csize, {}, dedicated, 1};
   if (csize > bytes) c.free_[bytes] = csize - bytes;
-  chunks_.push_back(std::move(c));
-  live_[base] = {(int)chunks_.size() - 1, bytes};
+  int idx = -1;
+  for (size_t ci = 0; ci < chunks_.size(); ci++)
+    if (!chunks_[ci].base) {  // reuse the slot of a released chunk
+      idx = (int)ci;
+      break;
+    }
+  if (idx < 0) {
+    chunks_.push_back(std::move(c));
+    idx = (int)chunks_.size() - 1;
+  } else {
+    chunks_[idx] = std::move(c);
+  }
+  live_[base] = {idx, bytes};
   used_ += bytes;
   peak_ = std::max(peak_, used_);
   return base;
@@ -92,6 +127,14 @@ void Arena::free(void* vp) {
   live_.erase(it);
   used_ -= sz;
   auto& c = chunks_[ci];
+  c.live--;
+  if (c.dedicated) {  // the owner has finished its work on this block (stream synchronised)
+    (void)hipFree(c.base);
+    reserved_ -= c.size;
+    c.base = nullptr;
+    c.free_.clear();
+    return;
+  }
   uint64_t off = p - c.base;
   auto nx = c.free_.lower_bound(off);
   if (nx != c.free_.end() && nx->first == off + sz) {  // merge right

@@ -2,9 +2,9 @@
 //
 // Restates the behaviour of the eigensolver the reference drives at src/ci/epci.cpp:589-621
 // (Spectra DavidsonSymEigsSolver, SortRule::SmallestAlge): start from unit vectors at the
-// nvec_init smallest diagonal entries, extend by t_k = r_k / (theta_k - diag), restart on the
-// current Ritz vectors when the basis would exceed nvec_max, converge when every one of the
-// nev residual norms is below tol.
+// nvec_init smallest diagonal entries, extend by t_k = r_k / (theta_k - diag) for the roots whose
+// residual is not yet below tol, restart on the current Ritz vectors when the basis would exceed
+// nvec_max, converge when every one of the nev residual norms is below tol.
 //
 // Everything stays in HBM: tall-skinny products (V^T W, V Y, Gram-Schmidt) are FP64 kernels
 // with a deterministic two-level reduction, and the projected eigenproblem is solved by a
@@ -158,6 +158,21 @@ __global__ void k_unit(double* __restrict__ v, int64_t n, int64_t ld, const int6
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int j = 0; j < k; j++) v[i + j * ld] = (i == pos[j]) ? 1.0 : 0.0;
+}
+// unit vectors at pos[j] plus eps * u(i, j), u uniform in [-1, 1) from a SplitMix64 hash of
+// (j, i): a start that overlaps every eigenvector
+__global__ void k_unit_perturbed(double* __restrict__ v, int64_t n, int64_t ld, const int64_t* __restrict__ pos, int k,
+                                 double eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int j = 0; j < k; j++) {
+    uint64_t z = (uint64_t)i * 64ull + (uint64_t)j + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    v[i + j * ld] = ((i == pos[j]) ? 1.0 : 0.0) + eps * u;
+  }
 }
 __global__ void k_scale_cols(double* __restrict__ v, int64_t n, int64_t ld, const double* __restrict__ s, int k) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -349,12 +364,15 @@ __global__ __launch_bounds__(1024) void k_jacobi(double* __restrict__ G, int m, 
 
 // Cholesky factor of a small Gram matrix B (n x n, leading dimension 64) with column dropping,
 // on the device: column j is kept when its residual norm after the kept columns before it
-// exceeds thr_j = max(rel * sqrt(g0[j][j]), 1e-30) (thr_j = 0 without g0).  Writes M (n x nk,
+// exceeds thr_j = max(rel * sqrt(g0[j][j]), 1e-30) (thr_j = 0 without g0), and, when rg is given,
+// when the residual norm sqrt(rg[j][j]) of the root it corrects is at least skip (converged
+// roots are not expanded).  Writes M (n x nk,
 // leading dimension 64, zero elsewhere) with Q = T M orthonormal, and nk.  n comes from n_dev
 // when given.  One wave: lane 0 factors, lane b inverts column b of the triangular factor.
 __global__ __launch_bounds__(64) void k_cholqr_small(const double* __restrict__ B, const int* __restrict__ n_dev,
                                                      int n_host, const double* __restrict__ g0, double rel,
-                                                     double* __restrict__ M, int* __restrict__ nk_out) {
+                                                     double* __restrict__ M, int* __restrict__ nk_out,
+                                                     const double* __restrict__ rg = nullptr, double skip = 0.0) {
   __shared__ double R[64][65];
   __shared__ double Ri[64][65];
   __shared__ int kept[64];
@@ -378,7 +396,8 @@ __global__ __launch_bounds__(64) void k_cholqr_small(const double* __restrict__ 
         d -= R[i][j] * R[i][j];
       }
       const double t = g0 ? fmax(rel * sqrt(fmax(g0[j + j * 64], 0.0)), 1e-30) : 0.0;
-      if (d > t * t && d > 0.0) {
+      const bool converged = rg && rg[j + j * 64] < skip * skip;
+      if (d > t * t && d > 0.0 && !converged) {
         R[j][j] = sqrt(d);
         kept[nk++] = j;
       }
@@ -530,8 +549,23 @@ void Davidson::initial_guess(hipStream_t st) {
                     [&](int64_t a, int64_t b) { return hd[a] < hd[b] || (hd[a] == hd[b] && a < b); });
   DevBuf<int64_t> pos(ci_->ctx, ninit_);
   PQG_HIP(hipMemcpyAsync(pos.get(), idx.data(), ninit_ * 8, hipMemcpyHostToDevice, st));
-  k_unit<<<g1(N_, 256), 256, 0, st>>>(V_.get(), N_, N_, pos.get(), ninit_);
-  PQG_HIP(hipGetLastError());
+  if (p_.guess_perturb > 0.0) {
+    // per-entry size guess_perturb * sqrt(3 / N): the admixture has norm ~guess_perturb
+    k_unit_perturbed<<<g1(N_, 256), 256, 0, st>>>(V_.get(), N_, N_, pos.get(), ninit_,
+                                                  p_.guess_perturb * std::sqrt(3.0 / (double)N_));
+    PQG_HIP(hipGetLastError());
+    mask_invalid<double>(ci_, V_.get(), N_, ninit_, st);
+    for (int pass = 0; pass < 2; pass++) {  // CholQR2
+      ts_gemm_tn(st, V_.get(), N_, ninit_, V_.get(), N_, ninit_, small_.get(), 64);
+      k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nullptr, ninit_, nullptr, 0.0, mbuf_.get(), nk_dev_.get());
+      PQG_HIP(hipGetLastError());
+      ts_gemm_nn(st, V_.get(), N_, ninit_, mbuf_.get(), 64, ninit_, tmp_.get(), N_, 1.0, 0.0);
+      PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)N_ * ninit_ * 8, hipMemcpyDeviceToDevice, st));
+    }
+  } else {
+    k_unit<<<g1(N_, 256), 256, 0, st>>>(V_.get(), N_, N_, pos.get(), ninit_);
+    PQG_HIP(hipGetLastError());
+  }
   PQG_HIP(hipStreamSynchronize(st));
   m_ = ninit_;
   nW_ = 0;
@@ -562,7 +596,7 @@ void Davidson::run_sigma(hipStream_t st) {
 // orthonormalise the block by two Cholesky-QR passes (columns that vanish are dropped) into
 // V[:, m:m+nt], all on the stream; the kept count (and the second pass's count, which must agree)
 // lands in nk_dev_[0..1].  Nothing is read back here; finish_append() commits the columns.
-void Davidson::append_async(hipStream_t st, double* T, int nt, const double* g0) {
+void Davidson::append_async(hipStream_t st, double* T, int nt, const double* g0, const double* rg) {
   nt = std::min(nt, nmax_ - m_);
   pending_nt_ = nt;
   if (nt <= 0) return;
@@ -576,7 +610,7 @@ void Davidson::append_async(hipStream_t st, double* T, int nt, const double* g0)
   }
   // pass 1: drop vanishing columns, orthonormalise the rest straight into V[:, m:]
   ts_gemm_tn(st, T, N_, nt, T, N_, nt, small_.get(), 64);
-  k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nullptr, nt, g0, 1e-10, mbuf_.get(), nk_dev_.get());
+  k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nullptr, nt, g0, 1e-10, mbuf_.get(), nk_dev_.get(), rg, p_.tol);
   PQG_HIP(hipGetLastError());
   double* dst = V_.get() + (int64_t)m_ * N_;
   ts_gemm_nn(st, T, N_, nt, mbuf_.get(), 64, nt, dst, N_, 1.0, 0.0);
@@ -766,7 +800,7 @@ void Davidson::post_sigma(hipStream_t st) {
   const bool speculate = !(mixed_ && need_restart);
   if (speculate) {
     if (need_restart) restart_to_ritz(st, ninit_);
-    append_async(st, Tp(), nev_, g0);
+    append_async(st, Tp(), nev_, g0, 2 * nev_ <= 64 ? rgram_.get() : nullptr);
   }
   read_status(st, rgram_.get());
   theta_h_.assign(status_h_ + 64, status_h_ + 64 + nev_);

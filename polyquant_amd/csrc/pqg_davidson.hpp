@@ -49,7 +49,8 @@ class Davidson {
  private:
   void initial_guess(hipStream_t st);
   int orthonormalize_append(hipStream_t st, double* T, int nt);
-  void append_async(hipStream_t st, double* T, int nt, const double* g0);
+  // rg: residual Gram of the roots the columns correct; columns of converged roots are dropped
+  void append_async(hipStream_t st, double* T, int nt, const double* g0, const double* rg = nullptr);
   int finish_append();
   void read_status(hipStream_t st, const double* rgram);
   void restart_to_ritz(hipStream_t st, int keep);

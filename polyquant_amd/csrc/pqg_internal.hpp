@@ -74,7 +74,10 @@ class Arena {
     char* base;
     uint64_t size;
     std::map<uint64_t, uint64_t> free_;  // offset -> size
+    bool dedicated = false;              // one large block: returned to the device on free
+    int live = 0;                        // blocks handed out from this chunk
   };
+  bool release_empty_locked();  // hipFree every chunk without live blocks; true if any
   int device_;
   uint64_t budget_;
   uint64_t reserved_ = 0;  // bytes obtained from hipMalloc

@@ -59,7 +59,11 @@ struct Request {
   double t_submit;
   double t_admit = 0.0;
   uint64_t reserve = 0;
+  double work = 0.0;  // determinants x roots: the admission order's size measure
 };
+
+// Requests waiting longer than this are admitted oldest first; below it, smallest work first.
+constexpr double kMaxWaitS = 30.0;
 
 }  // namespace
 
@@ -81,32 +85,46 @@ struct pqg_server {
 
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 
-  // Reserve budget for the first queued request that fits and hand it to a free slot.
+  // Reserve budget for a queued request that fits and hand it to a free slot.  Order: requests
+  // that have waited longer than kMaxWaitS, oldest first; otherwise the smallest work first,
+  // so short solves are not held behind long ones (ties: arrival order).
   void admit_locked() {
     for (int s = 0; s < max_slots; s++) {
       if (assigned[s]) continue;
       const uint64_t budget = ctx->arena->budget();
-      auto it = queue.begin();
-      for (; it != queue.end(); ++it) {
-        if (reserved + (*it)->reserve <= budget) break;
+      const double t = now();
+      auto best = queue.end();
+      bool best_old = false;
+      for (auto it = queue.begin(); it != queue.end(); ++it) {
+        if (reserved + (*it)->reserve > budget) continue;
+        const bool old = t - (*it)->t_submit > kMaxWaitS;
+        if (best == queue.end() || (old && !best_old) ||
+            (old == best_old && !old && (*it)->work < (*best)->work)) {
+          best = it;
+          best_old = old;
+        }
       }
+      auto it = best;
       if (it == queue.end()) {
         // nothing fits; a request larger than the whole budget can never run
-        if (!queue.empty() && resident == 0 && queue.front()->reserve > budget) {
-          auto& r = queue.front();
-          pqg_result res{};
-          res.request_id = r->id;
-          res.status = PQG_ERR_OOM;
-          std::snprintf(res.message, sizeof(res.message), "request needs more than the whole HBM budget");
-          res.t_submit = r->t_submit;
-          res.t_admit = res.t_done = now();
-          done.push_back(res);
-          queue.pop_front();
-          continue;
+        if (!queue.empty() && resident == 0) {
+          auto big = std::find_if(queue.begin(), queue.end(), [&](const auto& r) { return r->reserve > budget; });
+          if (big != queue.end()) {
+            pqg_result res{};
+            res.request_id = (*big)->id;
+            res.status = PQG_ERR_OOM;
+            std::snprintf(res.message, sizeof(res.message), "request needs more than the whole HBM budget");
+            res.t_submit = (*big)->t_submit;
+            res.t_admit = res.t_done = t;
+            done.push_back(res);
+            queue.erase(big);
+            s--;  // try this slot again
+            continue;
+          }
         }
         return;
       }
-      (*it)->t_admit = now();
+      (*it)->t_admit = t;
       reserved += (*it)->reserve;
       resident++;
       assigned[s] = std::move(*it);
@@ -199,8 +217,16 @@ pqg_status pqg_serve_submit(pqg_server* srv, const pqg_ci_desc* desc, const pqg_
     auto r = std::make_unique<Request>();
     r->desc.copy_from(*desc);
     r->params = *params;
-    const int nmax = params->nvec_max > 0 ? params->nvec_max : std::max(2 * params->nvec_init, 10 * params->nev);
-    r->reserve = estimate_ci_bytes(desc, nmax, params->precision);
+    r->reserve = estimate_ci_bytes(desc, params);
+    double ndet = 1.0;
+    for (int p = 0; p < desc->nspecies; p++)
+      for (int sp = 0; sp < 2; sp++) {
+        const int n = desc->n_orb[p], k = sp == 0 ? desc->n_alpha[p] : desc->n_beta[p];
+        double c = 1.0;
+        for (int i = 1; i <= k; i++) c = c * (n - k + i) / i;
+        ndet *= c;
+      }
+    r->work = ndet * std::max(1, params->nev);
     std::lock_guard<std::mutex> lk(srv->mu);
     r->id = srv->next_id++;
     r->t_submit = srv->now();

@@ -22,10 +22,10 @@ class Server:
         self._buf = (Result * 256)()
 
     def submit(self, prob, nev: Optional[int] = None, nvec_init: int = 5, nvec_max: int = 0, maxit: int = 500,
-               tol: float = 1e-6, precision: int = _lib.PQG_FP64) -> int:
+               tol: float = 1e-6, precision: int = _lib.PQG_FP64, perturb: float = 0.0) -> int:
         keep: list = []
         d = make_desc(prob, keep)
-        p = DavParams(nev or prob.nroots, nvec_init, nvec_max, maxit, tol, precision)
+        p = DavParams(nev or prob.nroots, nvec_init, nvec_max, maxit, tol, precision, perturb)
         rid = ct.c_int64()
         check(load().pqg_serve_submit(self.h, ct.byref(d), ct.byref(p), ct.byref(rid)))
         return rid.value

@@ -249,12 +249,13 @@ def small_problem(seed: int, n: int, na: int, nb: int, nroots: int = 1, nspin: i
     return prob
 
 
-def serving_requests(n_requests: int = 512, rate: float = 200.0, seed: int = SEED_BASE + 5):
+def serving_requests(n_requests: int = 512, rate: float = 200.0, seed: int = SEED_BASE + 5, only=None):
     """Request mix of the serving workload: Poisson arrivals and uneven problem sizes.
 
     Returns a list of (arrival_time_s, CIProblem).  Per request: n ~ U{6..14},
     alpha, beta ~ U{2..n//2} (independent), positron in {0, 1} with n_p ~ U{6..12},
-    roots ~ U{1..4}; integrals seeded per request with the Cholesky recipe.
+    roots ~ U{1..4}; integrals seeded per request with the Cholesky recipe.  ``only``: a
+    collection of request numbers whose integrals are built (the others come back as None).
     """
     rng = SplitMix64(seed)
     t = 0.0
@@ -270,6 +271,9 @@ def serving_requests(n_requests: int = 512, rate: float = 200.0, seed: int = SEE
         if has_pos:
             npo = int(rng.integers(6, 12)[0])
             sp.append(Species(npo, 1, 0, 1.0, 1))
+        if only is not None and r not in only:
+            out.append((t, None))
+            continue
         prob = cholesky_problem(seed * 1000003 + r, sp, roots, name=f"req{r}")
         out.append((t, prob))
     return out

@@ -227,3 +227,22 @@ def test_link_gemm_path_equals_enumeration_path(name, ctx, monkeypatch):
     assert np.max(np.abs(a - b)) < 1e-12 * scale
     a32 = fast.sigma(C, precision=PQG_MIXED)
     assert np.max(np.abs(a32 - b)) < 2e-5 * scale
+
+
+def test_davidson_perturbed_start_reaches_ground_state(ctx):
+    """Serving request 281 (8 orbitals, 3a3b, 3136 determinants): the reference's unit-vector
+    start converges to the second eigenvalue of the dense Hamiltonian (the ground state has no
+    weight on the lowest-diagonal determinants); the perturbed start finds the lowest one."""
+    from oracle import refcpu
+    from polyquant_amd.ci import DetSet
+
+    prob = wl.serving_requests(282, only={281})[281][1]
+    print("[gpu] case davidson_perturbed_start req281", flush=True)
+    oc = refcpu.OracleCI(prob)
+    w = np.linalg.eigvalsh(oc.sigma(np.eye(oc.N)))
+    ds = DetSet(prob, ctx)
+    E0, _, r0 = ds.davidson(1, nvec_init=8, tol=1e-7, want_vectors=False)
+    E1, _, r1 = ds.davidson(1, nvec_init=8, tol=1e-7, want_vectors=False, perturb=1e-2)
+    assert r0.converged == 1 and r1.converged == 1
+    assert abs(E0[0] - w[1]) < 1e-8  # the unit-vector start lands on the excited root
+    assert abs(E1[0] - w[0]) < 1e-8
