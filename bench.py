@@ -20,6 +20,7 @@ the full sigma and multiplied by the number of sigma calls of the GPU solve.
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
 import argparse
+import math
 import json
 import os
 import re
@@ -349,6 +350,78 @@ def bench_serve(args):
         print(json.dumps({"serve_leg": args.serve_leg, "requests": n, "seconds": round(total, 3)}), flush=True)
 
 
+def bench_sigma(args):
+    """Sigma throughput of the link-GEMM path: 16 orbitals 8a8b (165,636,900 determinants) or
+    water/6-31G (1,656,369), k = 4 vectors per application, FP64 and FP32, with the roofline of
+    the link-GEMM kernel.  Algorithmic flops of one k_lg_u launch = 2 * rows * L * K * columns
+    (M = L links per row string, K = off-diagonal column pairs + L, the GEMM as posed, without
+    padding); launch durations from HIP events on the CI stream."""
+    import numpy as np
+    import torch
+
+    from polyquant_amd import workloads as wl
+    from polyquant_amd._lib import PQG_FP64, PQG_MIXED
+    from polyquant_amd.ci import Context, DetSet
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    prob = wl.config3() if args.sigma_workload == "fci16" else wl.water_problem("6-31g")
+    ctx = Context(0)
+    ds = DetSet(prob, ctx)
+    N, k = ds.N_dets, 4
+    sp = prob.species[0]
+    n, na, nb = sp.n_orb, sp.n_alpha, sp.n_beta
+    n_a, n_b = math.comb(n, na), math.comb(n, nb)
+    La, Lb = na * (n - na), nb * (n - nb)
+    # GEMM flops of one application per vector: pass A (rows alpha, K = beta pairs + alpha links)
+    # and pass B (rows beta, K = beta links)
+    gemm_flops = 2.0 * n_a * La * (n * (n - 1) // 2 + La) * n_b + 2.0 * n_b * Lb * Lb * n_a
+    C = torch.randn(N * k, dtype=torch.float64, device="cuda") / math.sqrt(N)
+    S = torch.empty_like(C)
+    torch.cuda.synchronize()
+    names = ["sigma_pass_nc1", "lg_gemm", "lg_gather"]
+    res = {}
+    for tag, prec in (("fp64", PQG_FP64), ("fp32", PQG_MIXED)):
+        suf = "_f64" if prec == PQG_FP64 else "_f32"
+        for _ in range(args.warmup):
+            ds.sigma_device(C.data_ptr(), N, k, S.data_ptr(), N, prec)
+        ctx.profile(True)
+        for nm in names:
+            ctx.profile_read(nm + suf)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ds.sigma_device(C.data_ptr(), N, k, S.data_ptr(), N, prec)
+        dt = (time.perf_counter() - t0) / args.steps
+        ctx.profile(False)
+        prof = {nm: ctx.profile_read(nm + suf) for nm in names}
+        launches, gemm_ms, _ = prof["lg_gemm"]
+        gemm_s = gemm_ms * 1e-3 / args.steps
+        peak = 78.6 if prec == PQG_FP64 else 157.3
+        achieved = gemm_flops * k / gemm_s / 1e12
+        res[tag] = {"ms_per_sigma_4_vectors": round(dt * 1e3, 3),
+                    "det_vectors_per_s": round(N * k / dt, 1),
+                    "lg_gemm_ms": round(gemm_s * 1e3, 3), "lg_gemm_launches_per_sigma": launches // args.steps,
+                    "lg_gemm_tflops": round(achieved, 3), "lg_gemm_frac_of_peak": round(achieved / peak, 4),
+                    "peak_tflops": peak,
+                    "lg_gather_ms": round(prof["lg_gather"][1] / args.steps, 3),
+                    "enumeration_pass_ms": round(prof["sigma_pass_nc1"][1] / args.steps, 3)}
+        log(f"[sigma] {tag}: {res[tag]}")
+    fp64 = res["fp64"]
+    out = {
+        "metric": "sigma (H C) throughput, link-GEMM path", "value": fp64["det_vectors_per_s"],
+        "unit": "determinant-vectors/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic" if args.sigma_workload == "fci16" else "water 6-31G",
+        "config": {"workload": (f"{args.sigma_workload}: {n} orbitals, {na}a{nb}b, {N} determinants, "
+                                f"k = {k} vectors per sigma"), "gemm_flops_per_vector": gemm_flops},
+        "fp64": fp64, "fp32": res["fp32"],
+        "fp32_speedup": round(fp64["ms_per_sigma_4_vectors"] / res["fp32"]["ms_per_sigma_4_vectors"], 3),
+        "roofline": {"bound": "mfma", "achieved": fp64["lg_gemm_tflops"], "peak": 78.6, "unit": "TFLOP/s",
+                     "frac": fp64["lg_gemm_frac_of_peak"], "traffic": None, "kernel": "k_lg_u<double>"},
+    }
+    print(json.dumps(out), flush=True)
+    ds.close()
+    ctx.close()
+
+
 def bench_h2o(args):
     """Water/6-31G full CI (13 orbitals, 5a5b, 1,656,369 determinants), RHF orbitals from the CPU."""
     import numpy as np
@@ -414,9 +487,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--workload", default="davidson", choices=["davidson", "ao2mo", "fci16", "serve", "h2o"],
+    ap.add_argument("--workload", default="davidson", choices=["davidson", "ao2mo", "fci16", "serve", "h2o", "sigma"],
                     help="davidson (default): ep-FCI 48/48/48 Davidson solves; ao2mo: n = 320 transform")
     ap.add_argument("--h2o-nvec", type=int, default=5)
+    ap.add_argument("--sigma-workload", default="fci16", choices=["fci16", "h2o"])
     ap.add_argument("--serve-n", type=int, default=512)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-leg", default="both", choices=["both", "sequential", "serving"])
@@ -428,9 +502,9 @@ def main():
     ap.add_argument("--serve-save", default=None, help="write the leg's per-request results (JSON)")
     ap.add_argument("--serve-combine", nargs=2, default=None, metavar=("SEQ", "SRV"),
                     help="print the serving report from saved legs (SEQ: comma-separated sequential parts)")
-    ap.add_argument("--fci-nvec", type=int, default=8)
-    ap.add_argument("--fci-nmax", type=int, default=32)
-    ap.add_argument("--fci-maxit", type=int, default=200)
+    ap.add_argument("--fci-nvec", type=int, default=16)
+    ap.add_argument("--fci-nmax", type=int, default=48)
+    ap.add_argument("--fci-maxit", type=int, default=400)
     ap.add_argument("--ao-n", type=int, default=320)
     ap.add_argument("--ao-reps", type=int, default=2)
     ap.add_argument("--ao-budget-gb", type=float, default=120.0)
@@ -460,6 +534,9 @@ def main():
         return
     if args.workload == "h2o":
         bench_h2o(args)
+        return
+    if args.workload == "sigma":
+        bench_sigma(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the CI path is built and "
