@@ -192,7 +192,7 @@ def bench_fci16(args):
     for name, prec in (("fp64", PQG_FP64), ("mixed", PQG_MIXED)):
         t0 = time.perf_counter()
         E, _, rep = ds.davidson(nev, nvec_init=args.fci_nvec, nvec_max=args.fci_nmax, tol=1e-6, precision=prec,
-                                maxit=args.fci_maxit, want_vectors=False)
+                                maxit=args.fci_maxit, want_vectors=False, perturb=args.fci_perturb)
         dt = time.perf_counter() - t0
         res[name] = {"seconds": round(dt, 3), "iterations": rep.iterations, "sigma_vectors": rep.sigma_calls,
                      "sigma_vectors_fp64": rep.sigma_calls_fp64, "energies": [float(x) for x in E],
@@ -205,7 +205,8 @@ def bench_fci16(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "fci_16o_8a8b: electronic FCI, 16 orbitals, 8 alpha + 8 beta electrons, "
                                f"{N} determinants, {nev} roots, residual <= 1e-6",
-                   "num_subspace_vec": args.fci_nvec, "max_subspace": args.fci_nmax},
+                   "num_subspace_vec": args.fci_nvec, "max_subspace": args.fci_nmax,
+                   "start_perturbation": args.fci_perturb},
         "fp64": res["fp64"], "mixed": res["mixed"],
         "mixed_speedup": round(res["fp64"]["seconds"] / res["mixed"]["seconds"], 3),
         "max_abs_dE_mixed_vs_fp64": dE,
@@ -505,6 +506,8 @@ def main():
     ap.add_argument("--fci-nvec", type=int, default=16)
     ap.add_argument("--fci-nmax", type=int, default=48)
     ap.add_argument("--fci-maxit", type=int, default=400)
+    ap.add_argument("--fci-perturb", type=float, default=1e-2,
+                    help="start-vector perturbation of both solves; 0 = unit vectors")
     ap.add_argument("--ao-n", type=int, default=320)
     ap.add_argument("--ao-reps", type=int, default=2)
     ap.add_argument("--ao-budget-gb", type=float, default=120.0)

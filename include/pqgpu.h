@@ -71,7 +71,8 @@ typedef struct {
    * > 0 (no counterpart in the reference): add a deterministic pseudo-random admixture of
    * norm ~guess_perturb to each start vector, so the start overlaps every eigenvector (a
    * unit-vector start can converge to an excited root when the ground state has no weight
-   * on the lowest-diagonal determinants).                                                  */
+   * on the lowest-diagonal determinants).  It also stops expanding roots whose residual is
+   * already below tol.                                                                     */
   double guess_perturb;
 } pqg_dav_params;
 

@@ -1312,20 +1312,17 @@ static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k,
       const int kg = std::min(kGroup, k - v0);
       const T* Cg = C + (size_t)v0 * ld;
       T* Sg = S + (size_t)v0 * ld;
-      // diagonal and singles of every column set (with their spectator sums)
-      launch_pass<T>(ci, tb, rowsA, colsA, Cg, ld, Sg, ld, kg, F_DIAG | F_COLS | F_NODBL | F_NOPAIR, st);
+      // beta singles and doubles on the transpose
       transpose<T>(Cg, ld, CT, N, n0, M, kg, false, st);
-      // alpha singles (with their spectator sums), then beta doubles, on the transpose
-      launch_pass<T>(ci, tb, rowsB, {0}, CT, N, ST, N, kg, F_COLS | F_NODBL, st);
-      for (int v = 0; v < kg; v++) link_gemm_pass(ci, 1, CT + (size_t)v * N, ST + (size_t)v * N, st);
-      transpose<T>(ST, N, Sg, ld, M, n0, kg, true, st);
-      // couplings of beta singles with alpha / positron singles, and everything among alpha and
-      // positron strings; with positrons on C permuted to (alpha, positron strings, beta)
+      for (int v = 0; v < kg; v++) link_gemm_pass(ci, 1, CT + (size_t)v * N, ST + (size_t)v * N, false, st);
+      transpose<T>(ST, N, Sg, ld, M, n0, kg, false, st);
+      // everything else but the diagonal; with positrons on C permuted to (alpha, positron
+      // strings, beta)
       for (int v = 0; v < kg; v++) {
         const T* Cv = Cg + (size_t)v * ld;
         T* Sv = Sg + (size_t)v * ld;
         if (nP == 1) {
-          link_gemm_pass(ci, 0, Cv, Sv, st);
+          link_gemm_pass(ci, 0, Cv, Sv, true, st);
           continue;
         }
         T* Cp = CP + (size_t)v * N;
@@ -1334,13 +1331,13 @@ static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k,
           const int64_t z = std::min<int64_t>(65535, n0 - z0);
           transpose<T>(Cv + z0 * M, M, Cp + z0 * M, M, n1, nP, (int)z, false, st);
         }
-        PQG_HIP(hipMemsetAsync(Sp, 0, (size_t)N * sizeof(T), st));
-        link_gemm_pass(ci, 0, Cp, Sp, st);
+        link_gemm_pass(ci, 0, Cp, Sp, false, st);
         for (int64_t z0 = 0; z0 < n0; z0 += 65535) {
           const int64_t z = std::min<int64_t>(65535, n0 - z0);
           transpose<T>(Sp + z0 * M, M, Sv + z0 * M, M, nP, n1, (int)z, true, st);
         }
       }
+      link_gemm_diag(ci, Cg, ld, Sg, ld, kg, st);
     }
     return;
   }
@@ -1407,7 +1404,14 @@ uint64_t estimate_ci_bytes(const pqg_ci_desc* d, const pqg_dav_params* p) {
   // vectors, the enumeration kernel's transposes of the whole block, or the pair-GEMM partials
   uint64_t sig = std::max<uint64_t>((positrons ? 4 : 2) * N * std::min<uint64_t>(kb, 4) * 8, 2 * N * kb * 8);
   if (one_particle_sets >= 2) sig = std::max<uint64_t>(sig, 16ull * 6 * N * kb * 8);
-  const uint64_t other = N * 8 /* diagonal */ + lg_budget_bytes((int64_t)N, 8);
+  // link-GEMM chunk: U holds rows x links x chunk columns, at most the budget's worth
+  uint64_t lmax = 0;
+  for (int sp = 0; sp < d->nspecies; sp++)
+    for (int s = 0; s < 2; s++) {
+      const int k = s == 0 ? d->n_alpha[sp] : d->n_beta[sp];
+      lmax += (uint64_t)k * (d->n_orb[sp] - k);
+    }
+  const uint64_t other = N * 8 /* diagonal */ + std::min<uint64_t>(lg_budget_bytes((int64_t)N, 8), N * lmax * 8);
   return tables + dav + sig + other + (64ull << 20);
 }
 

@@ -50,8 +50,9 @@ struct Tables {
 // One-species full CI: link tables for the sigma terms computed as GEMMs on MFMA.
 struct LinkTab {
   int L = 0, nv = 0, nb = 0;  // links per string, virtuals, off-diagonal orbital pairs
+  int nin = 0;                // rows of the column-set map (electron sets)
   DevBuf<int2> links;         // [nstr][L]: {J | odd << 31, Q << 16 | rev}
-  DevBuf<int32_t> tbm;        // [nb][nstr]: (excited string << 1 | odd) or -1 (column set only)
+  DevBuf<int32_t> tbm;        // [nin][nstr]: (column index << 1 | odd) or -1 (electron sets)
 };
 struct LinkGemm {
   bool enabled = false;
@@ -132,11 +133,13 @@ void tensor_sigma(pqg_ci* ci, const float* C, float* S, int k, int64_t ld, hipSt
 bool link_gemm_eligible(const pqg_ci* ci);
 void link_gemm_setup(pqg_ci* ci);
 size_t lg_budget_bytes(int64_t N, size_t w);
-// pass 0: couplings of e-beta singles with every other set's singles, and the doubles and
-// single-single couplings among e-alpha and positron sets (C with positron strings permuted
-// in front of e-beta); pass 1: e-beta doubles on the transposed vector.  Accumulates into S
-// (one vector).
-void link_gemm_pass(pqg_ci* ci, int r, const double* C, double* S, hipStream_t st);
-void link_gemm_pass(pqg_ci* ci, int r, const float* C, float* S, hipStream_t st);
+// pass 0: everything but the diagonal and the e-beta singles/doubles (C with positron strings
+// permuted in front of e-beta); pass 1: e-beta singles and doubles on the transposed vector.
+// accumulate: add to S (one vector) or overwrite it.
+void link_gemm_pass(pqg_ci* ci, int pass, const double* C, double* S, bool accumulate, hipStream_t st);
+void link_gemm_pass(pqg_ci* ci, int pass, const float* C, float* S, bool accumulate, hipStream_t st);
+// S += diag(H) C for k vectors
+void link_gemm_diag(pqg_ci* ci, const double* C, int64_t ldc, double* S, int64_t lds, int k, hipStream_t st);
+void link_gemm_diag(pqg_ci* ci, const float* C, int64_t ldc, float* S, int64_t lds, int k, hipStream_t st);
 
 }  // namespace pqg

@@ -2,9 +2,10 @@
 //
 // Restates the behaviour of the eigensolver the reference drives at src/ci/epci.cpp:589-621
 // (Spectra DavidsonSymEigsSolver, SortRule::SmallestAlge): start from unit vectors at the
-// nvec_init smallest diagonal entries, extend by t_k = r_k / (theta_k - diag) for the roots whose
-// residual is not yet below tol, restart on the current Ritz vectors when the basis would exceed
-// nvec_max, converge when every one of the nev residual norms is below tol.
+// nvec_init smallest diagonal entries, extend by t_k = r_k / (theta_k - diag) (with a perturbed
+// start only for the roots whose residual is not yet below tol), restart on the current Ritz
+// vectors when the basis would exceed nvec_max, converge when every one of the nev residual
+// norms is below tol.
 //
 // Everything stays in HBM: tall-skinny products (V^T W, V Y, Gram-Schmidt) are FP64 kernels
 // with a deterministic two-level reduction, and the projected eigenproblem is solved by a
@@ -800,7 +801,9 @@ void Davidson::post_sigma(hipStream_t st) {
   const bool speculate = !(mixed_ && need_restart);
   if (speculate) {
     if (need_restart) restart_to_ritz(st, ninit_);
-    append_async(st, Tp(), nev_, g0, 2 * nev_ <= 64 ? rgram_.get() : nullptr);
+    // with the perturbed start every eigenvector has weight in the basis, so converged roots
+    // need not be expanded; with unit vectors their corrections can still bring in missing roots
+    append_async(st, Tp(), nev_, g0, (p_.guess_perturb > 0.0 && 2 * nev_ <= 64) ? rgram_.get() : nullptr);
   }
   read_status(st, rgram_.get());
   theta_h_.assign(status_h_ + 64, status_h_ + 64 + nev_);

@@ -1,26 +1,31 @@
-// Sigma terms of one-species full CI on the matrix cores ("link GEMM").
+// Full-CI sigma on the matrix cores ("link GEMM").
 //
-// C is viewed as an n_r x n_c matrix whose rows are indexed by the strings of set r and
-// whose columns by those of set c (r = alpha on the natural layout, r = beta on the
-// transposed vector).  A link of a string S is one single excitation S -> S - h + p,
-// numbered e = a * nv + b with h the a-th occupied and p the b-th virtual orbital of S
-// (ascending); Q(S, e) = idx2(h, p), s(S, e) = the excitation's phase, J(S, e) the excited
-// string and rev(S, e) the number of the link of J(S, e) that leads back to S.
+// C is viewed as an n_r x n_c matrix: its rows are indexed by tuples of "row" strings, its
+// columns by the strings of one "column" set.  Pass A works on the natural layout (rows =
+// electron alpha x positron strings, columns = electron beta; with positrons on C permuted per
+// alpha row to that order), pass B on the transposed vector (rows = electron beta x positron
+// strings, columns = electron alpha).  A link of a string S is one single excitation
+// S -> S - h + p, numbered e = a * nv + b with h the a-th occupied and p the b-th virtual
+// orbital of S (ascending); Q(S, e) = idx2(h, p), s(S, e) its phase, J(S, e) the excited string
+// and rev(S, e) the number of the link of J(S, e) that leads back to S.
 //
-// Two pieces of H are formed here (the diagonal and the singles with their spectator sums
-// stay in the enumeration kernel of pqg_ci.hip):
-//   r x c single-single couplings  sum_{Qr,Qc} (Qr|Qc) E^r_Qr E^c_Qc C     (r = alpha only)
-//   same-set doubles of r          1/2 sum_{Q,Q'} (Q|Q') E^r_Q E^r_Q' C over the paths
-//                                  I -> J -> K whose two links share neither hole nor particle
-//                                  (exactly the double excitations I <-> K; the four paths of
-//                                  each double add up to phase * ((h0p0|h1p1) - (h0p1|h1p0)),
-//                                  slater_condon_singleparticle.cpp:107-175)
-// Both are   sigma(I, m) += sum_e s(I,e) U[J(I,e)][rev(I,e)][m]   with, per row string J,
-//   U[J][e'][m] = sum_k A_J[e'][k] T_J[k][m]         (one GEMM per row string, on MFMA)
-//   k < nb:  T_J[k][m] = s * C(J, m ^ pair k)  when exactly one orbital of the off-diagonal
-//            pair k of set c is occupied in m (0 otherwise);  A_J[e'][k] = (Q(J,e')|pair k)
-//   k >= nb: T_J[nb+e][m] = C(J(J,e), m);  A_J[e'][nb+e] = 1/2 s(J,e) (Q(J,e')|Q(J,e)) when the
-//            links e', e of J have different holes and different particles (0 otherwise).
+// Everything but the diagonal is
+//   sigma(I, m) += sum_e s(I,e) U[J(I,e)][rev(I,e)][m],   U[J][e'][m] = sum_k A_J[e'][k] T_J[k][m]
+// with one GEMM per row tuple J (M = its links e', on MFMA) and K made of
+//   in-row rows (gathered from the staged row C(J, :) through a map of the column set):
+//     off-diagonal pairs kb of the column set (pass A):  T = s * C(J, m ^ pair kb) when exactly
+//       one orbital of kb is occupied in m;  A = (Q(J,e')|kb)          -> single x single couplings
+//     diagonal pairs jj of the column set:  T = n_j(m) C(J, m);  A = (Q(J,e')|jj)
+//                                                      -> spectator sums of the column strings
+//     the own row:  T = C(J, m);  A = single-excitation element of link e' without the column
+//       spectators (one-string part + spectator sums of the other strings of the row)
+//   row rows (the excited rows C(J(J,e), :) of the links e of J):
+//     same set:  A = 1/2 s(J,e) (Q(J,e')|Q(J,e)) when the links share neither hole nor particle
+//       -> same-set doubles (the four paths of each double add up to
+//          phase * ((h0p0|h1p1) - (h0p1|h1p0)), slater_condon_singleparticle.cpp:107-175)
+//     different row sets, e' in the lower one:  A = s(J,e) (Q(J,e')|Q(J,e))  -> couplings
+// Each single excitation is symmetric in its two strings, so the element seen from J's link
+// rev equals the reference's element of I -> J (slater_condon_singleparticle.cpp:53-105).
 // The GEMM writes U for a chunk of columns; a second kernel gathers it back (pull form, fixed
 // summation order, so results are bit-reproducible).
 #include <algorithm>
@@ -33,30 +38,36 @@
 namespace pqg {
 
 constexpr int LG_NT = 32;  // columns per tile
-constexpr int LG_MAXL = 128;
-// prefetch registers per thread: pair-map entries (nb * LG_NT / 256) and excited-row values
-// (L * LG_NT / 256); the kernel needs nb <= 128 and L <= 64
-constexpr int LG_TBMAX = 16, LG_TAMAX = 8;
-
+constexpr int LG_MAXL = 64;     // links per row tuple (four 16-row M blocks)
+constexpr int LG_MAXIN = 160;   // in-row rows
+constexpr int LG_MAXBO = 128;   // off-diagonal pairs of the column set (those go through the map)
 constexpr int LG_MAXSETS = 3;
 
-// Row strings are tuples over up to three string sets (e.g. electron alpha x positron alpha x
-// positron beta); the row index is sum_s i_s * stride_s.  Links of a row are the links of its
-// strings in set order, numbered from loff[s].
+// Row tuples range over up to three string sets (electron alpha or beta, positron alpha,
+// positron beta): row index = sum_u i_u * rstride[u].  The "active" sets (act[] = their
+// position in the row tuple) contribute links, numbered from loff[s] in set order.
 template <typename T>
 struct LgArgs {
   const T* C;          // this vector, n_r x n_c row-major
   T* U;                // chunk scratch [n_r][L][cw]
-  int nsets;
-  const int2* links[LG_MAXSETS];  // per set [n_s][L_s]: {J | odd << 31, Q << 16 | rev}
-  int Ls[LG_MAXSETS], nvs[LG_MAXSETS], ns[LG_MAXSETS], loff[LG_MAXSETS], np[LG_MAXSETS];
-  int64_t strides[LG_MAXSETS];
-  const T* Vrc[LG_MAXSETS];              // (P|Q): P a pair of row set s, Q a pair of the column set
-  const T* Vrr[LG_MAXSETS][LG_MAXSETS];  // (P|Q): P of row set s, Q of row set t (charges folded in)
-  const int32_t* tbm;  // column-set pair map [nb][n_c]: (index << 1 | odd) or -1
+  int nrow;            // sets in the row tuple
+  int64_t rstride[LG_MAXSETS];
+  int rns[LG_MAXSETS];
+  int nsets;           // active sets
+  int act[LG_MAXSETS];
+  const int2* links[LG_MAXSETS];  // per active set [n_s][L_s]: {J | odd << 31, Q << 16 | rev}
+  int Ls[LG_MAXSETS], nvs[LG_MAXSETS], loff[LG_MAXSETS], np[LG_MAXSETS];
+  const T* Vrc[LG_MAXSETS];              // (P|Q): P a pair of active set s, Q a pair of the column set
+  const T* Vrr[LG_MAXSETS][LG_MAXSETS];  // (P|Q): P of active set s, Q of active set t (charges folded in)
+  const T* Aone[LG_MAXSETS];             // one-string part of singles of active set s, [L_s][n_s]
+  const T* Xrow[LG_MAXSETS][LG_MAXSETS]; // spectator sums of row set u for singles of active set s, [n_u][np_s]
+  const int32_t* map;  // column-set map [nin][n_c]: (column index << 1 | odd) or -1
+  const uint64_t* cbits;  // column strings (bit j = orbital j occupied)
+  int nbo, ncd, nin;   // in-row rows: off-diagonal pairs, diagonal pairs, + the own row
   int np_c;
-  int n_r, n_c, L, nb, K;
+  int n_r, n_c, L, K;
   int c0, cw, gcols;
+  int accumulate;      // gather: add to S (1) or overwrite it (0)
   int lab_off, kbf_off, crow_off, tile_off;  // LDS byte offsets (links at 0)
 };
 
@@ -70,17 +81,23 @@ __device__ __forceinline__ X pick33(const X (&v)[LG_MAXSETS][LG_MAXSETS], int s,
   return s == 0 ? pick3(v[0], t) : (s == 1 ? pick3(v[1], t) : pick3(v[2], t));
 }
 
+template <typename T>
+__device__ __forceinline__ int row_index(const LgArgs<T>& a, int64_t J, int u) {
+  return (int)((J / pick3(a.rstride, u)) % pick3(a.rns, u));
+}
+
 // Links of row J into LDS: lq[e] = {target row | odd << 31, Q << 16 | rev}, lab[e] =
 // set << 16 | hole number << 8 | particle number.
 template <typename T>
-__device__ void lg_row_links(const LgArgs<T>& a, int64_t J, int2* lq, int* lab, int tid) {
-  for (int e = tid; e < a.L; e += 256) {
+__device__ void lg_row_links(const LgArgs<T>& a, int64_t J, int2* lq, int* lab, int tid, int nth) {
+  for (int e = tid; e < a.L; e += nth) {
     int s = 0;
     if (a.nsets > 1 && e >= a.loff[1]) s = 1;
     if (a.nsets > 2 && e >= a.loff[2]) s = 2;
     const int el = e - pick3(a.loff, s);
-    const int64_t stride = pick3(a.strides, s);
-    const int64_t i = (J / stride) % pick3(a.ns, s);
+    const int u = pick3(a.act, s);
+    const int64_t stride = pick3(a.rstride, u);
+    const int64_t i = row_index(a, J, u);
     const int2 l = pick3(a.links, s)[(size_t)i * pick3(a.Ls, s) + el];
     const int64_t tgt = J + ((int64_t)(l.x & 0x7fffffff) - i) * stride;
     lq[e] = make_int2((int)tgt | (l.x & (int)0x80000000), (l.y & ~0xffff) | (pick3(a.loff, s) + (l.y & 0xffff)));
@@ -98,53 +115,71 @@ __device__ __forceinline__ int offpair_hi(int k) {
   return p;
 }
 
-// Block = 4 waves.  With NSPLIT = 1 wave w owns M blocks w, w + 4, ... (MPW of them) and
-// every column of the tile; with NSPLIT = 2 waves (2j, 2j+1) share M block j and split the
-// columns.  The A fragments of a row string stay in registers for all of its tiles.
-template <typename T, int MPW, int NSPLIT, int KQ>
-__global__ __launch_bounds__(256) void k_lg_u(const LgArgs<T> a) {
+// Block = 4 x KSPLIT waves.  Wave w owns "slot" w & 3 (an M block, or with NSPLIT = 2 half the
+// columns of one); with KSPLIT = 2 waves w and w + 4 split the K steps in two halves whose
+// partial sums meet in LDS.  The A fragments of a row tuple stay in registers for all of its
+// tiles; the global loads of the next tile are issued before the MFMAs of the current one.
+template <typename T, int NSPLIT, int KSPLIT, int KQH>
+__global__ __launch_bounds__(256 * KSPLIT, 3 - KSPLIT) void k_lg_u(const LgArgs<T> a) {
+  constexpr int LG_THREADS = 256 * KSPLIT;
+  // prefetch registers per thread: in-row map entries and excited-row values of one tile
+  constexpr int LG_TBMAX = LG_MAXBO * LG_NT / LG_THREADS, LG_TAMAX = LG_MAXL * LG_NT / LG_THREADS;
   constexpr int NT = LG_NT, NF = NT / 16, NFW = NF / NSPLIT;
-  constexpr int NTP = NT + 16 / (int)sizeof(T);
+  constexpr int NTP = NT + 1;
+  constexpr int KP = 4 * KSPLIT * KQH;  // rows of the LDS tile; rows K..KP-1 stay zero
   typedef typename Mfma<T>::acc_t acc_t;
   extern __shared__ __align__(16) char smem[];
   int2* lq = reinterpret_cast<int2*>(smem);
-  T* crow = reinterpret_cast<T*>(smem + a.crow_off);
-  T* Ts = reinterpret_cast<T*>(smem + a.tile_off);
-  const int J = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int L = a.L, K = a.K, nb = a.nb;
-  constexpr int KP = 4 * KQ;  // rows of the LDS tile; rows K..KP-1 stay zero
-  // LDS: links of J, set/hole/particle numbers of each link, full pair index of each
-  // off-diagonal pair of the column set, the staged row C(J, :), the B tile
   int* lab = reinterpret_cast<int*>(smem + a.lab_off);
   int* kbf = reinterpret_cast<int*>(smem + a.kbf_off);
-  lg_row_links(a, J, lq, lab, tid);
-  for (int k = tid; k < nb; k += 256) kbf[k] = k + offpair_hi(k);
-  if (nb > 0)
-    for (int m = tid; m < a.n_c; m += 256) crow[m] = a.C[(size_t)J * a.n_c + m];
-  for (int i = tid; i < (KP - K) * NTP; i += 256) Ts[K * NTP + i] = T(0);
+  T* crow = reinterpret_cast<T*>(smem + a.crow_off);
+  T* Ts = reinterpret_cast<T*>(smem + a.tile_off);
+  T* Xs = Ts;  // partial sums of the second K half, after the MFMAs of a tile
+  const int64_t J = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int kh = w >> 2, slot = w & 3;
+  const int L = a.L, K = a.K, nin = a.nin;
+  lg_row_links(a, J, lq, lab, tid, LG_THREADS);
+  for (int k = tid; k < nin - 1; k += LG_THREADS) {
+    const int j = k - a.nbo;
+    kbf[k] = k < a.nbo ? k + offpair_hi(k) : j * (j + 1) / 2 + j;
+  }
+  for (int m = tid; m < a.n_c; m += LG_THREADS) crow[m] = a.C[(size_t)J * a.n_c + m];
+  for (int i = tid; i < (KP - K) * NTP; i += LG_THREADS) Ts[K * NTP + i] = T(0);
   __syncthreads();
 
-  const int mb0 = w / NSPLIT;
-  const int fb = (w % NSPLIT) * NFW;
+  const int mb0 = slot / NSPLIT;
+  const int fb = (slot % NSPLIT) * NFW;
   const bool active = 16 * mb0 < L;
-  T af[MPW][KQ];
-#pragma unroll
-  for (int i = 0; i < MPW; i++) {
-    const int ep = 16 * (mb0 + i * (4 / NSPLIT)) + (lane & 15);
+  T af[KQH];
+  {
+    const int ep = 16 * mb0 + (lane & 15);
     const bool rowok = ep < L;
     const int Qp = rowok ? (lq[ep].y >> 16) : 0;
     const int abp = rowok ? lab[ep] : 0;
     const int sp = abp >> 16;
+    // own row: one-string part of the single plus the other row strings' spectator sums
+    T own = T(0);
+    if (rowok) {
+      const int el = ep - pick3(a.loff, sp);
+      const int us = pick3(a.act, sp);
+      own = pick3(a.Aone, sp)[(size_t)el * pick3(a.rns, us) + row_index(a, J, us)];
+      for (int u = 0; u < a.nrow; u++) {
+        const T* X = pick33(a.Xrow, sp, u);
+        if (X) own += X[(size_t)row_index(a, J, u) * pick3(a.np, sp) + Qp];
+      }
+    }
 #pragma unroll
-    for (int q = 0; q < KQ; q++) {
-      const int k = 4 * q + (lane >> 4);
+    for (int q = 0; q < KQH; q++) {
+      const int k = 4 * (kh * KQH + q) + (lane >> 4);
       T v = T(0);
       if (rowok && k < K) {
-        if (k < nb) {
+        if (k < nin - 1) {
           v = pick3(a.Vrc, sp)[(size_t)Qp * a.np_c + kbf[k]];
+        } else if (k == nin - 1) {
+          v = own;
         } else {
-          const int e = k - nb;
+          const int e = k - nin;
           const int abe = lab[e];
           const int se = abe >> 16;
           const int2 l = lq[e];
@@ -159,17 +194,19 @@ __global__ __launch_bounds__(256) void k_lg_u(const LgArgs<T> a) {
           }
         }
       }
-      af[i][q] = v;
+      af[q] = v;
     }
   }
 
   const int64_t gbeg = (int64_t)a.c0 + (int64_t)blockIdx.y * a.gcols;
   const int64_t gend = std::min<int64_t>((int64_t)a.c0 + a.cw, gbeg + a.gcols);
-  // The global loads of the next tile (pair-map entries and excited rows) are issued into
-  // registers before the MFMAs of the current one; thread t holds entries t + 256 i.
-  const int nbt = nb * NT, nat = L * NT;
+  // Thread t holds entries t + LG_THREADS i of the off-diagonal pair rows and of the excited
+  // rows, all in column t % NT; the diagonal-pair and own rows come from its column string.
+  const int nbt = a.nbo * NT, nat = L * NT;
+  const int ncol = tid % NT;
   int tbv[LG_TBMAX];
   T tav[LG_TAMAX];
+  uint64_t cb = 0;
   // Branch-free: every slot loads from a clamped in-bounds address; which slots hold real
   // entries (and which columns lie inside the chunk) is decided when the tile is stored.
   const int64_t clast = (int64_t)a.n_c - 1;
@@ -177,23 +214,24 @@ __global__ __launch_bounds__(256) void k_lg_u(const LgArgs<T> a) {
     if (nbt > 0) {
 #pragma unroll
       for (int i = 0; i < LG_TBMAX; i++) {
-        const int idx = min(tid + 256 * i, nbt - 1);
+        const int idx = min(tid + LG_THREADS * i, nbt - 1);
         const int64_t m = min(m0 + (idx % NT), clast);
-        tbv[i] = a.tbm[(size_t)(idx / NT) * a.n_c + m];
+        tbv[i] = a.map[(size_t)(idx / NT) * a.n_c + m];
       }
     }
 #pragma unroll
     for (int i = 0; i < LG_TAMAX; i++) {
-      const int idx = min(tid + 256 * i, nat - 1);
+      const int idx = min(tid + LG_THREADS * i, nat - 1);
       const int64_t m = min(m0 + (idx % NT), clast);
       tav[i] = a.C[(size_t)(lq[idx / NT].x & 0x7fffffff) * a.n_c + m];
     }
+    cb = a.cbits[min(m0 + ncol, clast)];
   };
   if (gbeg < gend) prefetch(gbeg);
   for (int64_t m0 = gbeg; m0 < gend; m0 += NT) {
 #pragma unroll
     for (int i = 0; i < LG_TBMAX; i++) {
-      const int idx = tid + 256 * i;
+      const int idx = tid + LG_THREADS * i;
       if (idx < nbt) {
         const int x = (m0 + idx % NT < gend) ? tbv[i] : -1;
         T v = T(0);
@@ -204,43 +242,57 @@ __global__ __launch_bounds__(256) void k_lg_u(const LgArgs<T> a) {
         Ts[(idx / NT) * NTP + idx % NT] = v;
       }
     }
+    {
+      // diagonal pairs jj: n_j(m) C(J, m); the own row: C(J, m)
+      const bool in = m0 + ncol < gend;
+      const T cv = in ? crow[m0 + ncol] : T(0);
+      for (int r = a.nbo + tid / NT; r < nin; r += LG_THREADS / NT) {
+        const int j = r - a.nbo;
+        Ts[r * NTP + ncol] = (r == nin - 1 || ((cb >> j) & 1)) ? cv : T(0);
+      }
+    }
 #pragma unroll
     for (int i = 0; i < LG_TAMAX; i++) {
-      const int idx = tid + 256 * i;
-      if (idx < nat) Ts[(nb + idx / NT) * NTP + idx % NT] = (m0 + idx % NT < gend) ? tav[i] : T(0);
+      const int idx = tid + LG_THREADS * i;
+      if (idx < nat) Ts[(nin + idx / NT) * NTP + idx % NT] = (m0 + idx % NT < gend) ? tav[i] : T(0);
     }
     __syncthreads();
     if (m0 + NT < gend) prefetch(m0 + NT);
+    acc_t acc[NFW];
+#pragma unroll
+    for (int f = 0; f < NFW; f++) acc[f] = acc_t{0, 0, 0, 0};
     if (active) {
-      acc_t acc[MPW][NFW];
-#pragma unroll
-      for (int i = 0; i < MPW; i++)
-#pragma unroll
-        for (int f = 0; f < NFW; f++) acc[i][f] = acc_t{0, 0, 0, 0};
       // every K step runs: the padding rows of the tile and of A are zero
 #pragma unroll
-      for (int q = 0; q < KQ; q++) {
-        const T* brow = Ts + (4 * q + (lane >> 4)) * NTP + 16 * fb + (lane & 15);
+      for (int q = 0; q < KQH; q++) {
+        const T* brow = Ts + (4 * (kh * KQH + q) + (lane >> 4)) * NTP + 16 * fb + (lane & 15);
+#pragma unroll
+        for (int f = 0; f < NFW; f++) acc[f] = Mfma<T>::op(af[q], brow[16 * f], acc[f]);
+      }
+    }
+    if constexpr (KSPLIT == 2) {
+      __syncthreads();  // the tile is no longer read: its space takes the partial sums
+      if (active && kh == 1) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+          for (int f = 0; f < NFW; f++)
+            Xs[(16 * mb0 + Mfma<T>::row(r, lane)) * NT + 16 * (fb + f) + (lane & 15)] = acc[f][r];
+      }
+      __syncthreads();
+    }
+    if (active && kh == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int ep = 16 * mb0 + Mfma<T>::row(r, lane);
+        if (ep >= L) continue;
+        T* urow = a.U + ((size_t)J * L + ep) * a.cw;
 #pragma unroll
         for (int f = 0; f < NFW; f++) {
-          const T b = brow[16 * f];
-#pragma unroll
-          for (int i = 0; i < MPW; i++) acc[i][f] = Mfma<T>::op(af[i][q], b, acc[i][f]);
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < MPW; i++) {
-        const int mb = mb0 + i * (4 / NSPLIT);
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int ep = 16 * mb + Mfma<T>::row(r, lane);
-          if (ep >= L) continue;
-          T* urow = a.U + ((size_t)J * L + ep) * a.cw;
-#pragma unroll
-          for (int f = 0; f < NFW; f++) {
-            const int64_t m = m0 + 16 * (fb + f) + (lane & 15);
-            if (m < gend) urow[m - a.c0] = acc[i][f][r];
-          }
+          const int col = 16 * (fb + f) + (lane & 15);
+          const int64_t m = m0 + col;
+          if (m < gend)
+            urow[m - a.c0] = KSPLIT == 2 ? acc[f][r] + Xs[(16 * mb0 + Mfma<T>::row(r, lane)) * NT + col] : acc[f][r];
         }
       }
     }
@@ -248,13 +300,13 @@ __global__ __launch_bounds__(256) void k_lg_u(const LgArgs<T> a) {
   }
 }
 
-// sigma(I, c0 + j) += sum_e s(I,e) U[J(I,e)][rev(I,e)][j] for the columns j of a chunk
+// sigma(I, c0 + j) (+)= sum_e s(I,e) U[J(I,e)][rev(I,e)][j] for the columns j of a chunk
 template <typename T, int CPT>
 __global__ __launch_bounds__(256) void k_lg_gather(const LgArgs<T> a, T* __restrict__ S) {
   __shared__ int2 lq[LG_MAXL];
   __shared__ int lab[LG_MAXL];
   const int64_t I = blockIdx.x;
-  lg_row_links(a, I, lq, lab, threadIdx.x);
+  lg_row_links(a, I, lq, lab, threadIdx.x, 256);
   __syncthreads();
   const int L = a.L, cw = a.cw;
   const T* __restrict__ U = a.U;
@@ -279,8 +331,18 @@ __global__ __launch_bounds__(256) void k_lg_gather(const LgArgs<T> a, T* __restr
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
     const int col = j0 + 256 * j;
-    if (col < cw) dst[col] += acc[j];
+    if (col < cw) dst[col] = a.accumulate ? dst[col] + acc[j] : acc[j];
   }
+}
+
+// S[v][i] += Hd[i] * C[v][i]  (the diagonal, diagonal_Hii of slater_condon.cpp:102-153)
+template <typename T>
+__global__ void k_lg_diag(const double* __restrict__ Hd, const T* __restrict__ C, int64_t ldc, T* __restrict__ S,
+                          int64_t lds, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = blockIdx.y;
+  S[v * lds + i] += (T)Hd[i] * C[v * ldc + i];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -305,25 +367,38 @@ static int64_t lg_rank(uint64_t s, const std::vector<std::vector<uint64_t>>& bn)
 }
 
 // U scratch per chunk: about eight vectors' worth (a chunk re-stages each row of C once),
-// at most 16 GiB
+// at least 1 GiB (one chunk for small spaces), at most 16 GiB
 size_t lg_budget_bytes(int64_t N, size_t w) {
-  return std::min<size_t>(16ull << 30, std::max<size_t>(64ull << 20, 8ull * (size_t)N * w));
+  return std::min<size_t>(16ull << 30, std::max<size_t>(1ull << 30, 8ull * (size_t)N * w));
 }
 
-// K steps are a compile-time bound (multiple of 8) so the MFMA loop has no branches
-static int lg_kq_bound(int kq) { return kq <= 48 ? (kq + 7) / 8 * 8 : 0; }
+// K steps per wave group: a compile-time bound so the MFMA loop has no branches.  With the
+// K range split over two wave groups: 4, 8, .., 24, 26 steps each; unsplit: 8, 16, .., 48, 52.
+static int lg_kqh_bound(int kq, int ksplit) {
+  const int kqh = (kq + ksplit - 1) / ksplit;
+  const int g = 4 * (3 - ksplit);  // step granularity
+  if (kqh <= 12 * g) return (kqh + g - 1) / g * g;
+  return kqh <= 13 * g ? 13 * g : 0;
+}
 
-// Row sets of the two passes.  Pass A (natural layout; with positrons, on C permuted to
-// (e-alpha, positron strings, e-beta)): rows = e-alpha x positron strings, with the links of
-// e-alpha and of every positron set that holds a particle; columns = e-beta.  Pass B
-// (transposed vector): rows = e-beta x positron strings with the e-beta links only; columns =
-// e-alpha.
+// Wave groups splitting K: two (eight waves sharing one row tuple), except for FP32 on small
+// spaces, where one group per workgroup measured faster (water/6-31G FP32 sigma: 5.2 ms vs
+// 5.8 ms; 16 orbitals 8a8b: 730 ms with two groups vs 841 ms with one).
+static int lg_ksplit(const pqg_ci* ci, size_t w) {
+  if (const char* e = std::getenv("PQG_LG_KSPLIT")) return std::atoi(e) == 2 ? 2 : 1;
+  return (w == 4 && ci->n_int < 20000000) ? 1 : 2;
+}
+
+// Pass A (natural layout; with positrons, on C permuted to (e-alpha, positron strings,
+// e-beta)): row tuple = (e-alpha, positron alpha, positron beta), active = e-alpha and every
+// positron set with links; columns = e-beta.  Pass B (transposed vector): row tuple =
+// (e-beta, positron alpha, positron beta), active = e-beta; columns = e-alpha.
 struct LgPass {
-  int nsets = 0;
-  int slot[LG_MAXSETS] = {0, 0, 0};
-  int64_t stride[LG_MAXSETS] = {1, 1, 1};
-  int col = 1;        // column slot
-  bool pairs = true;  // in-row block over the column set's off-diagonal pairs
+  int nrow = 0, rslot[LG_MAXSETS] = {0, 0, 0};
+  int64_t rstride[LG_MAXSETS] = {1, 1, 1};
+  int nsets = 0, act[LG_MAXSETS] = {0, 0, 0};
+  int col = 1;
+  int nbo = 0, ncd = 0, nin = 0;
   int64_t n_r = 0;
   int L = 0;
 };
@@ -335,31 +410,41 @@ static int64_t positron_strings(const pqg_ci* ci) {
 static LgPass lg_pass_desc(const pqg_ci* ci, int pass) {
   LgPass d;
   const int64_t nP = positron_strings(ci);
-  if (pass == 0) {
-    d.col = 1;
-    d.pairs = true;
-    d.n_r = (int64_t)ci->slot[0].nstr * nP;
-    d.slot[d.nsets] = 0;
-    d.stride[d.nsets++] = nP;
-    if (ci->nslot == 4) {
-      if (ci->lg.tab[2].L > 0) {
-        d.slot[d.nsets] = 2;
-        d.stride[d.nsets++] = ci->slot[3].nstr;
-      }
-      if (ci->lg.tab[3].L > 0) {
-        d.slot[d.nsets] = 3;
-        d.stride[d.nsets++] = 1;
-      }
-    }
-  } else {
-    d.col = 0;
-    d.pairs = false;
-    d.n_r = (int64_t)ci->slot[1].nstr * nP;
-    d.slot[d.nsets] = 1;
-    d.stride[d.nsets++] = nP;
+  const int lead = pass == 0 ? 0 : 1;
+  d.col = 1 - lead;
+  d.n_r = (int64_t)ci->slot[lead].nstr * nP;
+  d.rslot[d.nrow] = lead;
+  d.rstride[d.nrow++] = nP;
+  if (ci->nslot == 4) {
+    d.rslot[d.nrow] = 2;
+    d.rstride[d.nrow++] = ci->slot[3].nstr;
+    d.rslot[d.nrow] = 3;
+    d.rstride[d.nrow++] = 1;
   }
-  for (int q = 0; q < d.nsets; q++) d.L += ci->lg.tab[d.slot[q]].L;
+  for (int u = 0; u < d.nrow; u++) {
+    const int t = d.rslot[u];
+    const int L = ci->slot[t].nocc * ci->slot[t].nvirt;
+    if (L > 0 && (u == 0 || pass == 0)) {
+      d.act[d.nsets++] = u;
+      d.L += L;
+    }
+  }
+  const int nc = ci->slot[d.col].n_orb;
+  d.nbo = pass == 0 ? nc * (nc - 1) / 2 : 0;
+  d.ncd = nc;
+  d.nin = d.nbo + d.ncd + 1;
   return d;
+}
+
+static size_t lg_lds_bytes(const LgPass& d, int n_c, int kqh, int ksplit, size_t w) {
+  auto r16 = [](size_t x) { return (x + 15) / 16 * 16; };
+  const size_t ntp = LG_NT + 1;
+  size_t off = r16((size_t)d.L * 8);      // links
+  off = r16(off + (size_t)d.L * 4);       // hole/particle numbers
+  off = r16(off + (size_t)d.nin * 4);     // pair indices
+  off = r16(off + (size_t)n_c * w);       // staged row
+  // tile (also holds the 64 x LG_NT partial sums of the second K half)
+  return off + std::max<size_t>((size_t)4 * ksplit * kqh * ntp, (size_t)64 * LG_NT) * w;
 }
 
 bool link_gemm_eligible(const pqg_ci* ci) {
@@ -369,24 +454,15 @@ bool link_gemm_eligible(const pqg_ci* ci) {
     if (!ci->slot[t].complete) return false;
   for (int t = 0; t < 2; t++)
     if (ci->slot[t].nocc < 1 || ci->slot[t].nvirt < 1) return false;
-  const Slot& sb = ci->slot[1];
-  if (sb.n_orb * (sb.n_orb - 1) / 2 * LG_NT > 256 * LG_TBMAX) return false;
   for (int pass = 0; pass < 2; pass++) {
-    // link counts as the pass would see them
-    int L = 0;
-    for (int t = 0; t < ci->nslot; t++) {
-      const bool in = pass == 0 ? (t != 1) : (t == 1);
-      if (in) L += ci->slot[t].nocc * ci->slot[t].nvirt;
+    const LgPass d = lg_pass_desc(ci, pass);
+    if (d.L > LG_MAXL || d.nin > LG_MAXIN || d.nbo > LG_MAXBO) return false;
+    for (int ks = 1; ks <= 2; ks++) {
+      const int kqh = lg_kqh_bound((d.nin + d.L + 3) / 4, ks);
+      if (!kqh) return false;
+      if (lg_lds_bytes(d, ci->slot[d.col].nstr, kqh, ks, 8) > 160 * 1024) return false;
     }
-    if (L * LG_NT > 256 * LG_TAMAX) return false;
-    const Slot& sc = ci->slot[pass == 0 ? 1 : 0];
-    const int nb = pass == 0 ? sc.n_orb * (sc.n_orb - 1) / 2 : 0;
-    const int KQ = lg_kq_bound((nb + L + 3) / 4);
-    if (!KQ) return false;
-    const size_t lds = 2048 + (nb > 0 ? (size_t)sc.nstr * 8 : 0) + (size_t)4 * KQ * (LG_NT + 2) * 8;
-    if (lds > 160 * 1024 - 1024) return false;
-    const int64_t nr = (int64_t)ci->slot[pass == 0 ? 0 : 1].nstr * positron_strings(ci);
-    if (nr >= (1ll << 31) - 1) return false;
+    if (d.n_r >= (1ll << 31) - 1) return false;
   }
   return true;
 }
@@ -407,35 +483,41 @@ void link_gemm_setup(pqg_ci* ci) {
     lt.L = s.nocc * nv;
     lt.nv = nv;
     lt.nb = n * (n - 1) / 2;
-    if (lt.L == 0) continue;
-    std::vector<int2> links((size_t)s.nstr * lt.L);
-    for (int i = 0; i < s.nstr; i++) {
-      const uint64_t S = s.sorted[i];
-      int occ[64], vir[64], no = 0, nvv = 0;
-      for (int o = 0; o < n; o++) {
-        if ((S >> o) & 1) occ[no++] = o;
-        else vir[nvv++] = o;
-      }
-      for (int a = 0; a < no; a++)
-        for (int b = 0; b < nvv; b++) {
-          const int h = occ[a], p = vir[b];
-          const uint64_t Jb = S ^ ((1ull << h) | (1ull << p));
-          const int lo = std::min(h, p), hi = std::max(h, p);
-          const uint64_t btw = ((1ull << hi) - 1) & ~((2ull << lo) - 1);
-          const int odd = __builtin_popcountll(S & btw) & 1;
-          const int Q = hi * (hi + 1) / 2 + lo;
-          const int ar = __builtin_popcountll(Jb & ((1ull << p) - 1));
-          const int br = h - __builtin_popcountll(Jb & ((1ull << h) - 1));
-          const int64_t tgt = lg_rank(Jb, bn);
-          links[(size_t)i * lt.L + a * nv + b] =
-              make_int2((int)((uint32_t)tgt | ((uint32_t)odd << 31)), (Q << 16) | (ar * nv + br));
+    if (lt.L > 0) {
+      std::vector<int2> links((size_t)s.nstr * lt.L);
+      for (int i = 0; i < s.nstr; i++) {
+        const uint64_t S = s.sorted[i];
+        int occ[64], vir[64], no = 0, nvv = 0;
+        for (int o = 0; o < n; o++) {
+          if ((S >> o) & 1) occ[no++] = o;
+          else vir[nvv++] = o;
         }
+        for (int a = 0; a < no; a++)
+          for (int b = 0; b < nvv; b++) {
+            const int h = occ[a], p = vir[b];
+            const uint64_t Jb = S ^ ((1ull << h) | (1ull << p));
+            const int lo = std::min(h, p), hi = std::max(h, p);
+            const uint64_t btw = ((1ull << hi) - 1) & ~((2ull << lo) - 1);
+            const int odd = __builtin_popcountll(S & btw) & 1;
+            const int Q = hi * (hi + 1) / 2 + lo;
+            const int ar = __builtin_popcountll(Jb & ((1ull << p) - 1));
+            const int br = h - __builtin_popcountll(Jb & ((1ull << h) - 1));
+            const int64_t tgt = lg_rank(Jb, bn);
+            links[(size_t)i * lt.L + a * nv + b] =
+                make_int2((int)((uint32_t)tgt | ((uint32_t)odd << 31)), (Q << 16) | (ar * nv + br));
+          }
+      }
+      lt.links.reset(ci->ctx, links.size());
+      PQG_HIP(hipMemcpy(lt.links.get(), links.data(), links.size() * sizeof(int2), hipMemcpyHostToDevice));
     }
-    lt.links.reset(ci->ctx, links.size());
-    PQG_HIP(hipMemcpy(lt.links.get(), links.data(), links.size() * sizeof(int2), hipMemcpyHostToDevice));
-    if (t == 1) {  // only the electron beta set serves as the column set of the pair block
-      std::vector<int32_t> map((size_t)lt.nb * s.nstr, -1);
-      for (int p = 1; p < n; p++)
+    if (t < 2) {
+      // column-set map: e-beta (pass A) with its off-diagonal pairs, e-alpha (pass B) without;
+      // then the diagonal pairs (occupied orbitals) and the own row
+      const int nbo = t == 1 ? lt.nb : 0;
+      const int nin = nbo + n + 1;
+      lt.nin = nin;
+      std::vector<int32_t> map((size_t)nin * s.nstr, -1);
+      for (int p = 1; p < n && t == 1; p++)
         for (int q = 0; q < p; q++) {
           const int kb = p * (p - 1) / 2 + q;
           const uint64_t btw = ((1ull << p) - 1) & ~((2ull << q) - 1);
@@ -447,28 +529,47 @@ void link_gemm_setup(pqg_ci* ci) {
             map[(size_t)kb * s.nstr + m] = (int32_t)((lg_rank(Jb, bn) << 1) | odd);
           }
         }
+      for (int m = 0; m < s.nstr; m++) {
+        const uint64_t S = s.sorted[m];
+        for (int j = 0; j < n; j++)
+          if ((S >> j) & 1) map[(size_t)(nbo + j) * s.nstr + m] = m << 1;
+        map[(size_t)(nin - 1) * s.nstr + m] = m << 1;
+      }
       lt.tbm.reset(ci->ctx, map.size());
       PQG_HIP(hipMemcpy(lt.tbm.get(), map.data(), map.size() * 4, hipMemcpyHostToDevice));
     }
   }
 }
 
-template <typename T, int MPW, int NSPLIT, int KQ>
+template <typename T, int NSPLIT, int KSPLIT, int KQH>
 static void lg_go(const LgArgs<T>& a, dim3 grid, size_t lds, hipStream_t st) {
-  auto kern = k_lg_u<T, MPW, NSPLIT, KQ>;
+  auto kern = k_lg_u<T, NSPLIT, KSPLIT, KQH>;
   PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  kern<<<grid, 256, lds, st>>>(a);
+  kern<<<grid, 256 * KSPLIT, lds, st>>>(a);
 }
 
-template <typename T, int MPW, int NSPLIT>
-static void lg_go_kq(const LgArgs<T>& a, int KQ, dim3 grid, size_t lds, hipStream_t st) {
-  switch (KQ) {
-    case 8: lg_go<T, MPW, NSPLIT, 8>(a, grid, lds, st); break;
-    case 16: lg_go<T, MPW, NSPLIT, 16>(a, grid, lds, st); break;
-    case 24: lg_go<T, MPW, NSPLIT, 24>(a, grid, lds, st); break;
-    case 32: lg_go<T, MPW, NSPLIT, 32>(a, grid, lds, st); break;
-    case 40: lg_go<T, MPW, NSPLIT, 40>(a, grid, lds, st); break;
-    default: lg_go<T, MPW, NSPLIT, 48>(a, grid, lds, st); break;
+template <typename T, int NSPLIT>
+static void lg_go_kq(const LgArgs<T>& a, int ksplit, int kqh, dim3 grid, size_t lds, hipStream_t st) {
+  if (ksplit == 2) {
+    switch (kqh) {
+      case 4: lg_go<T, NSPLIT, 2, 4>(a, grid, lds, st); break;
+      case 8: lg_go<T, NSPLIT, 2, 8>(a, grid, lds, st); break;
+      case 12: lg_go<T, NSPLIT, 2, 12>(a, grid, lds, st); break;
+      case 16: lg_go<T, NSPLIT, 2, 16>(a, grid, lds, st); break;
+      case 20: lg_go<T, NSPLIT, 2, 20>(a, grid, lds, st); break;
+      case 24: lg_go<T, NSPLIT, 2, 24>(a, grid, lds, st); break;
+      default: lg_go<T, NSPLIT, 2, 26>(a, grid, lds, st); break;
+    }
+  } else {
+    switch (kqh) {
+      case 8: lg_go<T, NSPLIT, 1, 8>(a, grid, lds, st); break;
+      case 16: lg_go<T, NSPLIT, 1, 16>(a, grid, lds, st); break;
+      case 24: lg_go<T, NSPLIT, 1, 24>(a, grid, lds, st); break;
+      case 32: lg_go<T, NSPLIT, 1, 32>(a, grid, lds, st); break;
+      case 40: lg_go<T, NSPLIT, 1, 40>(a, grid, lds, st); break;
+      case 48: lg_go<T, NSPLIT, 1, 48>(a, grid, lds, st); break;
+      default: lg_go<T, NSPLIT, 1, 52>(a, grid, lds, st); break;
+    }
   }
 }
 
@@ -476,11 +577,15 @@ static const double* vblock(pqg_ci* ci, int t, int u, double) { return ci->V64[t
 static const float* vblock(pqg_ci* ci, int t, int u, float) {
   return t == u ? ci->lg.V32s[t].get() : ci->V32[t][u].get();
 }
+static const double* aone(pqg_ci* ci, int t, double) { return ci->slot[t].A.get(); }
+static const float* aone(pqg_ci* ci, int t, float) { return ci->slot[t].A32.get(); }
+static const double* xblock(pqg_ci* ci, int t, int u, double) { return ci->X64[t][u].get(); }
+static const float* xblock(pqg_ci* ci, int t, int u, float) { return ci->X32[t][u].get(); }
 
 // U bytes per column of a pass, and the chunk width that keeps U within its budget
-static size_t lg_per_col(const pqg_ci* ci, const LgPass& d, size_t w) { return (size_t)d.n_r * d.L * w; }
+static size_t lg_per_col(const LgPass& d, size_t w) { return (size_t)d.n_r * d.L * w; }
 static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w) {
-  const size_t per_col = lg_per_col(ci, d, w);
+  const size_t per_col = lg_per_col(d, w);
   const size_t want = std::min<size_t>(lg_budget_bytes(ci->n_int, w), per_col * (size_t)n_c);
   const size_t budget = std::max<size_t>(per_col * LG_NT, want);
   int cw = (int)std::min<size_t>((size_t)n_c, budget / per_col);
@@ -488,57 +593,68 @@ static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w) {
   return cw;
 }
 
-// One pass of the link GEMM for one vector.
+// One pass of the link GEMM for one vector; accumulate: add to S or overwrite it.
 template <typename T>
-static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, hipStream_t st) {
+static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hipStream_t st) {
   LinkGemm& lg = ci->lg;
   const LgPass d = lg_pass_desc(ci, pass);
-  if (d.L == 0) return;
+  PQG_REQUIRE(d.L > 0, PQG_ERR_UNSUPPORTED, "link GEMM pass without links");
   const Slot& sc = ci->slot[d.col];
   LgArgs<T> a{};
   a.C = C;
+  a.nrow = d.nrow;
+  for (int u = 0; u < d.nrow; u++) {
+    a.rstride[u] = d.rstride[u];
+    a.rns[u] = ci->slot[d.rslot[u]].nstr;
+  }
   a.nsets = d.nsets;
   int off = 0;
   for (int q = 0; q < d.nsets; q++) {
-    const int t = d.slot[q];
+    const int t = d.rslot[d.act[q]];
+    a.act[q] = d.act[q];
     a.links[q] = lg.tab[t].links.get();
     a.Ls[q] = lg.tab[t].L;
     a.nvs[q] = lg.tab[t].nv;
-    a.ns[q] = ci->slot[t].nstr;
     a.loff[q] = off;
     a.np[q] = ci->slot[t].npair;
-    a.strides[q] = d.stride[q];
     a.Vrc[q] = vblock(ci, t, d.col, T{});
-    for (int q2 = 0; q2 < d.nsets; q2++) a.Vrr[q][q2] = vblock(ci, t, d.slot[q2], T{});
+    a.Aone[q] = aone(ci, t, T{});
+    for (int q2 = 0; q2 < d.nsets; q2++) a.Vrr[q][q2] = vblock(ci, t, d.rslot[d.act[q2]], T{});
+    for (int u = 0; u < d.nrow; u++) a.Xrow[q][u] = d.rslot[u] == t ? nullptr : xblock(ci, t, d.rslot[u], T{});
     off += lg.tab[t].L;
   }
-  a.tbm = d.pairs ? lg.tab[d.col].tbm.get() : nullptr;
+  a.map = lg.tab[d.col].tbm.get();
+  a.cbits = sc.bits.get();
+  a.nbo = d.nbo;
+  a.ncd = d.ncd;
+  a.nin = d.nin;
   a.np_c = sc.npair;
   a.n_r = (int)d.n_r;
   a.n_c = sc.nstr;
   a.L = d.L;
-  a.nb = d.pairs ? lg.tab[d.col].nb : 0;
-  a.K = a.nb + a.L;
-  const int KQ = lg_kq_bound((a.K + 3) / 4);
-  constexpr int NTP = LG_NT + 16 / (int)sizeof(T);
+  a.K = d.nin + d.L;
+  a.accumulate = accumulate ? 1 : 0;
+  const int ksplit = lg_ksplit(ci, sizeof(T));
+  const int kqh = lg_kqh_bound((a.K + 3) / 4, ksplit);
   auto r16 = [](size_t x) { return (x + 15) / 16 * 16; };
-  a.lab_off = (int)r16((size_t)a.L * sizeof(int2));
+  a.lab_off = (int)r16((size_t)a.L * 8);
   a.kbf_off = (int)r16(a.lab_off + (size_t)a.L * 4);
-  a.crow_off = (int)r16(a.kbf_off + (size_t)a.nb * 4);
-  a.tile_off = (int)r16(a.crow_off + (a.nb > 0 ? (size_t)a.n_c * sizeof(T) : 0));
-  const size_t lds = a.tile_off + (size_t)4 * KQ * NTP * sizeof(T);
+  a.crow_off = (int)r16(a.kbf_off + (size_t)a.nin * 4);
+  a.tile_off = (int)r16(a.crow_off + (size_t)a.n_c * sizeof(T));
+  const size_t lds = lg_lds_bytes(d, a.n_c, kqh, ksplit, sizeof(T));
   // U holds n_r * L * cw values; sized for the larger of the two passes so it grows once
   const int cw = lg_chunk(ci, d, a.n_c, sizeof(T));
-  size_t need = lg_per_col(ci, d, sizeof(T)) * (size_t)cw;
+  size_t need = lg_per_col(d, sizeof(T)) * (size_t)cw;
   {
     const LgPass o = lg_pass_desc(ci, 1 - pass);
     const int n_co = ci->slot[o.col].nstr;
-    need = std::max(need, lg_per_col(ci, o, sizeof(T)) * (size_t)lg_chunk(ci, o, n_co, sizeof(T)));
+    need = std::max(need, lg_per_col(o, sizeof(T)) * (size_t)lg_chunk(ci, o, n_co, sizeof(T)));
   }
   grow(lg.U, ci->ctx, need, st);
   a.U = reinterpret_cast<T*>(lg.U.get());
   const int mb = (a.L + 15) / 16;
-  const int64_t flops_per_col = 2ll * a.n_r * (16ll * mb) * (4ll * KQ);
+  // algorithmic flops of the GEMM as posed (M = L links, K = in-row rows + links, no padding)
+  const int64_t flops_per_col = 2ll * a.n_r * a.L * a.K;
   for (int c0 = 0; c0 < a.n_c; c0 += cw) {
     a.c0 = c0;
     a.cw = std::min(cw, a.n_c - c0);
@@ -548,8 +664,8 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, hipStream_t st) {
     dim3 grid((unsigned)a.n_r, (unsigned)lg_grid(a.cw, a.gcols));
     {
       ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "lg_gemm_f64" : "lg_gemm_f32", flops_per_col * a.cw);
-      if (mb <= 2) lg_go_kq<T, 1, 2>(a, KQ, grid, lds, st);
-      else lg_go_kq<T, 1, 1>(a, KQ, grid, lds, st);
+      if (mb <= 2) lg_go_kq<T, 2>(a, ksplit, kqh, grid, lds, st);
+      else lg_go_kq<T, 1>(a, ksplit, kqh, grid, lds, st);
       PQG_HIP(hipGetLastError());
     }
     {
@@ -563,7 +679,24 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, hipStream_t st) {
   }
 }
 
-void link_gemm_pass(pqg_ci* ci, int pass, const double* C, double* S, hipStream_t st) { lg_pass<double>(ci, pass, C, S, st); }
-void link_gemm_pass(pqg_ci* ci, int pass, const float* C, float* S, hipStream_t st) { lg_pass<float>(ci, pass, C, S, st); }
+void link_gemm_pass(pqg_ci* ci, int pass, const double* C, double* S, bool accumulate, hipStream_t st) {
+  lg_pass<double>(ci, pass, C, S, accumulate, st);
+}
+void link_gemm_pass(pqg_ci* ci, int pass, const float* C, float* S, bool accumulate, hipStream_t st) {
+  lg_pass<float>(ci, pass, C, S, accumulate, st);
+}
+
+template <typename T>
+static void lg_diag(pqg_ci* ci, const T* C, int64_t ldc, T* S, int64_t lds, int k, hipStream_t st) {
+  dim3 g((unsigned)lg_grid(ci->n_int, 256), (unsigned)k);
+  k_lg_diag<T><<<g, 256, 0, st>>>(ci->hdiag.get(), C, ldc, S, lds, ci->n_int);
+  PQG_HIP(hipGetLastError());
+}
+void link_gemm_diag(pqg_ci* ci, const double* C, int64_t ldc, double* S, int64_t lds, int k, hipStream_t st) {
+  lg_diag<double>(ci, C, ldc, S, lds, k, st);
+}
+void link_gemm_diag(pqg_ci* ci, const float* C, int64_t ldc, float* S, int64_t lds, int k, hipStream_t st) {
+  lg_diag<float>(ci, C, ldc, S, lds, k, st);
+}
 
 }  // namespace pqg
