@@ -355,8 +355,9 @@ def bench_sigma(args):
     """Sigma throughput of the link-GEMM path: 16 orbitals 8a8b (165,636,900 determinants) or
     water/6-31G (1,656,369), k = 4 vectors per application, FP64 and FP32, with the roofline of
     the link-GEMM kernel.  Algorithmic flops of one k_lg_u launch = 2 * rows * L * K * columns
-    (M = L links per row string, K = off-diagonal column pairs + L, the GEMM as posed, without
-    padding); launch durations from HIP events on the CI stream."""
+    (M = L links per row string, K = the column set's off-diagonal and diagonal pairs + the own
+    row + L, the GEMM as posed, without padding); launch durations from HIP events on the CI
+    stream."""
     import numpy as np
     import torch
 
@@ -373,13 +374,15 @@ def bench_sigma(args):
     n, na, nb = sp.n_orb, sp.n_alpha, sp.n_beta
     n_a, n_b = math.comb(n, na), math.comb(n, nb)
     La, Lb = na * (n - na), nb * (n - nb)
-    # GEMM flops of one application per vector: pass A (rows alpha, K = beta pairs + alpha links)
-    # and pass B (rows beta, K = beta links)
-    gemm_flops = 2.0 * n_a * La * (n * (n - 1) // 2 + La) * n_b + 2.0 * n_b * Lb * Lb * n_a
+    # GEMM flops of one application per vector: pass A (rows alpha; K = beta off-diagonal and
+    # diagonal pairs, the own row, the alpha links) and pass B (rows beta; K = alpha diagonal
+    # pairs, the own row, the beta links)
+    gemm_flops = (2.0 * n_a * La * (n * (n - 1) // 2 + n + 1 + La) * n_b
+                  + 2.0 * n_b * Lb * (n + 1 + Lb) * n_a)
     C = torch.randn(N * k, dtype=torch.float64, device="cuda") / math.sqrt(N)
     S = torch.empty_like(C)
     torch.cuda.synchronize()
-    names = ["sigma_pass_nc1", "lg_gemm", "lg_gather"]
+    names = ["lg_gemm", "lg_gather"]
     res = {}
     for tag, prec in (("fp64", PQG_FP64), ("fp32", PQG_MIXED)):
         suf = "_f64" if prec == PQG_FP64 else "_f32"
@@ -403,8 +406,7 @@ def bench_sigma(args):
                     "lg_gemm_ms": round(gemm_s * 1e3, 3), "lg_gemm_launches_per_sigma": launches // args.steps,
                     "lg_gemm_tflops": round(achieved, 3), "lg_gemm_frac_of_peak": round(achieved / peak, 4),
                     "peak_tflops": peak,
-                    "lg_gather_ms": round(prof["lg_gather"][1] / args.steps, 3),
-                    "enumeration_pass_ms": round(prof["sigma_pass_nc1"][1] / args.steps, 3)}
+                    "lg_gather_ms": round(prof["lg_gather"][1] / args.steps, 3)}
         log(f"[sigma] {tag}: {res[tag]}")
     fp64 = res["fp64"]
     out = {
