@@ -69,7 +69,8 @@ def cpu_baseline(prob, sigma_calls, nvec, threads):
 
 PMC_SUMMARY = {"tensor_gemm_f64": "k_pair_gemm_f64_ep_fci_48e_48p_pmc.md",
                "sigma_pass_nc3_f64": "k_sigma_pass_nc3_f64_ep_fci_48e_48p_pmc.md",
-               "sigma_pass_nc1_f64": "k_sigma_pass_nc1_f64_ep_fci_48e_48p_pmc.md"}
+               "sigma_pass_nc1_f64": "k_sigma_pass_nc1_f64_ep_fci_48e_48p_pmc.md",
+               "rdm_group": "k_rdm_group_fci_16o_8a8b.md"}
 
 
 def traffic_from_profiles(kernel):
@@ -425,6 +426,100 @@ def bench_sigma(args):
     ctx.close()
 
 
+def bench_rdm(args):
+    """One-particle density matrices (create_1rdm) of k = 4 vectors of the 16-orbital 8a8b space
+    (165,636,900 determinants), alpha and beta string sets.  k_rdm_group is HBM-bound: its
+    algorithmic bytes per launch are the vectors read once (N * 8 * k) plus the group Gram
+    entries it writes (C(16, 7) groups x 45 x k x 8); launch durations from HIP events on the CI
+    stream."""
+    import numpy as np
+    import torch
+
+    from polyquant_amd import workloads as wl
+    from polyquant_amd.ci import Context, DetSet
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    prob = wl.config3()
+    ctx = Context(0)
+    ds = DetSet(prob, ctx)
+    sp = prob.species[0]
+    N, k, n = ds.N_dets, 4, sp.n_orb
+    n_t, L = math.comb(n, sp.n_alpha), sp.n_alpha * (n - sp.n_alpha)
+    C = torch.randn(N * k, dtype=torch.float64, device="cuda") / math.sqrt(N)
+    D = torch.empty(k * n * n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        for spin in (0, 1):
+            ds.create_1rdm_device(C.data_ptr(), N, k, 0, spin, D.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    ctx.profile(True)
+    for nm in ("rdm_group", "rdm_links"):
+        ctx.profile_read(nm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ctx.profile(False)
+    launches, kern_ms, _ = ctx.profile_read("rdm_group")
+    kernel = "k_rdm_group<9>"
+    m = n - sp.n_alpha + 1
+    out_bytes = math.comb(n, sp.n_alpha - 1) * (m * (m + 1) // 2) * k * 8
+    if launches == 0:
+        launches, kern_ms, _ = ctx.profile_read("rdm_links")
+        kernel = "k_rdm_links"
+        out_bytes = n_t * (L + 1) * k * 8
+    per_launch_s = kern_ms * 1e-3 / launches
+    alg_bytes = N * 8 * k + out_bytes
+    achieved = alg_bytes / per_launch_s / 1e9
+    log(f"[rdm] {dt * 1e3:.2f} ms per step (2 string sets x {k} vectors), {kernel} "
+        f"{per_launch_s * 1e3:.3f} ms/launch, {achieved:.0f} GB/s")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import refcpu
+
+        t0 = time.perf_counter()
+        oc = refcpu.OracleCI(prob)
+        log(f"[rdm] oracle determinant set {time.perf_counter() - t0:.1f}s")
+        c0 = C[:N].cpu().numpy()
+        s_big, s_small = 4096, 512
+        t0 = time.perf_counter()
+        oc.rdm1_sample(c0, 0, 0, s_big)
+        t_big = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oc.rdm1_sample(c0, 0, 0, s_small)
+        t_small = time.perf_counter() - t0
+        rows_big, rows_small = -(-N // s_big), -(-N // s_small)
+        per_det = max((t_small - t_big) / (rows_small - rows_big), 1e-12)
+        t_step = per_det * N * 2 * k
+        cpu = {"value": round(N * 2 * k / t_step, 1), "unit": "determinant-vectors/s", "cores": args.cpu_threads,
+               "kind": "port",
+               "sample": (f"oracle create_1rdm on determinants i = 0 mod {s_small} ({rows_small} of {N}) and mod "
+                          f"{s_big}, one vector, alpha set; {per_det * 1e6:.3f} us per determinant on "
+                          f"{args.cpu_threads} threads -> {t_step:.1f} s for 2 sets x {k} vectors")}
+        log(f"[rdm] cpu {cpu}")
+    out = {
+        "metric": "one-particle density matrix throughput", "value": round(N * 2 * k / dt, 1),
+        "unit": "determinant-vectors/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"fci_16o_8a8b: 1-RDM of the alpha and beta string sets, {N} determinants, "
+                                f"k = {k} vectors per call")},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4),
+                     "traffic": traffic_from_profiles("rdm_group") if kernel.startswith("k_rdm_group") else None,
+                     "kernel": kernel,
+                     "algorithmic_bytes_per_launch": alg_bytes, "ms_per_launch": round(per_launch_s * 1e3, 4)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ds.close()
+    ctx.close()
+
+
 def bench_h2o(args):
     """Water/6-31G full CI (13 orbitals, 5a5b, 1,656,369 determinants), RHF orbitals from the CPU."""
     import numpy as np
@@ -490,7 +585,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--workload", default="davidson", choices=["davidson", "ao2mo", "fci16", "serve", "h2o", "sigma"],
+    ap.add_argument("--workload", default="davidson", choices=["davidson", "ao2mo", "fci16", "serve", "h2o", "sigma", "rdm"],
                     help="davidson (default): ep-FCI 48/48/48 Davidson solves; ao2mo: n = 320 transform")
     ap.add_argument("--h2o-nvec", type=int, default=5)
     ap.add_argument("--sigma-workload", default="fci16", choices=["fci16", "h2o"])
@@ -539,6 +634,9 @@ def main():
         return
     if args.workload == "h2o":
         bench_h2o(args)
+        return
+    if args.workload == "rdm":
+        bench_rdm(args)
         return
     if args.workload == "sigma":
         bench_sigma(args)

@@ -124,6 +124,13 @@ pqg_status pqg_ci_sigma(pqg_ci* ci, const double* C, int64_t ldc, int k, double*
 pqg_status pqg_ci_davidson(pqg_ci* ci, const pqg_dav_params* params, double* E, double* X, int64_t ldx,
                            pqg_dav_report* report);
 
+/* create_1rdm (src/ci/detset/rdm1.cpp:6-79) for k states at once: the one-particle reduced
+ * density matrix of the (part, spin) string set for each column of C (N x k, reference order,
+ * column-major).  D receives k consecutive n_orb x n_orb column-major blocks, element (k, l) =
+ * <C| a+_l a_k |C> of that set (symmetric).  on_device != 0: C and D are device pointers.   */
+pqg_status pqg_ci_rdm1(pqg_ci* ci, const double* C, int64_t ldc, int k, int part, int spin, double* D,
+                       int on_device);
+
 /* --- parity dumps ------------------------------------------------------------------------ */
 /* unique_dets[part][spin] strings in the reference order (determinant_creation.cpp:139-146) */
 pqg_status pqg_ci_dump_strings(pqg_ci* ci, int part, int spin, uint64_t* out);

@@ -11,6 +11,7 @@
 //   * symmetric-scatter ("singleshot") sigma ....... src/ci/detset/direct_{single,mixed}particle.cpp
 //   * slow sigma ................................... src/ci/detset/direct.cpp:43-53
 //   * Davidson driver wiring ....................... src/ci/epci.cpp:589-621 (Spectra fork behaviour)
+//   * one-particle density matrix .................. src/ci/detset/rdm1.cpp:6-79
 //   * AO->MO quarter transform ..................... src/integral/integral.cpp:207-516
 //
 // Documented deviations (arithmetic is unchanged):
@@ -879,6 +880,61 @@ void rc_diag(void* h, double* out) {
 }
 double rc_slater_condon(void* h, int64_t i, int64_t j) { return slater_condon(*static_cast<DetSet*>(h), i, j); }
 
+// create_1rdm (src/ci/detset/rdm1.cpp:6-79): one-particle density matrix of the string set
+// (p, s) for state vector C (reference address order), n x n column-major, rows/cols = active
+// orbitals.  Round-robin OpenMP determinants with thread-private matrices summed at the end
+// (rdm1.cpp:18-72).  Diagonal: C_i^2 on every occupied orbital of the (p, s) string (:29-35).
+// Off-diagonal: each single excitation j of that string (unique_singles, :38) whose determinant
+// is in the space (:41-43) and has a smaller address (:46-48) adds phase * C_i * C_j to (hole,
+// part) and (part, hole) (:49-67).  Contributions with |c| <= 1e-16 are dropped (:10, :33, :64).
+// The orbital-symmetry filter (:62-64) always passes for C1 labels.
+void rdm1(const DetSet& d, int p, int s, const double* C, double* D, int64_t stride = 1) {
+  const int n = d.n_orb[p];
+  const int q = 2 * p + s;
+  constexpr double thresh = 1e-16;
+  const int nth = omp_get_max_threads();
+  std::vector<std::vector<double>> priv(nth, std::vector<double>((size_t)n * n, 0.0));
+#pragma omp parallel
+  {
+    const int tid = omp_get_thread_num();
+    const int nt = omp_get_num_threads();
+    double* M = priv[tid].data();
+    std::vector<int> occ, virt, holes, parts;
+    for (int64_t r = tid; r * stride < d.N; r += nt) {
+      const int64_t i = r * stride;
+      const int32_t* I = d.unfold[i].data();
+      const uint64_t Di = d.strings[p][s][I[q]];
+      occ.clear();
+      virt.clear();
+      get_occ_virt(&Di, 1, n, occ, virt);
+      const double cii = C[i] * C[i];
+      if (std::abs(cii) > thresh)
+        for (int o : occ) M[o + (size_t)o * n] += cii;
+      for (int jdx : d.singles[p][s][I[q]]) {
+        int32_t J[4] = {I[0], I[1], I[2], I[3]};
+        J[q] = jdx;
+        const int32_t j = d.address(J);
+        if (j < 0 || i < j) continue;
+        const uint64_t Dj = d.strings[p][s][jdx];
+        holes.clear();
+        parts.clear();
+        get_holes(&Di, &Dj, 1, holes);
+        get_parts(&Di, &Dj, 1, parts);
+        const double c = get_phase(&Di, 1, holes, parts) * C[i] * C[j];
+        if (std::abs(c) > thresh) {
+          M[holes[0] + (size_t)parts[0] * n] += c;
+          M[parts[0] + (size_t)holes[0] * n] += c;
+        }
+      }
+    }
+  }
+  for (size_t e = 0; e < (size_t)n * n; e++) {
+    double v = 0.0;
+    for (int t = 0; t < nth; t++) v += priv[t][e];
+    D[e] = v;
+  }
+}
+
 // operator* / create_sigma (direct.cpp:5-34)
 int rc_sigma(void* h, const double* C, int64_t ldc, int k, double* S, int64_t lds) {
   auto* d = static_cast<DetSet*>(h);
@@ -886,6 +942,29 @@ int rc_sigma(void* h, const double* C, int64_t ldc, int k, double* S, int64_t ld
     sigma_one(*d, C, ldc, k, S, lds);
   else
     sigma_two(*d, C, ldc, k, S, lds);
+  return 0;
+}
+
+// create_1rdm for k state vectors (columns of C): D receives k n x n column-major blocks.
+int rc_rdm1(void* h, const double* C, int64_t ldc, int k, int p, int s, double* D) {
+  auto* d = static_cast<DetSet*>(h);
+  if (p < 0 || p >= d->nsp || s < 0 || s > 1) {
+    g_err = "particle or spin index out of range";
+    return 1;
+  }
+  const size_t nn = (size_t)d->n_orb[p] * d->n_orb[p];
+  for (int v = 0; v < k; v++) rdm1(*d, p, s, C + (size_t)v * ldc, D + v * nn);
+  return 0;
+}
+
+// create_1rdm restricted to determinants i = 0, stride, 2*stride, ... (timing sample).
+int rc_rdm1_sample(void* h, const double* C, int64_t ldc, int p, int s, double* D, int64_t stride) {
+  auto* d = static_cast<DetSet*>(h);
+  if (p < 0 || p >= d->nsp || s < 0 || s > 1) {
+    g_err = "particle or spin index out of range";
+    return 1;
+  }
+  rdm1(*d, p, s, C, D, stride < 1 ? 1 : stride);
   return 0;
 }
 

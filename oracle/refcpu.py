@@ -60,6 +60,8 @@ def lib():
         L.rc_slater_condon.argtypes = [ct.c_void_p, ct.c_int64, ct.c_int64]
         L.rc_sigma.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64]
         L.rc_sigma_slow.argtypes = L.rc_sigma.argtypes
+        L.rc_rdm1.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]
+        L.rc_rdm1_sample.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_void_p, ct.c_int64]
         L.rc_sigma_sample.argtypes = L.rc_sigma.argtypes + [ct.c_int64]
         L.rc_sigma_rows.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
                                     ct.c_void_p]
@@ -209,6 +211,27 @@ class OracleCI:
         S = np.zeros_like(C, order="F")
         lib().rc_sigma(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(S), S.shape[0])
         return S
+
+    def rdm1(self, C, part=0, spin=0):
+        """create_1rdm (rdm1.cpp:6-79) of the (part, spin) string set: (n, n) for a 1-D C,
+        else (k, n, n) with D[v][row][col]."""
+        one = np.ndim(C) == 1
+        C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)
+        n = self.prob.species[part].n_orb
+        D = np.zeros((C.shape[1], n, n))
+        if lib().rc_rdm1(self.h, _ptr(C), C.shape[0], C.shape[1], part, spin, _ptr(D)) != 0:
+            raise ValueError(lib().rc_last_error().decode())
+        D = D.transpose(0, 2, 1)
+        return D[0] if one else D
+
+    def rdm1_sample(self, c, part, spin, stride):
+        """create_1rdm over determinants i = 0 mod stride only (a timing sample)."""
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        n = self.prob.species[part].n_orb
+        D = np.zeros((n, n))
+        if lib().rc_rdm1_sample(self.h, _ptr(c), c.shape[0], part, spin, _ptr(D), int(stride)) != 0:
+            raise ValueError(lib().rc_last_error().decode())
+        return D.T
 
     def sigma_sample(self, C, stride):
         C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)

@@ -174,6 +174,23 @@ class DetSet:
         """sigma on device buffers (e.g. torch tensors' data_ptr())."""
         check(load().pqg_ci_sigma(self.h, ct.c_void_p(C_ptr), ldc, k, ct.c_void_p(S_ptr), lds, 1, precision))
 
+    def create_1rdm(self, C: np.ndarray, part: int = 0, spin: int = 0) -> np.ndarray:
+        """create_1rdm (src/ci/detset/rdm1.cpp:6-79) of the (part, spin) string set for every
+        column of C (host, reference order): an (n_orb, n_orb) array for a 1-D C, else
+        (k, n_orb, n_orb).  Computed on the GPU."""
+        one = np.ndim(C) == 1
+        C = np.asfortranarray(np.atleast_2d(np.asarray(C, dtype=np.float64).T).T)
+        # an out-of-range part is passed through so that the library reports it
+        n = self.prob.species[part].n_orb if 0 <= part < self.prob.nspecies else 1
+        D = np.zeros((C.shape[1], n, n))
+        check(load().pqg_ci_rdm1(self.h, _ptr(C), C.shape[0], C.shape[1], part, spin, _ptr(D), 0))
+        D = D.transpose(0, 2, 1)  # column-major blocks -> D[k][row][col]
+        return D[0] if one else D
+
+    def create_1rdm_device(self, C_ptr: int, ldc: int, k: int, part: int, spin: int, D_ptr: int):
+        """create_1rdm on device buffers: D receives k column-major n_orb x n_orb blocks."""
+        check(load().pqg_ci_rdm1(self.h, ct.c_void_p(C_ptr), ldc, k, part, spin, ct.c_void_p(D_ptr), 1))
+
     def __matmul__(self, C):
         return self.sigma(C)
 
@@ -241,3 +258,31 @@ class EPCI:
         self.C_ci = X
         self.report = rep
         return self.energies
+
+    def calculate_NOs(self, states=None, orbitals=None):
+        """POLYQUANT_EPCI::calculate_NOs (src/ci/epci.cpp:200-311) without frozen core: the
+        1-RDM of every (species, spin) for each listed state on the GPU, then diag_dm_helper
+        (epci.cpp:131-146) on the host: occupations in descending order and the natural orbitals,
+        in the MO basis or, when orbitals[part][spin] (MO coefficients) is given, in its basis.
+        A restricted species (one spin block) takes its beta matrix from the alpha one
+        (epci.cpp:233-235).  Returns {state: {(part, spin): (dm1, occ, orbs)}}."""
+        if self.C_ci is None:
+            raise RuntimeError("run() first")
+        states = list(range(self.num_states)) if states is None else list(states)
+        out = {st: {} for st in states}
+        C = self.C_ci[:, states]
+        for part, sp in enumerate(self.prob.species):
+            for spin in range(2 if (sp.n_alpha + sp.n_beta) > 1 else 1):
+                if spin == 1 and sp.nspin_blocks == 1:
+                    for st in states:
+                        out[st][(part, 1)] = out[st][(part, 0)]
+                    continue
+                D = self.detset.create_1rdm(C, part, spin)
+                for q, st in enumerate(states):
+                    occ, vec = np.linalg.eigh(D[q])
+                    order = np.argsort(-occ, kind="stable")
+                    occ, vec = occ[order], vec[:, order]
+                    if orbitals is not None:
+                        vec = np.asarray(orbitals[part][spin]) @ vec
+                    out[st][(part, spin)] = (D[q], occ, vec)
+        return out

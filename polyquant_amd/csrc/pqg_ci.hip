@@ -32,30 +32,14 @@
 #include <unordered_map>
 
 #include "pqg_ci.hpp"
+#include "pqg_strings.hpp"
 
 namespace pqg {
 
 // ---------------------------------------------------------------------------------------
 // device helpers
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ int ctz64(uint64_t x) { return __builtin_ctzll(x); }
 __device__ __forceinline__ int pairidx(int i, int j) { return i > j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
-__device__ __forceinline__ uint64_t low_mask(int n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
-// bits strictly between lo and hi
-__device__ __forceinline__ uint64_t between_mask(int lo, int hi) { return low_mask(hi) & ~low_mask(lo + 1); }
-
-__device__ __forceinline__ int str_index(const SlotDev& s, const uint32_t* tab, uint64_t J) {
-  if (s.nocc == 1 && !s.r2i) return __builtin_ctzll(J);  // colex rank of {p} is p
-  uint32_t r = 0;
-  int e = s.nocc;
-  const int w = s.nocc + 1;
-  for (int c = s.nchunk - 1; c >= 0; --c) {
-    const uint32_t x = (uint32_t)(J >> (8 * c)) & 255u;
-    r += tab[((c << 8) + x) * w + e];
-    e -= __popc(x);
-  }
-  return s.r2i ? s.r2i[r] : (int)r;
-}
 
 // Visit all single excitations I -> J of one string: (e, h, p, Q = idx2(h, p), odd phase, J).
 // e enumerates occupied h (ascending) outer, virtual p (ascending) inner.

@@ -31,6 +31,12 @@ struct Slot {
   DevBuf<uint32_t> rtab;
   DevBuf<int32_t> r2i;
   DevBuf<int32_t> d_ref2int;
+  // one-particle density matrix: groups of the strings that share nocc - 1 orbitals (built on
+  // first use): bits of each shared core and the string index of each member (-1: not in set)
+  int64_t ngroup = 0;
+  int gm = 0;                      // members per group, n_orb - nocc + 1
+  DevBuf<uint64_t> gbits;
+  DevBuf<int32_t> gmem;
   DevBuf<double> A, D, E1;         // singles same-slot part, doubles values, diagonal part
   DevBuf<float> A32, D32;
   SlotDev dev() const {

@@ -269,3 +269,75 @@ def test_sigma_rows_equals_full_sigma(two_species):
     C = np.random.default_rng(1).standard_normal((oc.N, 2))
     rows = np.arange(0, oc.N, 3)
     assert np.max(np.abs(oc.sigma_rows(C, rows) - oc.sigma(C)[rows])) < 1e-13 * max(1.0, np.max(np.abs(C)))
+
+
+def _rdm_bruteforce(prob, dets, c, slot, n):
+    """<c| a+_p a_q |c> of one string slot by second quantisation on the determinant tuples."""
+    index = {d: i for i, d in enumerate(dets)}
+    M = np.zeros((n, n))
+    for j, det in enumerate(dets):
+        if c[j] == 0.0:
+            continue
+        for p in range(n):
+            for q in range(n):
+                r = bruteforce.apply_E(det[slot], p, q)
+                if r is None:
+                    continue
+                sg, ns = r
+                i = index.get(det[:slot] + (ns,) + det[slot + 1:])
+                if i is not None:
+                    M[p, q] += sg * c[i] * c[j]
+    return M
+
+
+RDM_KINDS = ["restricted", "unrestricted", "truncated", "open_shell", "positron", "positron_two_spin"]
+
+
+def _rdm_problem(kind):
+    return {
+        "restricted": lambda: wl.small_problem(41, 5, 2, 2),
+        "unrestricted": lambda: wl.small_problem(42, 5, 2, 2, nspin=2),
+        "truncated": lambda: wl.small_problem(43, 6, 3, 3, ex_level=(2, 2, 2)),
+        "open_shell": lambda: wl.small_problem(44, 5, 3, 1),
+        "positron": lambda: wl.small_problem(45, 4, 1, 1, positron=(4, 1, 0)),
+        "positron_two_spin": lambda: wl.small_problem(46, 4, 2, 1, positron=(3, 2, 1)),
+    }[kind]()
+
+
+@pytest.mark.parametrize("kind", RDM_KINDS)
+def test_rdm1_matches_bruteforce(kind):
+    """create_1rdm (rdm1.cpp:6-79) against <C|a+_p a_q|C> built from creation/annihilation
+    operators on occupation strings (Jordan-Wigner sign), for random vectors and the ground
+    state; the trace is the particle count of the set for a normalised vector."""
+    prob = _rdm_problem(kind)
+    oc = refcpu.OracleCI(prob)
+    dets = bruteforce.oracle_dets(oc)
+    rng = np.random.default_rng(7)
+    C = rng.standard_normal((oc.N, 2))
+    C /= np.linalg.norm(C, axis=0)
+    w, V = np.linalg.eigh(oc.dense())
+    C = np.column_stack([C, V[:, 0]])
+    slots = [(p, s) for p in range(prob.nspecies) for s in range(2)]
+    for si, (p, s) in enumerate(slots):
+        n = prob.species[p].n_orb
+        nocc = prob.species[p].n_alpha if s == 0 else prob.species[p].n_beta
+        D = oc.rdm1(C, p, s)
+        for v in range(C.shape[1]):
+            M = _rdm_bruteforce(prob, dets, C[:, v], si, n)
+            assert np.max(np.abs(D[v] - M)) < 1e-13, (kind, p, s, v)
+            assert np.allclose(D[v], D[v].T, atol=1e-14)
+            assert abs(np.trace(D[v]) - nocc) < 1e-12
+
+
+def test_rdm1_single_determinant_is_occupation_diagonal():
+    """A unit vector on one determinant gives diag(occupation numbers of its string)."""
+    prob = wl.small_problem(47, 6, 3, 2)
+    oc = refcpu.OracleCI(prob)
+    U = oc.unfold_all()
+    for addr in (0, 7, oc.N - 1):
+        c = np.zeros(oc.N)
+        c[addr] = 1.0
+        for s in range(2):
+            bits = int(oc.strings(0, s)[U[addr, s]])
+            D = oc.rdm1(c, 0, s)
+            assert np.array_equal(D, np.diag([(bits >> o) & 1 for o in range(6)]).astype(float))
