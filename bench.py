@@ -478,6 +478,20 @@ def bench_rdm(args):
     achieved = alg_bytes / per_launch_s / 1e9
     log(f"[rdm] {dt * 1e3:.2f} ms per step (2 string sets x {k} vectors), {kernel} "
         f"{per_launch_s * 1e3:.3f} ms/launch, {achieved:.0f} GB/s")
+    # <S^2> of the same vectors (evaluate_s2): boundary permutation + k_s2
+    ctx.profile(True)
+    ctx.profile_read("s2")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s2v = ds.evaluate_s2_device(C.data_ptr(), N, k)
+    s2_dt = (time.perf_counter() - t0) / args.steps
+    ctx.profile(False)
+    s2_launches, s2_ms, _ = ctx.profile_read("s2")
+    s2_launch_s = s2_ms * 1e-3 / s2_launches
+    s2 = {"ms_per_call_4_vectors": round(s2_dt * 1e3, 3), "k_s2_ms": round(s2_launch_s * 1e3, 3),
+          "k_s2_algorithmic_GBps": round(N * 8 * k / s2_launch_s / 1e9, 1),
+          "values": [round(float(x), 6) for x in s2v[:, 0]]}
+    log(f"[rdm] S^2: {s2}")
     cpu = None
     if not args.no_cpu:
         from oracle import refcpu
@@ -513,7 +527,7 @@ def bench_rdm(args):
                      "traffic": traffic_from_profiles("rdm_group") if kernel.startswith("k_rdm_group") else None,
                      "kernel": kernel,
                      "algorithmic_bytes_per_launch": alg_bytes, "ms_per_launch": round(per_launch_s * 1e3, 4)},
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu, "s2": s2,
     }
     print(json.dumps(out), flush=True)
     ds.close()

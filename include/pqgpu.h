@@ -131,6 +131,12 @@ pqg_status pqg_ci_davidson(pqg_ci* ci, const pqg_dav_params* params, double* E, 
 pqg_status pqg_ci_rdm1(pqg_ci* ci, const double* C, int64_t ldc, int k, int part, int spin, double* D,
                        int on_device);
 
+/* evaluate_s2 (src/ci/detset/s2.cpp:5-130): <C|S^2|C> of each species for each column of C
+ * (N x k, reference order).  S2 receives k x nspecies values, column-major
+ * (S2[state + species * k], the reference's S_squared(state, species)), always host memory.
+ * on_device != 0: C is a device pointer.                                                   */
+pqg_status pqg_ci_s2(pqg_ci* ci, const double* C, int64_t ldc, int k, double* S2, int on_device);
+
 /* --- parity dumps ------------------------------------------------------------------------ */
 /* unique_dets[part][spin] strings in the reference order (determinant_creation.cpp:139-146) */
 pqg_status pqg_ci_dump_strings(pqg_ci* ci, int part, int spin, uint64_t* out);

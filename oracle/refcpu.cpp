@@ -12,6 +12,7 @@
 //   * slow sigma ................................... src/ci/detset/direct.cpp:43-53
 //   * Davidson driver wiring ....................... src/ci/epci.cpp:589-621 (Spectra fork behaviour)
 //   * one-particle density matrix .................. src/ci/detset/rdm1.cpp:6-79
+//   * <S^2> per species ............................ src/ci/detset/s2.cpp:5-130
 //   * AO->MO quarter transform ..................... src/integral/integral.cpp:207-516
 //
 // Documented deviations (arithmetic is unchanged):
@@ -945,6 +946,60 @@ int rc_sigma(void* h, const double* C, int64_t ldc, int k, double* S, int64_t ld
   return 0;
 }
 
+// evaluate_s2 (src/ci/detset/s2.cpp:5-130): <C|S^2|C> of each species for one state vector.
+// Diagonal (:44-77): with xor = alpha ^ beta (unpaired orbitals) and na = xor & alpha,
+// sz = |na| - |xor| / 2 and the element is |na| + sz (sz - 1).  Off-diagonal (:80-112): every
+// alpha single i -> j combined with every beta single j -> i of the same determinant, in the
+// space, adds -phase_a * phase_b * C_I * C_J (both orders of each pair are visited).
+void s2(const DetSet& d, const double* C, double* out /* [nsp] */) {
+  const int nth = omp_get_max_threads();
+  std::vector<std::array<double, 2>> priv(nth, std::array<double, 2>{0.0, 0.0});
+#pragma omp parallel
+  {
+    const int tid = omp_get_thread_num();
+    const int nt = omp_get_num_threads();
+    std::vector<int> ha, pa, hb, pb;
+    for (int64_t i = tid; i < d.N; i += nt) {
+      const double ci = C[i];
+      if (ci == 0.0) continue;
+      const int32_t* I = d.unfold[i].data();
+      for (int p = 0; p < d.nsp; p++) {
+        const uint64_t Da = d.strings[p][0][I[2 * p]], Db = d.strings[p][1][I[2 * p + 1]];
+        const uint64_t x = Da ^ Db, na = x & Da;
+        const int num_total = std::popcount(x), num_a = std::popcount(na);
+        const double sz = num_a - 0.5 * num_total;
+        priv[tid][p] += ci * ci * (num_a + sz * (sz - 1.0));
+        for (int ja : d.singles[p][0][I[2 * p]]) {
+          for (int jb : d.singles[p][1][I[2 * p + 1]]) {
+            int32_t J[4] = {I[0], I[1], I[2], I[3]};
+            J[2 * p] = ja;
+            J[2 * p + 1] = jb;
+            const int32_t j = d.address(J);
+            if (j < 0) continue;
+            const double cj = C[j];
+            if (cj == 0.0) continue;
+            const uint64_t Ea = d.strings[p][0][ja], Eb = d.strings[p][1][jb];
+            ha.clear(); pa.clear(); hb.clear(); pb.clear();
+            get_holes(&Da, &Ea, 1, ha);
+            get_parts(&Da, &Ea, 1, pa);
+            get_holes(&Db, &Eb, 1, hb);
+            get_parts(&Db, &Eb, 1, pb);
+            if (ha[0] == pb[0] && hb[0] == pa[0]) {
+              const double ph = get_phase(&Da, 1, ha, pa) * get_phase(&Db, 1, hb, pb);
+              priv[tid][p] += -ph * ci * cj;
+            }
+          }
+        }
+      }
+    }
+  }
+  for (int p = 0; p < d.nsp; p++) {
+    double v = 0.0;
+    for (int t = 0; t < nth; t++) v += priv[t][p];
+    out[p] = v;
+  }
+}
+
 // create_1rdm for k state vectors (columns of C): D receives k n x n column-major blocks.
 int rc_rdm1(void* h, const double* C, int64_t ldc, int k, int p, int s, double* D) {
   auto* d = static_cast<DetSet*>(h);
@@ -954,6 +1009,17 @@ int rc_rdm1(void* h, const double* C, int64_t ldc, int k, int p, int s, double* 
   }
   const size_t nn = (size_t)d->n_orb[p] * d->n_orb[p];
   for (int v = 0; v < k; v++) rdm1(*d, p, s, C + (size_t)v * ldc, D + v * nn);
+  return 0;
+}
+
+// evaluate_s2 for k state vectors: S2[state + species * k] (k x nspecies, column-major).
+int rc_s2(void* h, const double* C, int64_t ldc, int k, double* S2) {
+  auto* d = static_cast<DetSet*>(h);
+  double tmp[2];
+  for (int v = 0; v < k; v++) {
+    s2(*d, C + (size_t)v * ldc, tmp);
+    for (int p = 0; p < d->nsp; p++) S2[v + (size_t)p * k] = tmp[p];
+  }
   return 0;
 }
 

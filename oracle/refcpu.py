@@ -61,6 +61,7 @@ def lib():
         L.rc_sigma.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64]
         L.rc_sigma_slow.argtypes = L.rc_sigma.argtypes
         L.rc_rdm1.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p]
+        L.rc_s2.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p]
         L.rc_rdm1_sample.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_int, ct.c_void_p, ct.c_int64]
         L.rc_sigma_sample.argtypes = L.rc_sigma.argtypes + [ct.c_int64]
         L.rc_sigma_rows.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int64, ct.c_int, ct.c_void_p, ct.c_int64,
@@ -223,6 +224,13 @@ class OracleCI:
             raise ValueError(lib().rc_last_error().decode())
         D = D.transpose(0, 2, 1)
         return D[0] if one else D
+
+    def s2(self, C):
+        """evaluate_s2 (s2.cpp:5-130): <C|S^2|C> per state (rows) and species (columns)."""
+        C = np.asfortranarray(np.atleast_2d(C.T).T, dtype=np.float64)
+        out = np.zeros((self.prob.nspecies, C.shape[1]))
+        lib().rc_s2(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(out))
+        return out.T.copy()
 
     def rdm1_sample(self, c, part, spin, stride):
         """create_1rdm over determinants i = 0 mod stride only (a timing sample)."""

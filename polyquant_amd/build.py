@@ -19,7 +19,7 @@ OUT = os.path.join(HERE, "libpqgpu.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["pqg_ctx.cpp", "pqg_ci.hip", "pqg_tensor.hip", "pqg_linkgemm.hip", "pqg_davidson.hip", "pqg_ao2mo.hip", "pqg_rdm.hip",
-           "pqg_serve.cpp"]
+           "pqg_spin.hip", "pqg_serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PQG_ARCH", "gfx950")

@@ -187,6 +187,20 @@ class DetSet:
         D = D.transpose(0, 2, 1)  # column-major blocks -> D[k][row][col]
         return D[0] if one else D
 
+    def evaluate_s2(self, C: np.ndarray) -> np.ndarray:
+        """evaluate_s2 (src/ci/detset/s2.cpp:5-130): <C|S^2|C> per state (rows) and species
+        (columns) for the columns of C (host, reference order).  Computed on the GPU."""
+        C = np.asfortranarray(np.atleast_2d(np.asarray(C, dtype=np.float64).T).T)
+        out = np.zeros((self.prob.nspecies, C.shape[1]))
+        check(load().pqg_ci_s2(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(out), 0))
+        return out.T.copy()
+
+    def evaluate_s2_device(self, C_ptr: int, ldc: int, k: int) -> np.ndarray:
+        """evaluate_s2 of k device-resident vectors; the (k, nspecies) result on the host."""
+        out = np.zeros((self.prob.nspecies, k))
+        check(load().pqg_ci_s2(self.h, ct.c_void_p(C_ptr), ldc, k, _ptr(out), 1))
+        return out.T.copy()
+
     def create_1rdm_device(self, C_ptr: int, ldc: int, k: int, part: int, spin: int, D_ptr: int):
         """create_1rdm on device buffers: D receives k column-major n_orb x n_orb blocks."""
         check(load().pqg_ci_rdm1(self.h, ct.c_void_p(C_ptr), ldc, k, part, spin, ct.c_void_p(D_ptr), 1))
@@ -259,8 +273,16 @@ class EPCI:
         self.report = rep
         return self.energies
 
+    def calculate_S_squared(self):
+        """POLYQUANT_EPCI::calculate_S_squared (src/ci/epci.cpp:317-332): <S^2> of every state
+        (rows) and species (columns)."""
+        if self.C_ci is None:
+            raise RuntimeError("run() first")
+        self.S_squared = self.detset.evaluate_s2(self.C_ci)
+        return self.S_squared
+
     def calculate_NOs(self, states=None, orbitals=None):
-        """POLYQUANT_EPCI::calculate_NOs (src/ci/epci.cpp:200-311) without frozen core: the
+        """POLYQUANT_EPCI::calculate_NOs (src/ci/epci.cpp:200-315) without frozen core: the
         1-RDM of every (species, spin) for each listed state on the GPU, then diag_dm_helper
         (epci.cpp:131-146) on the host: occupations in descending order and the natural orbitals,
         in the MO basis or, when orbitals[part][spin] (MO coefficients) is given, in its basis.

@@ -114,3 +114,36 @@ def oracle_dets(oc):
     strs = [oc.strings(p, s) for p in range(prob.nspecies) for s in range(2)]
     U = oc.unfold_all()
     return [tuple(int(strs[k][U[i, k]]) for k in range(len(strs))) for i in range(oc.N)]
+
+
+def s_squared(prob, dets, species, c):
+    """<c| S^2 |c> of one species, S^2 = S- S+ + Sz (Sz + 1), with S+ = sum_i a+_{i alpha} a_{i beta}
+    acting on the species' combined spin-orbital string (alpha orbitals 0..n-1, beta n..2n-1)."""
+    n = prob.species[species].n_orb
+    sa, sb = 2 * species, 2 * species + 1
+    mask = (1 << n) - 1
+    index = {d: i for i, d in enumerate(dets)}
+    total = 0.0
+    for j, det in enumerate(dets):
+        if c[j] == 0.0:
+            continue
+        comb = det[sa] | (det[sb] << n)
+        nal, nbe = bin(det[sa]).count("1"), bin(det[sb]).count("1")
+        sz = 0.5 * (nal - nbe)
+        total += c[j] * c[j] * sz * (sz + 1.0)
+        for i in range(n):
+            r1 = apply_E(comb, i, n + i)  # S+
+            if r1 is None:
+                continue
+            g1, s1 = r1
+            for k in range(n):
+                r2 = apply_E(s1, n + k, k)  # S-
+                if r2 is None:
+                    continue
+                g2, s2 = r2
+                nd = list(det)
+                nd[sa], nd[sb] = s2 & mask, s2 >> n
+                t = index.get(tuple(nd))
+                if t is not None:
+                    total += g1 * g2 * c[t] * c[j]
+    return total

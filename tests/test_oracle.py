@@ -341,3 +341,37 @@ def test_rdm1_single_determinant_is_occupation_diagonal():
             bits = int(oc.strings(0, s)[U[addr, s]])
             D = oc.rdm1(c, 0, s)
             assert np.array_equal(D, np.diag([(bits >> o) & 1 for o in range(6)]).astype(float))
+
+
+@pytest.mark.parametrize("kind", RDM_KINDS)
+def test_s2_matches_bruteforce(kind):
+    """evaluate_s2 (s2.cpp:5-130) against <C|S- S+ + Sz(Sz+1)|C> built from spin-flip operators on
+    the combined alpha|beta occupation string, for random vectors and the ground state."""
+    prob = _rdm_problem(kind)
+    oc = refcpu.OracleCI(prob)
+    dets = bruteforce.oracle_dets(oc)
+    rng = np.random.default_rng(17)
+    C = rng.standard_normal((oc.N, 2))
+    C /= np.linalg.norm(C, axis=0)
+    w, V = np.linalg.eigh(oc.dense())
+    C = np.column_stack([C, V[:, 0]])
+    S2 = oc.s2(C)
+    assert S2.shape == (3, prob.nspecies)
+    for v in range(3):
+        for p in range(prob.nspecies):
+            ref = bruteforce.s_squared(prob, dets, p, C[:, v])
+            assert abs(S2[v, p] - ref) < 1e-12, (kind, v, p, S2[v, p], ref)
+
+
+def test_s2_of_eigenstates_is_s_times_s_plus_one():
+    """Restricted full CI commutes with S^2: every eigenvector has <S^2> = S(S+1), and the
+    spectrum of 5 orbitals 2a2b holds singlets, triplets and quintets."""
+    prob = wl.small_problem(48, 5, 2, 2)
+    oc = refcpu.OracleCI(prob)
+    w, V = np.linalg.eigh(oc.dense())
+    S2 = oc.s2(V)[:, 0]
+    allowed = np.array([0.0, 2.0, 6.0])
+    dist = np.min(np.abs(S2[:, None] - allowed[None, :]), axis=1)
+    assert np.max(dist) < 1e-9
+    for a in allowed:
+        assert np.any(np.abs(S2 - a) < 1e-9)
