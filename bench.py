@@ -614,6 +614,8 @@ def main():
     ap.add_argument("--serve-save", default=None, help="write the leg's per-request results (JSON)")
     ap.add_argument("--serve-combine", nargs=2, default=None, metavar=("SEQ", "SRV"),
                     help="print the serving report from saved legs (SEQ: comma-separated sequential parts)")
+    ap.add_argument("--dav-nvec", type=int, default=16, help="num_subspace_vec of the default workload")
+    ap.add_argument("--dav-nmax", type=int, default=64, help="largest search space of the default workload")
     ap.add_argument("--fci-nvec", type=int, default=16)
     ap.add_argument("--fci-nmax", type=int, default=48)
     ap.add_argument("--fci-maxit", type=int, default=400)
@@ -681,7 +683,7 @@ def main():
 
     # num_subspace_vec = 16 and a 64-vector search space (the reference's knobs,
     # epci.cpp:589-608; its defaults 5 / max(10, 10*nstates) need ~2.5x more iterations here)
-    nvec_init, nvec_max = 16, 64
+    nvec_init, nvec_max = args.dav_nvec, args.dav_nmax
 
     def solve(precision):
         return ds.davidson(nev, nvec_init=nvec_init, nvec_max=nvec_max, tol=1e-6, precision=precision,
