@@ -314,8 +314,13 @@ static void launch_group(const GroupArgs& a, int k, hipStream_t st) {
 
 static void rdm1_grouped(pqg_ci* ci, Slot& sl, const double* X, int64_t ldx, int k, double* D, hipStream_t st) {
   const int64_t N = ci->n_int, R = N / sl.nstr;
+  // rho chunks: enough workgroups for the chip, and up to 4 chunks of at least 3200 columns
+  // (measured at 16 orbitals 8a8b: 1 chunk 5.56 ms, 2: 5.24, 4: 5.12, 8: 8.61 -- beyond 4 the
+  // per-workgroup reduction of the m(m+1)/2 entries dominates)
   int64_t nchunk = std::max<int64_t>(1, (4096 + sl.ngroup * k - 1) / (sl.ngroup * k));
+  nchunk = std::max<int64_t>(nchunk, std::min<int64_t>(4, R / 3200));
   nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, R / 2048));
+  if (const char* e = std::getenv("PQG_RDM_NCHUNK")) nchunk = std::max<int64_t>(1, std::min<int64_t>(std::atoll(e), R));
   const int64_t chunk = (R + nchunk - 1) / nchunk;
   nchunk = (R + chunk - 1) / chunk;
   const int m = sl.gm;
