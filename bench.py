@@ -488,8 +488,9 @@ def bench_rdm(args):
     ctx.profile(False)
     s2_launches, s2_ms, _ = ctx.profile_read("s2")
     s2_launch_s = s2_ms * 1e-3 / s2_launches
-    s2 = {"ms_per_call_4_vectors": round(s2_dt * 1e3, 3), "k_s2_ms": round(s2_launch_s * 1e3, 3),
-          "k_s2_algorithmic_GBps": round(N * 8 * k / s2_launch_s / 1e9, 1),
+    s2 = {"ms_per_call_4_vectors": round(s2_dt * 1e3, 3), "kernel": "k_s2p",
+          "kernel_ms": round(s2_launch_s * 1e3, 3),
+          "kernel_algorithmic_GBps": round(N * 8 * k / s2_launch_s / 1e9, 1),
           "values": [round(float(x), 6) for x in s2v[:, 0]]}
     log(f"[rdm] S^2: {s2}")
     cpu = None

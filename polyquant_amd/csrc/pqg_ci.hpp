@@ -129,6 +129,8 @@ void to_reference(pqg_ci* ci, const double* in, int64_t ldi, double* ref, int64_
 // zero entries outside a truncated space (no-op for full CI)
 template <typename T>
 void mask_invalid(pqg_ci* ci, T* v, int64_t ld, int k, hipStream_t st);
+// moves string set t of k internal-order vectors to the outermost index (X[n_t][A][B])
+void slot_outer(pqg_ci* ci, int t, const double* C, int64_t ldc, double* out, int k, hipStream_t st);
 // bytes of device memory a CI of this description needs (admission control)
 uint64_t estimate_ci_bytes(const pqg_ci_desc* d, const pqg_dav_params* p);
 pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* desc, hipStream_t stream);

@@ -362,6 +362,20 @@ static void rdm1_grouped(pqg_ci* ci, Slot& sl, const double* X, int64_t ldx, int
   PQG_HIP(hipStreamSynchronize(st));
 }
 
+// Moves set t of k internal-order vectors (leading dimension ldc) to the outermost index:
+// out (leading dimension N) = X[n_t][A][B].
+void slot_outer(pqg_ci* ci, int t, const double* C, int64_t ldc, double* out, int k, hipStream_t st) {
+  const int64_t N = ci->n_int, n_t = ci->slot[t].nstr, B = ci->stride[t], A = N / (n_t * B);
+  if (B == 1 && (A + 63) / 64 <= 65535) {
+    dim3 g((unsigned)((n_t + 63) / 64), (unsigned)((A + 63) / 64), (unsigned)k);
+    k_slot_outer_t<<<g, 256, 0, st>>>(C, ldc, out, A, n_t, N);
+  } else {
+    const int64_t blocks = std::min<int64_t>((N + 255) / 256, 1 << 16);
+    k_slot_outer<<<dim3((unsigned)blocks, 1, (unsigned)k), 256, 0, st>>>(C, ldc, out, A, n_t, B, N);
+  }
+  PQG_HIP(hipGetLastError());
+}
+
 // Full CI with the grouped form: the vectors stay in reference order (no boundary permutation).
 static bool rdm1_reference_order(pqg_ci* ci, int t) { return ci->fci && build_groups(ci, ci->slot[t]); }
 

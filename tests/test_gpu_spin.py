@@ -51,6 +51,23 @@ def test_s2_matches_oracle(name, ctx):
     assert np.max(np.abs(got - ref)) < 1e-12, (name, got, ref)
 
 
+@pytest.mark.parametrize("name", ["fci_open_shell_8o_4a2b", "cisd_h2o_layout_7o_5a5b", "ep_fci_5e_2a2b_4p_2a1b",
+                                  "ep_truncated_psh_layout", "fci_12o_6a5b"])
+def test_s2_raising_form_equals_direct_form(name, ctx, monkeypatch):
+    """|S+ C|^2 + Sz(Sz+1)|C|^2 over beta cores equals the per-determinant spin-flip sum."""
+    from polyquant_amd.ci import DetSet
+
+    prob = CASES[name]()
+    print(f"[gpu] case s2 raising vs direct {name}", flush=True)
+    ds = DetSet(prob, ctx)
+    C = np.random.default_rng(18).standard_normal((ds.N_dets, 2))
+    C /= np.linalg.norm(C, axis=0)
+    a = ds.evaluate_s2(C)
+    monkeypatch.setenv("PQG_S2_DIRECT", "1")
+    b = ds.evaluate_s2(C)
+    assert np.max(np.abs(a - b)) < 1e-12, (name, a, b)
+
+
 def test_s2_water_fci_matches_oracle(ctx):
     from oracle import refcpu
     from polyquant_amd.ci import DetSet
