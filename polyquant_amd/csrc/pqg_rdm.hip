@@ -220,10 +220,31 @@ __global__ __launch_bounds__(256) void k_rdm_group(GroupArgs a) {
 #pragma unroll
       for (int j = i; j < MG; j++) acc[q++] += y[i] * y[j];
   }
+  if constexpr (T <= 64) {
+    // reduce-scatter over the wave: at offset o, a lane keeps the half of its entries selected
+    // by its bit o and adds its partner's copy of them; after six steps lane q holds the wave's
+    // sum of entry q (entries T..63 are zero padding)
+    double v[64];
 #pragma unroll
-  for (int q = 0; q < T; q++) {
-    const double t = wave_sum(acc[q]);
-    if (lane == 0) red[wave][q] = t;
+    for (int q = 0; q < 64; q++) v[q] = q < T ? acc[q] : 0.0;
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const int o = 32 >> s, len = 64 >> s, h = len / 2;
+      const bool hi = (lane & o) != 0;
+#pragma unroll
+      for (int i = 0; i < h; i++) {
+        const double send = hi ? v[i] : v[i + h];
+        const double keep = hi ? v[i + h] : v[i];
+        v[i] = keep + __shfl_xor(send, o, 64);
+      }
+    }
+    if (lane < T) red[wave][lane] = v[0];
+  } else {
+#pragma unroll
+    for (int q = 0; q < T; q++) {
+      const double t = wave_sum(acc[q]);
+      if (lane == 0) red[wave][q] = t;
+    }
   }
   __syncthreads();
   // entry (i, j) of the MG-wide packing -> (i, j) of the m-wide packing
@@ -314,11 +335,12 @@ static void launch_group(const GroupArgs& a, int k, hipStream_t st) {
 
 static void rdm1_grouped(pqg_ci* ci, Slot& sl, const double* X, int64_t ldx, int k, double* D, hipStream_t st) {
   const int64_t N = ci->n_int, R = N / sl.nstr;
-  // rho chunks: enough workgroups for the chip, and up to 4 chunks of at least 3200 columns
-  // (measured at 16 orbitals 8a8b: 1 chunk 5.56 ms, 2: 5.24, 4: 5.12, 8: 8.61 -- beyond 4 the
-  // per-workgroup reduction of the m(m+1)/2 entries dominates)
+  // rho chunks: enough workgroups for the chip, and up to 8 chunks of at least 1600 columns
+  // (measured at 16 orbitals 8a8b, 4 vectors: 1 chunk 5.60 ms, 4: 4.99, 8: 4.77, 12: 4.95,
+  // 16: 5.71 -- more chunks put more of the repeated row reads in the MALL until the per-
+  // workgroup reduction of the m(m+1)/2 entries dominates)
   int64_t nchunk = std::max<int64_t>(1, (4096 + sl.ngroup * k - 1) / (sl.ngroup * k));
-  nchunk = std::max<int64_t>(nchunk, std::min<int64_t>(4, R / 3200));
+  nchunk = std::max<int64_t>(nchunk, std::min<int64_t>(8, R / 1600));
   nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, R / 2048));
   if (const char* e = std::getenv("PQG_RDM_NCHUNK")) nchunk = std::max<int64_t>(1, std::min<int64_t>(std::atoll(e), R));
   const int64_t chunk = (R + nchunk - 1) / nchunk;
