@@ -629,6 +629,36 @@ __global__ void k_int2ref(PermArgs p, const double* __restrict__ in, int64_t ldi
   ref[k * ldr + a] = in[k * ldi + ref_to_int(p, a)];
 }
 
+// Full CI with two string sets: the reference address is i_a * n_b + i_b and the internal one
+// r2i_a[i_a] * n_b + r2i_b[i_b], so a row moves as a whole and is permuted within itself.  One
+// workgroup per (row, vector) stages the row in LDS, so both the read and the write are
+// contiguous (the element-wise kernels above scatter one side).  A row of up to 20480 strings
+// takes up to 160 KB of LDS, one workgroup per CU, so the workgroup has 1024 threads to keep
+// enough loads in flight.
+__global__ __launch_bounds__(1024) void k_ref2int_rows(const int32_t* __restrict__ r2i_a, const int32_t* __restrict__ r2i_b,
+                                                      int nb, const double* __restrict__ ref, int64_t ldr,
+                                                      double* __restrict__ in, int64_t ldi) {
+  extern __shared__ double row[];
+  const int64_t i = blockIdx.x;
+  const double* src = ref + (int64_t)blockIdx.y * ldr + i * nb;
+  for (int j = threadIdx.x; j < nb; j += 1024) row[r2i_b[j]] = src[j];
+  __syncthreads();
+  double* dst = in + (int64_t)blockIdx.y * ldi + (int64_t)r2i_a[i] * nb;
+  for (int j = threadIdx.x; j < nb; j += 1024) dst[j] = row[j];
+}
+
+__global__ __launch_bounds__(1024) void k_int2ref_rows(const int32_t* __restrict__ r2i_a, const int32_t* __restrict__ r2i_b,
+                                                      int nb, const double* __restrict__ in, int64_t ldi,
+                                                      double* __restrict__ ref, int64_t ldr) {
+  extern __shared__ double row[];
+  const int64_t i = blockIdx.x;
+  const double* src = in + (int64_t)blockIdx.y * ldi + (int64_t)r2i_a[i] * nb;
+  for (int j = threadIdx.x; j < nb; j += 1024) row[j] = src[j];
+  __syncthreads();
+  double* dst = ref + (int64_t)blockIdx.y * ldr + i * nb;
+  for (int j = threadIdx.x; j < nb; j += 1024) dst[j] = row[r2i_b[j]];
+}
+
 template <typename T>
 __global__ void k_mask(const uint8_t* __restrict__ valid, T* __restrict__ v, int64_t ld, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -875,7 +905,27 @@ static PermArgs perm_args(const pqg_ci* ci) {
   return p;
 }
 
+// the row-staged permutation applies to full CI with two string sets whose beta row fits LDS
+static bool rows_permutable(const pqg_ci* ci) {
+  const int nb = ci->slot[1].nstr;
+  return ci->fci && ci->nslot == 2 && nb >= 256 && (size_t)nb * 8 <= 160 * 1024 && ci->slot[0].nstr < (1 << 30);
+}
+
+static void set_row_lds(const void* kern, size_t bytes) {
+  if (bytes > 64 * 1024)
+    PQG_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 void to_internal(pqg_ci* ci, const double* ref, int64_t ldr, double* in, int64_t ldi, int k, hipStream_t st) {
+  if (rows_permutable(ci)) {
+    const int nb = ci->slot[1].nstr;
+    const size_t lds = (size_t)nb * 8;
+    set_row_lds(reinterpret_cast<const void*>(k_ref2int_rows), lds);
+    k_ref2int_rows<<<dim3((unsigned)ci->slot[0].nstr, (unsigned)k), 1024, lds, st>>>(
+        ci->slot[0].d_ref2int.get(), ci->slot[1].d_ref2int.get(), nb, ref, ldr, in, ldi);
+    PQG_HIP(hipGetLastError());
+    return;
+  }
   if (!ci->fci)
     for (int j = 0; j < k; j++) PQG_HIP(hipMemsetAsync(in + j * ldi, 0, ci->n_int * 8, st));
   dim3 g(grid1(ci->n_ref, 256), k);
@@ -884,6 +934,15 @@ void to_internal(pqg_ci* ci, const double* ref, int64_t ldr, double* in, int64_t
 }
 
 void to_reference(pqg_ci* ci, const double* in, int64_t ldi, double* ref, int64_t ldr, int k, hipStream_t st) {
+  if (rows_permutable(ci)) {
+    const int nb = ci->slot[1].nstr;
+    const size_t lds = (size_t)nb * 8;
+    set_row_lds(reinterpret_cast<const void*>(k_int2ref_rows), lds);
+    k_int2ref_rows<<<dim3((unsigned)ci->slot[0].nstr, (unsigned)k), 1024, lds, st>>>(
+        ci->slot[0].d_ref2int.get(), ci->slot[1].d_ref2int.get(), nb, in, ldi, ref, ldr);
+    PQG_HIP(hipGetLastError());
+    return;
+  }
   dim3 g(grid1(ci->n_ref, 256), k);
   k_int2ref<<<g, 256, 0, st>>>(perm_args(ci), in, ldi, ref, ldr);
   PQG_HIP(hipGetLastError());
