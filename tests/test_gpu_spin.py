@@ -92,3 +92,33 @@ def test_s2_of_davidson_roots_is_spin_pure(ctx):
     S2 = ci.calculate_S_squared()[:, 0]
     allowed = np.array([0.0, 2.0, 6.0, 12.0])
     assert np.max(np.min(np.abs(S2[:, None] - allowed[None, :]), axis=1)) < 1e-8, S2
+
+
+def test_s2_full_size_16o_8a8b(ctx, monkeypatch):
+    """165,636,900 determinants: the spin-raising and direct forms agree on a random vector, and a
+    single determinant gives its number of alpha-only orbitals (the off-diagonal terms vanish)."""
+    from polyquant_amd.ci import DetSet
+
+    prob = wl.config3(1)
+    print("[gpu] case s2 full size 16o 8a8b", flush=True)
+    ds = DetSet(prob, ctx)
+    N = ds.N_dets
+    C = np.random.default_rng(19).standard_normal(N)
+    C /= np.linalg.norm(C)
+    a = ds.evaluate_s2(C)[0, 0]
+    # determinants: (alpha index, beta index) = (0, 0) is the closed-shell string pair; the last
+    # alpha string (occupying the 8 highest orbitals) with the first beta string has 8 + 8 unpaired
+    strs_a, strs_b = ds.strings(0, 0), ds.strings(0, 1)
+    nb = len(strs_b)
+    picks = [(0, 0), (len(strs_a) - 1, 0), (17, 5)]
+    E = np.zeros((N, len(picks)))
+    expect = []
+    for q, (ia, ib) in enumerate(picks):
+        E[ia * nb + ib, q] = 1.0
+        A, B = int(strs_a[ia]), int(strs_b[ib])
+        expect.append(bin(A & ~B).count("1"))
+    got = ds.evaluate_s2(E)[:, 0]
+    assert np.max(np.abs(got - np.array(expect, dtype=float))) < 1e-12, (got, expect)
+    monkeypatch.setenv("PQG_S2_DIRECT", "1")
+    b = ds.evaluate_s2(C)[0, 0]
+    assert abs(a - b) < 1e-11, (a, b)
