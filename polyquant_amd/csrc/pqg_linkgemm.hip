@@ -37,7 +37,8 @@
 
 namespace pqg {
 
-constexpr int LG_NT = 32;  // columns per tile
+constexpr int LG_NT = 32;  // columns per tile (FP64; FP32 takes LG_NT32 when its LDS allows)
+constexpr int LG_NT32 = 64;
 constexpr int LG_MAXL = 64;     // links per row tuple (four 16-row M blocks)
 constexpr int LG_MAXIN = 160;   // in-row rows
 constexpr int LG_MAXBO = 128;   // off-diagonal pairs of the column set (those go through the map)
@@ -119,12 +120,12 @@ __device__ __forceinline__ int offpair_hi(int k) {
 // columns of one); with KSPLIT = 2 waves w and w + 4 split the K steps in two halves whose
 // partial sums meet in LDS.  The A fragments of a row tuple stay in registers for all of its
 // tiles; the global loads of the next tile are issued before the MFMAs of the current one.
-template <typename T, int NSPLIT, int KSPLIT, int KQH>
+template <typename T, int NSPLIT, int KSPLIT, int KQH, int NT_>
 __global__ __launch_bounds__(256 * KSPLIT, 3 - KSPLIT) void k_lg_u(const LgArgs<T> a) {
   constexpr int LG_THREADS = 256 * KSPLIT;
   // prefetch registers per thread: in-row map entries and excited-row values of one tile
-  constexpr int LG_TBMAX = LG_MAXBO * LG_NT / LG_THREADS, LG_TAMAX = LG_MAXL * LG_NT / LG_THREADS;
-  constexpr int NT = LG_NT, NF = NT / 16, NFW = NF / NSPLIT;
+  constexpr int NT = NT_, NF = NT / 16, NFW = NF / NSPLIT;
+  constexpr int LG_TBMAX = LG_MAXBO * NT / LG_THREADS, LG_TAMAX = LG_MAXL * NT / LG_THREADS;
   constexpr int NTP = NT + 1;
   constexpr int KP = 4 * KSPLIT * KQH;  // rows of the LDS tile; rows K..KP-1 stay zero
   typedef typename Mfma<T>::acc_t acc_t;
@@ -436,15 +437,15 @@ static LgPass lg_pass_desc(const pqg_ci* ci, int pass) {
   return d;
 }
 
-static size_t lg_lds_bytes(const LgPass& d, int n_c, int kqh, int ksplit, size_t w) {
+static size_t lg_lds_bytes(const LgPass& d, int n_c, int kqh, int ksplit, size_t w, int nt = LG_NT) {
   auto r16 = [](size_t x) { return (x + 15) / 16 * 16; };
-  const size_t ntp = LG_NT + 1;
+  const size_t ntp = nt + 1;
   size_t off = r16((size_t)d.L * 8);      // links
   off = r16(off + (size_t)d.L * 4);       // hole/particle numbers
   off = r16(off + (size_t)d.nin * 4);     // pair indices
   off = r16(off + (size_t)n_c * w);       // staged row
-  // tile (also holds the 64 x LG_NT partial sums of the second K half)
-  return off + std::max<size_t>((size_t)4 * ksplit * kqh * ntp, (size_t)64 * LG_NT) * w;
+  // tile (also holds the 64 x nt partial sums of the second K half)
+  return off + std::max<size_t>((size_t)4 * ksplit * kqh * ntp, (size_t)64 * nt) * w;
 }
 
 bool link_gemm_eligible(const pqg_ci* ci) {
@@ -541,35 +542,50 @@ void link_gemm_setup(pqg_ci* ci) {
   }
 }
 
-template <typename T, int NSPLIT, int KSPLIT, int KQH>
+template <typename T, int NSPLIT, int KSPLIT, int KQH, int NT>
 static void lg_go(const LgArgs<T>& a, dim3 grid, size_t lds, hipStream_t st) {
-  auto kern = k_lg_u<T, NSPLIT, KSPLIT, KQH>;
+  auto kern = k_lg_u<T, NSPLIT, KSPLIT, KQH, NT>;
   PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   kern<<<grid, 256 * KSPLIT, lds, st>>>(a);
 }
 
+template <typename T, int NSPLIT, int NT>
+static void lg_go_k2(const LgArgs<T>& a, int kqh, dim3 grid, size_t lds, hipStream_t st) {
+  switch (kqh) {
+    case 4: lg_go<T, NSPLIT, 2, 4, NT>(a, grid, lds, st); break;
+    case 8: lg_go<T, NSPLIT, 2, 8, NT>(a, grid, lds, st); break;
+    case 12: lg_go<T, NSPLIT, 2, 12, NT>(a, grid, lds, st); break;
+    case 16: lg_go<T, NSPLIT, 2, 16, NT>(a, grid, lds, st); break;
+    case 20: lg_go<T, NSPLIT, 2, 20, NT>(a, grid, lds, st); break;
+    case 24: lg_go<T, NSPLIT, 2, 24, NT>(a, grid, lds, st); break;
+    default: lg_go<T, NSPLIT, 2, 26, NT>(a, grid, lds, st); break;
+  }
+}
+
 template <typename T, int NSPLIT>
+static void lg_go_k1(const LgArgs<T>& a, int kqh, dim3 grid, size_t lds, hipStream_t st) {
+  switch (kqh) {
+    case 8: lg_go<T, NSPLIT, 1, 8, LG_NT>(a, grid, lds, st); break;
+    case 16: lg_go<T, NSPLIT, 1, 16, LG_NT>(a, grid, lds, st); break;
+    case 24: lg_go<T, NSPLIT, 1, 24, LG_NT>(a, grid, lds, st); break;
+    case 32: lg_go<T, NSPLIT, 1, 32, LG_NT>(a, grid, lds, st); break;
+    case 40: lg_go<T, NSPLIT, 1, 40, LG_NT>(a, grid, lds, st); break;
+    case 48: lg_go<T, NSPLIT, 1, 48, LG_NT>(a, grid, lds, st); break;
+    default: lg_go<T, NSPLIT, 1, 52, LG_NT>(a, grid, lds, st); break;
+  }
+}
+
+// the 64-column tile exists only for the kernel with the K split over two wave groups
+template <typename T, int NSPLIT, int NT>
 static void lg_go_kq(const LgArgs<T>& a, int ksplit, int kqh, dim3 grid, size_t lds, hipStream_t st) {
-  if (ksplit == 2) {
-    switch (kqh) {
-      case 4: lg_go<T, NSPLIT, 2, 4>(a, grid, lds, st); break;
-      case 8: lg_go<T, NSPLIT, 2, 8>(a, grid, lds, st); break;
-      case 12: lg_go<T, NSPLIT, 2, 12>(a, grid, lds, st); break;
-      case 16: lg_go<T, NSPLIT, 2, 16>(a, grid, lds, st); break;
-      case 20: lg_go<T, NSPLIT, 2, 20>(a, grid, lds, st); break;
-      case 24: lg_go<T, NSPLIT, 2, 24>(a, grid, lds, st); break;
-      default: lg_go<T, NSPLIT, 2, 26>(a, grid, lds, st); break;
-    }
+  if constexpr (NT != LG_NT) {
+    PQG_REQUIRE(ksplit == 2, PQG_ERR_UNSUPPORTED, "64-column link-GEMM tile without the K split");
+    lg_go_k2<T, NSPLIT, NT>(a, kqh, grid, lds, st);
   } else {
-    switch (kqh) {
-      case 8: lg_go<T, NSPLIT, 1, 8>(a, grid, lds, st); break;
-      case 16: lg_go<T, NSPLIT, 1, 16>(a, grid, lds, st); break;
-      case 24: lg_go<T, NSPLIT, 1, 24>(a, grid, lds, st); break;
-      case 32: lg_go<T, NSPLIT, 1, 32>(a, grid, lds, st); break;
-      case 40: lg_go<T, NSPLIT, 1, 40>(a, grid, lds, st); break;
-      case 48: lg_go<T, NSPLIT, 1, 48>(a, grid, lds, st); break;
-      default: lg_go<T, NSPLIT, 1, 52>(a, grid, lds, st); break;
-    }
+    if (ksplit == 2)
+      lg_go_k2<T, NSPLIT, NT>(a, kqh, grid, lds, st);
+    else
+      lg_go_k1<T, NSPLIT>(a, kqh, grid, lds, st);
   }
 }
 
@@ -584,13 +600,23 @@ static const float* xblock(pqg_ci* ci, int t, int u, float) { return ci->X32[t][
 
 // U bytes per column of a pass, and the chunk width that keeps U within its budget
 static size_t lg_per_col(const LgPass& d, size_t w) { return (size_t)d.n_r * d.L * w; }
-static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w) {
+static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w, int nt) {
   const size_t per_col = lg_per_col(d, w);
   const size_t want = std::min<size_t>(lg_budget_bytes(ci->n_int, w), per_col * (size_t)n_c);
-  const size_t budget = std::max<size_t>(per_col * LG_NT, want);
+  const size_t budget = std::max<size_t>(per_col * nt, want);
   int cw = (int)std::min<size_t>((size_t)n_c, budget / per_col);
-  if (cw < n_c) cw = std::max(LG_NT, cw / LG_NT * LG_NT);
+  if (cw < n_c) cw = std::max(nt, cw / nt * nt);
   return cw;
+}
+
+// tile width of a pass: 64 columns for the FP32 kernel with the K split over two wave groups
+// when the tile and the staged row fit LDS, else 32 (measured, FP32 sigma of 4 vectors: 16
+// orbitals 8a8b 675 ms with 64 columns, 726 ms with 32; water 6-31G, which takes the 256-thread
+// kernel, 8.3 ms with 64 and 5.1 ms with 32)
+static int lg_nt(const LgPass& d, int n_c, size_t w, int ksplit, int kqh) {
+  if (w != 4 || ksplit != 2) return LG_NT;
+  if (const char* e = std::getenv("PQG_LG_NT")) return std::atoi(e) == 64 ? LG_NT32 : LG_NT;
+  return lg_lds_bytes(d, n_c, kqh, ksplit, w, LG_NT32) <= 160 * 1024 ? LG_NT32 : LG_NT;
 }
 
 // One pass of the link GEMM for one vector; accumulate: add to S or overwrite it.
@@ -641,14 +667,17 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
   a.kbf_off = (int)r16(a.lab_off + (size_t)a.L * 4);
   a.crow_off = (int)r16(a.kbf_off + (size_t)a.nin * 4);
   a.tile_off = (int)r16(a.crow_off + (size_t)a.n_c * sizeof(T));
-  const size_t lds = lg_lds_bytes(d, a.n_c, kqh, ksplit, sizeof(T));
+  const int nt = lg_nt(d, a.n_c, sizeof(T), ksplit, kqh);
+  const size_t lds = lg_lds_bytes(d, a.n_c, kqh, ksplit, sizeof(T), nt);
   // U holds n_r * L * cw values; sized for the larger of the two passes so it grows once
-  const int cw = lg_chunk(ci, d, a.n_c, sizeof(T));
+  const int cw = lg_chunk(ci, d, a.n_c, sizeof(T), nt);
   size_t need = lg_per_col(d, sizeof(T)) * (size_t)cw;
   {
     const LgPass o = lg_pass_desc(ci, 1 - pass);
     const int n_co = ci->slot[o.col].nstr;
-    need = std::max(need, lg_per_col(o, sizeof(T)) * (size_t)lg_chunk(ci, o, n_co, sizeof(T)));
+    const int kqo = lg_kqh_bound((o.nin + o.L + 3) / 4, ksplit);
+    need = std::max(need, lg_per_col(o, sizeof(T)) * (size_t)lg_chunk(ci, o, n_co, sizeof(T),
+                                                                       lg_nt(o, n_co, sizeof(T), ksplit, kqo)));
   }
   grow(lg.U, ci->ctx, need, st);
   a.U = reinterpret_cast<T*>(lg.U.get());
@@ -658,14 +687,24 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
   for (int c0 = 0; c0 < a.n_c; c0 += cw) {
     a.c0 = c0;
     a.cw = std::min(cw, a.n_c - c0);
-    const int tiles = lg_grid(a.cw, LG_NT);
+    const int tiles = lg_grid(a.cw, nt);
     const int gy = std::max(1, std::min(tiles, lg_grid(2048, a.n_r)));
-    a.gcols = lg_grid(tiles, gy) * LG_NT;
+    a.gcols = lg_grid(tiles, gy) * nt;
     dim3 grid((unsigned)a.n_r, (unsigned)lg_grid(a.cw, a.gcols));
     {
       ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "lg_gemm_f64" : "lg_gemm_f32", flops_per_col * a.cw);
-      if (mb <= 2) lg_go_kq<T, 2>(a, ksplit, kqh, grid, lds, st);
-      else lg_go_kq<T, 1>(a, ksplit, kqh, grid, lds, st);
+      if constexpr (sizeof(T) == 4) {
+        if (nt == LG_NT32) {
+          if (mb <= 2) lg_go_kq<T, 2, LG_NT32>(a, ksplit, kqh, grid, lds, st);
+          else lg_go_kq<T, 1, LG_NT32>(a, ksplit, kqh, grid, lds, st);
+        } else {
+          if (mb <= 2) lg_go_kq<T, 2, LG_NT>(a, ksplit, kqh, grid, lds, st);
+          else lg_go_kq<T, 1, LG_NT>(a, ksplit, kqh, grid, lds, st);
+        }
+      } else {
+        if (mb <= 2) lg_go_kq<T, 2, LG_NT>(a, ksplit, kqh, grid, lds, st);
+        else lg_go_kq<T, 1, LG_NT>(a, ksplit, kqh, grid, lds, st);
+      }
       PQG_HIP(hipGetLastError());
     }
     {
