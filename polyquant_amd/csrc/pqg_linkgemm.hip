@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 #include "pqg_ci.hpp"
 #include "pqg_mfma.hpp"
@@ -50,7 +51,7 @@ constexpr int LG_MAXSETS = 3;
 template <typename T>
 struct LgArgs {
   const T* C;          // this vector, n_r x n_c row-major
-  T* U;                // chunk scratch [n_r][L][cw]
+  T* U;                // chunk scratch [n_r][L][ldu]
   int nrow;            // sets in the row tuple
   int64_t rstride[LG_MAXSETS];
   int rns[LG_MAXSETS];
@@ -68,6 +69,7 @@ struct LgArgs {
   int np_c;
   int n_r, n_c, L, K;
   int c0, cw, gcols;
+  int ldu;             // U row stride: cw rounded up to 4 elements, so rows start 16-byte aligned
   int accumulate;      // gather: add to S (1) or overwrite it (0)
   int lab_off, kbf_off, crow_off, tile_off;  // LDS byte offsets (links at 0)
 };
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(256 * KSPLIT, 3 - KSPLIT) void k_lg_u(const LgArgs<
       for (int r = 0; r < 4; r++) {
         const int ep = 16 * mb0 + Mfma<T>::row(r, lane);
         if (ep >= L) continue;
-        T* urow = a.U + ((size_t)J * L + ep) * a.cw;
+        T* urow = a.U + ((size_t)J * L + ep) * a.ldu;
 #pragma unroll
         for (int f = 0; f < NFW; f++) {
           const int col = 16 * (fb + f) + (lane & 15);
@@ -302,8 +304,13 @@ __global__ __launch_bounds__(256 * KSPLIT, 3 - KSPLIT) void k_lg_u(const LgArgs<
 }
 
 // sigma(I, c0 + j) (+)= sum_e s(I,e) U[J(I,e)][rev(I,e)][j] for the columns j of a chunk
+// sigma(I, c0 + m) (+)= sum_e s(I, e) U[J(I, e)][rev(I, e)][m]: each thread takes CPT runs of
+// VEC = 16 / sizeof(T) consecutive columns, read as one 16-byte vector per U row (rows are
+// 16-byte aligned); the sum over links runs in a fixed order (bit-reproducible).
 template <typename T, int CPT>
 __global__ __launch_bounds__(256) void k_lg_gather(const LgArgs<T> a, T* __restrict__ S) {
+  constexpr int VEC = 16 / sizeof(T);
+  typedef typename std::conditional<sizeof(T) == 8, double2, float4>::type vec_t;
   __shared__ int2 lq[LG_MAXL];
   __shared__ int lab[LG_MAXL];
   const int64_t I = blockIdx.x;
@@ -311,28 +318,34 @@ __global__ __launch_bounds__(256) void k_lg_gather(const LgArgs<T> a, T* __restr
   __syncthreads();
   const int L = a.L, cw = a.cw;
   const T* __restrict__ U = a.U;
-  const int j0 = blockIdx.y * 256 * CPT + threadIdx.x;
-  T acc[CPT];
+  const int v0 = blockIdx.y * 256 * CPT + threadIdx.x;  // first vector index of this thread
+  T acc[CPT][VEC];
 #pragma unroll
-  for (int j = 0; j < CPT; j++) acc[j] = T(0);
+  for (int j = 0; j < CPT; j++)
+#pragma unroll
+    for (int q = 0; q < VEC; q++) acc[j][q] = T(0);
   for (int e = 0; e < L; e++) {
     const int2 l = lq[e];
-    const T* src = U + ((size_t)(l.x & 0x7fffffff) * L + (l.y & 0xffff)) * cw;
+    const T* src = U + ((size_t)(l.x & 0x7fffffff) * L + (l.y & 0xffff)) * a.ldu;
     const bool neg = l.x < 0;
 #pragma unroll
     for (int j = 0; j < CPT; j++) {
-      const int col = j0 + 256 * j;
-      if (col < cw) {
-        const T u = src[col];
-        acc[j] += neg ? -u : u;
+      const int col = (v0 + 256 * j) * VEC;
+      if (col < cw) {  // the vector may run past cw into the row padding: those lanes are dropped
+        const vec_t u = *reinterpret_cast<const vec_t*>(src + col);
+        const T* uv = reinterpret_cast<const T*>(&u);
+#pragma unroll
+        for (int q = 0; q < VEC; q++) acc[j][q] += neg ? -uv[q] : uv[q];
       }
     }
   }
   T* dst = S + (size_t)I * a.n_c + a.c0;
 #pragma unroll
   for (int j = 0; j < CPT; j++) {
-    const int col = j0 + 256 * j;
-    if (col < cw) dst[col] = a.accumulate ? dst[col] + acc[j] : acc[j];
+    const int col = (v0 + 256 * j) * VEC;
+#pragma unroll
+    for (int q = 0; q < VEC; q++)
+      if (col + q < cw) dst[col + q] = a.accumulate ? dst[col + q] + acc[j][q] : acc[j][q];
   }
 }
 
@@ -669,15 +682,16 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
   a.tile_off = (int)r16(a.crow_off + (size_t)a.n_c * sizeof(T));
   const int nt = lg_nt(d, a.n_c, sizeof(T), ksplit, kqh);
   const size_t lds = lg_lds_bytes(d, a.n_c, kqh, ksplit, sizeof(T), nt);
-  // U holds n_r * L * cw values; sized for the larger of the two passes so it grows once
+  // U holds n_r * L * ldu values (ldu = cw rounded up to 4); sized for the larger of the two
+  // passes so it grows once
   const int cw = lg_chunk(ci, d, a.n_c, sizeof(T), nt);
-  size_t need = lg_per_col(d, sizeof(T)) * (size_t)cw;
+  size_t need = lg_per_col(d, sizeof(T)) * (size_t)(cw + 3);
   {
     const LgPass o = lg_pass_desc(ci, 1 - pass);
     const int n_co = ci->slot[o.col].nstr;
     const int kqo = lg_kqh_bound((o.nin + o.L + 3) / 4, ksplit);
-    need = std::max(need, lg_per_col(o, sizeof(T)) * (size_t)lg_chunk(ci, o, n_co, sizeof(T),
-                                                                       lg_nt(o, n_co, sizeof(T), ksplit, kqo)));
+    need = std::max(need, lg_per_col(o, sizeof(T)) * (size_t)(lg_chunk(ci, o, n_co, sizeof(T),
+                                                                        lg_nt(o, n_co, sizeof(T), ksplit, kqo)) + 3));
   }
   grow(lg.U, ci->ctx, need, st);
   a.U = reinterpret_cast<T*>(lg.U.get());
@@ -687,6 +701,7 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
   for (int c0 = 0; c0 < a.n_c; c0 += cw) {
     a.c0 = c0;
     a.cw = std::min(cw, a.n_c - c0);
+    a.ldu = (a.cw + 3) / 4 * 4;
     const int tiles = lg_grid(a.cw, nt);
     const int gy = std::max(1, std::min(tiles, lg_grid(2048, a.n_r)));
     a.gcols = lg_grid(tiles, gy) * nt;
@@ -711,7 +726,7 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
       ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "lg_gather_f64" : "lg_gather_f32",
                      (int64_t)a.n_r * a.cw * (a.L + 2) * (int64_t)sizeof(T));
       constexpr int CPT = 4;
-      dim3 g2((unsigned)a.n_r, (unsigned)lg_grid(a.cw, 256 * CPT));
+      dim3 g2((unsigned)a.n_r, (unsigned)lg_grid(a.cw, 256 * CPT * (16 / (int)sizeof(T))));
       k_lg_gather<T, CPT><<<g2, 256, 0, st>>>(a, S);
       PQG_HIP(hipGetLastError());
     }
