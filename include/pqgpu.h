@@ -114,6 +114,11 @@ pqg_status pqg_ci_diag(pqg_ci* ci, double* hdiag);
 /* operator()(i, j) -> Slater_Condon (determinant_set.hpp:139, slater_condon.cpp:4-100) for
  * npairs (i, j) address pairs at once.                                                    */
 pqg_status pqg_ci_elements(pqg_ci* ci, const int64_t* i, const int64_t* j, int64_t npairs, double* out);
+/* create_ham + the mirrored dense copy of the exact_diag branch (explicit_ham.cpp:51-80,
+ * epci.cpp:633-643): the full N x N Hamiltonian in reference address order, column-major with
+ * leading dimension ldh, element (i, j) = Slater_Condon(min(i, j), max(i, j)) (diagonal
+ * diagonal_Hii), so H is exactly symmetric.  on_device != 0: H is a device pointer.          */
+pqg_status pqg_ci_dense_h(pqg_ci* ci, double* H, int64_t ldh, int on_device);
 /* operator*(C) (direct.cpp:5-19): S = H C for k vectors, N x k column-major.
  * on_device != 0: C and S are device pointers.  precision: PQG_FP64 or PQG_MIXED (FP32).   */
 pqg_status pqg_ci_sigma(pqg_ci* ci, const double* C, int64_t ldc, int k, double* S, int64_t lds,

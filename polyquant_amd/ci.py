@@ -159,6 +159,15 @@ class DetSet:
         check(load().pqg_ci_elements(self.h, _ptr(i), _ptr(j), len(i), _ptr(out)))
         return out
 
+    def create_ham_dense(self) -> np.ndarray:
+        """create_ham plus the mirrored dense copy that exact_diag diagonalises
+        (src/ci/detset/explicit_ham.cpp:51-80, src/ci/epci.cpp:633-643): N x N, reference
+        address order, built on the GPU."""
+        n = self.N_dets
+        H = np.zeros((n, n), order="F")
+        check(load().pqg_ci_dense_h(self.h, _ptr(H), n, 0))
+        return H
+
     def __call__(self, i: int, j: int) -> float:
         """operator()(i, j) -> Slater_Condon(i, j)."""
         return float(self.elements([i], [j])[0])
@@ -248,10 +257,12 @@ class DetSet:
 
 
 class EPCI:
-    """Solve flow of POLYQUANT_EPCI::run (src/ci/epci.cpp:575-621) with the matrix-free path."""
+    """Solve flow of POLYQUANT_EPCI::run (src/ci/epci.cpp:575-662): the matrix-free Davidson path
+    or, with exact_diag, the dense Hamiltonian diagonalised in full."""
 
     def __init__(self, prob, ctx: Optional[Context] = None, num_states: int = 1, num_subspace_vec: int = 5,
-                 iteration_max: int = 500, convergence_E: float = 1e-6, precision: int = _lib.PQG_FP64):
+                 iteration_max: int = 500, convergence_E: float = 1e-6, precision: int = _lib.PQG_FP64,
+                 exact_diag: bool = False):
         self.prob = prob
         self.detset = DetSet(prob, ctx)
         self.num_states = num_states
@@ -259,12 +270,21 @@ class EPCI:
         self.iteration_max = iteration_max
         self.convergence_E = convergence_E
         self.precision = precision
+        self.exact_diag = exact_diag
         self.constant_shift = prob.e_shift  # E_nuc + frozen-core energy (epci.cpp:584-594)
         self.energies = None
         self.C_ci = None
         self.report = None
 
     def run(self):
+        if self.exact_diag:
+            # epci.cpp:633-662: dense H, all eigenpairs on the host, the lowest num_states kept
+            w, v = np.linalg.eigh(self.detset.create_ham_dense())
+            self.all_energies = w + self.constant_shift
+            self.energies = self.all_energies[:self.num_states].copy()
+            self.C_ci = np.asfortranarray(v[:, :self.num_states])
+            self.report = None
+            return self.energies
         nvec_max = max(2 * self.num_subspace_vec, 10 * self.num_states)
         E, X, rep = self.detset.davidson(self.num_states, self.num_subspace_vec, nvec_max, self.iteration_max,
                                          self.convergence_E, self.precision)

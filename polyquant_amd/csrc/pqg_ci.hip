@@ -683,15 +683,9 @@ struct ElemArgs {
 
 __device__ __forceinline__ int rank_among(uint64_t set, int bit) { return __popcll(set & low_mask(bit)); }
 
-__global__ void k_elements(ElemArgs a, const int64_t* __restrict__ I, const int64_t* __restrict__ J, int64_t npairs,
-                           double* __restrict__ out) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= npairs) return;
-  const int64_t ia = I[q], ja = J[q];
-  if (ia == ja) {
-    out[q] = a.Hd[ia];
-    return;
-  }
+// Slater_Condon element between the determinants at internal addresses ia and ja
+__device__ double element_at(const ElemArgs& a, int64_t ia, int64_t ja) {
+  if (ia == ja) return a.Hd[ia];
   int ii[kSlots], jj[kSlots], deg[kSlots];
   uint64_t bi[kSlots], bj[kSlots];
   int64_t ri = ia, rj = ja;
@@ -749,7 +743,25 @@ __global__ void k_elements(ElemArgs a, const int64_t* __restrict__ I, const int6
       if (odd & 1) v = -v;
     }
   }
-  out[q] = v;
+  return v;
+}
+
+__global__ void k_elements(ElemArgs a, const int64_t* __restrict__ I, const int64_t* __restrict__ J, int64_t npairs,
+                           double* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npairs) return;
+  out[q] = element_at(a, I[q], J[q]);
+}
+
+// Dense H in reference address order (column j = blockIdx.y), from the element of the lower
+// reference address pair (min, max) for both positions, so H is exactly symmetric as the
+// reference's mirrored upper triangle is (epci.cpp:634-643).
+__global__ void k_dense_h(ElemArgs a, PermArgs p, int64_t n, int64_t j0, double* __restrict__ H, int64_t ldh) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = j0 + blockIdx.y;
+  if (i >= n) return;
+  const int64_t lo = min(i, j), hi = max(i, j);
+  H[i + (int64_t)blockIdx.y * ldh] = element_at(a, ref_to_int(p, lo), ref_to_int(p, hi));
 }
 
 // For every string I of a set: its singles (which = 1) or doubles (which = 2) as
@@ -1462,6 +1474,18 @@ uint64_t estimate_ci_bytes(const pqg_ci_desc* d, const pqg_dav_params* p) {
 
 using namespace pqg;
 
+static ElemArgs elem_args(pqg_ci* ci) {
+  ElemArgs a{};
+  a.nslot = ci->nslot;
+  a.tb = tables64(ci);
+  a.Hd = ci->hdiag.get();
+  for (int t = 0; t < ci->nslot; t++) {
+    a.sl[t] = ci->slot[t].dev();
+    a.stride[t] = ci->stride[t];
+  }
+  return a;
+}
+
 extern "C" {
 
 pqg_status pqg_ci_create(pqg_ctx* ctx, const pqg_ci_desc* desc, pqg_ci** out) {
@@ -1513,6 +1537,33 @@ static int64_t ref_to_int_host(const pqg_ci* ci, int64_t a) {
   return ia;
 }
 
+pqg_status pqg_ci_dense_h(pqg_ci* ci, double* H, int64_t ldh, int on_device) {
+  return guard([&] {
+    PQG_REQUIRE(ci && H, PQG_ERR_ARG, "NULL argument");
+    const int64_t n = ci->n_ref;
+    PQG_REQUIRE(ldh >= n, PQG_ERR_ARG, "leading dimension smaller than N");
+    hipStream_t st = ci->stream;
+    const ElemArgs a = elem_args(ci);
+    const PermArgs p = perm_args(ci);
+    // column blocks: at most 65535 columns per launch (grid.y) and, for a host matrix, a
+    // staging block of at most 512 MiB
+    int64_t ncol = std::min<int64_t>(n, 65535);
+    if (!on_device) ncol = std::max<int64_t>(1, std::min<int64_t>(ncol, (int64_t(512) << 20) / (8 * n)));
+    DevBuf<double> stage;
+    if (!on_device) stage.reset(ci->ctx, (size_t)n * ncol);
+    for (int64_t j0 = 0; j0 < n; j0 += ncol) {
+      const int64_t nc = std::min(ncol, n - j0);
+      double* dst = on_device ? H + j0 * ldh : stage.get();
+      const int64_t ld = on_device ? ldh : n;
+      k_dense_h<<<dim3(grid1(n, 256), (unsigned)nc), 256, 0, st>>>(a, p, n, j0, dst, ld);
+      PQG_HIP(hipGetLastError());
+      if (!on_device)
+        PQG_HIP(hipMemcpy2DAsync(H + j0 * ldh, ldh * 8, stage.get(), n * 8, n * 8, nc, hipMemcpyDeviceToHost, st));
+    }
+    PQG_HIP(hipStreamSynchronize(st));
+  });
+}
+
 pqg_status pqg_ci_elements(pqg_ci* ci, const int64_t* i, const int64_t* j, int64_t npairs, double* out) {
   return guard([&] {
     PQG_REQUIRE(ci && i && j && out, PQG_ERR_ARG, "NULL argument");
@@ -1529,14 +1580,7 @@ pqg_status pqg_ci_elements(pqg_ci* ci, const int64_t* i, const int64_t* j, int64
     DevBuf<double> dout(ci->ctx, npairs);
     PQG_HIP(hipMemcpyAsync(di.get(), ii.data(), npairs * 8, hipMemcpyHostToDevice, st));
     PQG_HIP(hipMemcpyAsync(dj.get(), jj.data(), npairs * 8, hipMemcpyHostToDevice, st));
-    ElemArgs a{};
-    a.nslot = ci->nslot;
-    a.tb = tables64(ci);
-    a.Hd = ci->hdiag.get();
-    for (int t = 0; t < ci->nslot; t++) {
-      a.sl[t] = ci->slot[t].dev();
-      a.stride[t] = ci->stride[t];
-    }
+    const ElemArgs a = elem_args(ci);
     k_elements<<<grid1(npairs, 256), 256, 0, st>>>(a, di.get(), dj.get(), npairs, dout.get());
     PQG_HIP(hipGetLastError());
     PQG_HIP(hipMemcpyAsync(out, dout.get(), npairs * 8, hipMemcpyDeviceToHost, st));

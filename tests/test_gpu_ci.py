@@ -246,3 +246,70 @@ def test_davidson_perturbed_start_reaches_ground_state(ctx):
     assert r0.converged == 1 and r1.converged == 1
     assert abs(E0[0] - w[1]) < 1e-8  # the unit-vector start lands on the excited root
     assert abs(E1[0] - w[0]) < 1e-8
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_dense_hamiltonian_matches_oracle(name, ctx):
+    """create_ham + exact_diag's mirrored copy (explicit_ham.cpp:51-80, epci.cpp:633-643): the
+    dense H equals the oracle's Slater_Condon matrix and is exactly symmetric."""
+    prob, ds, oc = _pair(name, ctx)
+    H = ds.create_ham_dense()
+    Hor = oc.dense()
+    scale = max(1.0, np.max(np.abs(Hor)))
+    assert np.max(np.abs(H - Hor)) < 1e-12 * scale
+    assert np.array_equal(H, H.T)
+    iu, ju = np.triu_indices(ds.N_dets)
+    assert np.array_equal(H[iu, ju], ds.elements(iu, ju))
+
+
+def test_dense_hamiltonian_device_pointer_padded(ctx):
+    """pqg_ci_dense_h into a device buffer with ldh > N equals the host call."""
+    import ctypes as ct
+
+    from polyquant_amd import _lib
+    from polyquant_amd.ci import DetSet
+
+    hip = ct.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+    hip.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+    hip.hipFree.argtypes = [ct.c_void_p]
+    D2H = 2
+    prob = PROBLEMS["ep_fci_5e_2a2b_4p_2a1b"]()
+    print("[gpu] case dense_h device ep_fci_5e_2a2b_4p_2a1b", flush=True)
+    ds = DetSet(prob, ctx)
+    n = ds.N_dets
+    ldh = n + 5
+    p = ct.c_void_p()
+    assert hip.hipMalloc(ct.byref(p), ldh * n * 8) == 0
+    try:
+        _lib.check(_lib.load().pqg_ci_dense_h(ds.h, p, ldh, 1))
+        out = np.zeros((ldh, n), order="F")
+        assert hip.hipMemcpy(out.ctypes.data, p, ldh * n * 8, D2H) == 0
+    finally:
+        hip.hipFree(p)
+    assert np.array_equal(out[:n], ds.create_ham_dense())
+    with pytest.raises(_lib.PqgError):
+        _lib.check(_lib.load().pqg_ci_dense_h(ds.h, p, n - 1, 1))
+
+
+def test_exact_diag_water_sto3g_cisd(ctx):
+    """EPCI with exact_diag (epci.cpp:633-662) on H2O/STO-3G CISD: the reference's two lowest
+    energies (calculation_test.cpp:259-260) and the Davidson roots of the same space."""
+    import os
+
+    from polyquant_amd.ci import EPCI
+
+    prob = wl.water_problem("sto-3g", nroots=2, ex_level=(2, 2, 2))
+    print("[gpu] case exact_diag water_sto3g_cisd", flush=True)
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "h2o_sto3g_cisd_ham_elements.npz"))
+    ci = EPCI(prob, ctx, num_states=2, exact_diag=True)
+    E = ci.run()
+    assert abs(E[0] - float(fx["e0"])) < 1e-8
+    assert abs(E[1] - float(fx["e1"])) < 1e-6
+    assert ci.C_ci.shape == (141, 2)
+    dv = EPCI(prob, ctx, num_states=2, num_subspace_vec=20, convergence_E=1e-10)
+    Ed = dv.run()
+    assert np.max(np.abs(E - Ed)) < 1e-8
+    # the eigenvectors feed the same post-processing as the Davidson ones
+    s2 = ci.calculate_S_squared()
+    assert np.max(np.abs(s2[:, 0] - np.round(s2[:, 0]))) < 1e-8
