@@ -119,6 +119,13 @@ pqg_status pqg_ci_elements(pqg_ci* ci, const int64_t* i, const int64_t* j, int64
  * leading dimension ldh, element (i, j) = Slater_Condon(min(i, j), max(i, j)) (diagonal
  * diagonal_Hii), so H is exactly symmetric.  on_device != 0: H is a device pointer.          */
 pqg_status pqg_ci_dense_h(pqg_ci* ci, double* H, int64_t ldh, int on_device);
+/* create_S_sq_minus_expected_S_sq_matrix_singleshot (s2.cpp:156-255) as a dense matrix, and
+ * its square for the second-order penalty (s2.cpp:141-147): P = S^2 - expected * I of species
+ * part (power 1) or P^2 (power 2, summed over the determinants of the space), N x N reference
+ * order, column-major, leading dimension lds, exactly symmetric.  create_S_sq_penalty
+ * (s2.cpp:126-153) adds spin_penalty * this to H.  on_device != 0: S is a device pointer.     */
+pqg_status pqg_ci_dense_s2(pqg_ci* ci, int part, double expected, int power, double* S, int64_t lds,
+                           int on_device);
 /* operator*(C) (direct.cpp:5-19): S = H C for k vectors, N x k column-major.
  * on_device != 0: C and S are device pointers.  precision: PQG_FP64 or PQG_MIXED (FP32).   */
 pqg_status pqg_ci_sigma(pqg_ci* ci, const double* C, int64_t ldc, int k, double* S, int64_t lds,

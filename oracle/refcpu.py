@@ -232,6 +232,40 @@ class OracleCI:
         lib().rc_s2(self.h, _ptr(C), C.shape[0], C.shape[1], _ptr(out))
         return out.T.copy()
 
+    def s2_penalty(self, part, expected=0.0, power=1):
+        """create_S_sq_minus_expected_S_sq_matrix_singleshot (s2.cpp:156-255) as a dense matrix
+        (pure-Python loops, small spaces): for every determinant the diagonal
+        num_a + sz (sz - 1) - expected, and for every pair of an alpha single and a beta single
+        of it whose target lies in the space, -phase_a phase_b when the alpha hole is the beta
+        particle and the beta hole the alpha particle.  power 2: P @ P (s2.cpp:147)."""
+        sa, sb = 2 * part, 2 * part + 1
+        Sa, Sb = self.strings(part, 0), self.strings(part, 1)
+        U = self.unfold_all()
+        index = {tuple(int(x) for x in U[i]): i for i in range(self.N)}
+        P = np.zeros((self.N, self.N))
+        for i in range(self.N):
+            ia, ib = int(U[i, sa]), int(U[i, sb])
+            da, db = int(Sa[ia]), int(Sb[ib])
+            x = da ^ db
+            num_total = bin(x).count("1")
+            num_a = bin(x & da).count("1")
+            sz = num_a - 0.5 * num_total
+            P[i, i] += num_a + sz * (sz - 1.0) - expected
+            for ja in self.excitations(1, part, 0, ia):
+                for jb in self.excitations(1, part, 1, ib):
+                    t = [int(v) for v in U[i]]
+                    t[sa], t[sb] = int(ja), int(jb)
+                    j = index.get(tuple(t))
+                    if j is None:
+                        continue
+                    ea, eb = int(Sa[ja]), int(Sb[jb])
+                    ha, pa = get_holes([da], [ea]), get_parts([da], [ea])
+                    hb, pb = get_holes([db], [eb]), get_parts([db], [eb])
+                    if ha[0] == pb[0] and hb[0] == pa[0]:
+                        phase = get_phase([da], ha, pa) * get_phase([db], hb, pb)
+                        P[i, j] += -phase
+        return P @ P if power == 2 else P
+
     def rdm1_sample(self, c, part, spin, stride):
         """create_1rdm over determinants i = 0 mod stride only (a timing sample)."""
         c = np.ascontiguousarray(c, dtype=np.float64)

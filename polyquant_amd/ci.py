@@ -168,6 +168,15 @@ class DetSet:
         check(load().pqg_ci_dense_h(self.h, _ptr(H), n, 0))
         return H
 
+    def create_s2_penalty_dense(self, part: int, expected: float = 0.0, power: int = 1) -> np.ndarray:
+        """create_S_sq_minus_expected_S_sq_matrix_singleshot (src/ci/detset/s2.cpp:156-255) as a
+        dense N x N matrix P = S^2 - expected I of species part (power 1), or P @ P (power 2, the
+        second-order penalty of s2.cpp:141-147), built on the GPU."""
+        n = self.N_dets
+        S = np.zeros((n, n), order="F")
+        check(load().pqg_ci_dense_s2(self.h, part, float(expected), power, _ptr(S), n, 0))
+        return S
+
     def __call__(self, i: int, j: int) -> float:
         """operator()(i, j) -> Slater_Condon(i, j)."""
         return float(self.elements([i], [j])[0])
@@ -262,7 +271,8 @@ class EPCI:
 
     def __init__(self, prob, ctx: Optional[Context] = None, num_states: int = 1, num_subspace_vec: int = 5,
                  iteration_max: int = 500, convergence_E: float = 1e-6, precision: int = _lib.PQG_FP64,
-                 exact_diag: bool = False):
+                 exact_diag: bool = False, spin_penalty_type: Optional[str] = None,
+                 spin_penalty: Optional[list] = None, expected_S2: Optional[list] = None):
         self.prob = prob
         self.detset = DetSet(prob, ctx)
         self.num_states = num_states
@@ -271,6 +281,16 @@ class EPCI:
         self.convergence_E = convergence_E
         self.precision = precision
         self.exact_diag = exact_diag
+        # ci_keywords spin_penalty_type / spin_penalty / expected_S2 (calculation.cpp:280-312)
+        if spin_penalty_type not in (None, "first_order", "second_order"):
+            raise ValueError("spin_penalty_type can only be first_order or second_order")
+        self.first_order_spin_penalty = spin_penalty_type == "first_order"
+        self.second_order_spin_penalty = spin_penalty_type == "second_order"
+        nsp = len(prob.species)
+        self.spin_penalty = list(spin_penalty) if spin_penalty is not None else [0.0] * nsp
+        self.expected_S2 = list(expected_S2) if expected_S2 is not None else [0.0] * nsp
+        if spin_penalty_type is not None and not exact_diag:
+            raise ValueError("the spin penalty is added to the explicit Hamiltonian: use exact_diag")
         self.constant_shift = prob.e_shift  # E_nuc + frozen-core energy (epci.cpp:584-594)
         self.energies = None
         self.C_ci = None
@@ -278,8 +298,12 @@ class EPCI:
 
     def run(self):
         if self.exact_diag:
-            # epci.cpp:633-662: dense H, all eigenpairs on the host, the lowest num_states kept
-            w, v = np.linalg.eigh(self.detset.create_ham_dense())
+            # epci.cpp:622-662: dense H (+ the spin penalty), all eigenpairs on the host, the
+            # lowest num_states kept
+            H = self.detset.create_ham_dense()
+            if self.first_order_spin_penalty or self.second_order_spin_penalty:
+                self.add_spin_penalty(H)
+            w, v = np.linalg.eigh(H)
             self.all_energies = w + self.constant_shift
             self.energies = self.all_energies[:self.num_states].copy()
             self.C_ci = np.asfortranarray(v[:, :self.num_states])
@@ -292,6 +316,22 @@ class EPCI:
         self.C_ci = X
         self.report = rep
         return self.energies
+
+    def add_spin_penalty(self, H: np.ndarray):
+        """create_S_sq_penalty (src/ci/detset/s2.cpp:126-153): H += a (S^2) (first order,
+        expected 0) or H += a (S^2 - expected)^2 (second order) per species, a = spin_penalty of
+        that species.  As in the reference, the species counter advances only past a nonzero
+        penalty, so a zero penalty also switches off every later species."""
+        power = 1 if self.first_order_spin_penalty else 2
+        idx = 0
+        for _ in self.prob.species:
+            alpha = self.spin_penalty[idx]
+            if alpha == 0.0:
+                continue
+            expected = 0.0 if power == 1 else self.expected_S2[idx]
+            H += alpha * self.detset.create_s2_penalty_dense(idx, expected, power)
+            idx += 1
+        return H
 
     def calculate_S_squared(self):
         """POLYQUANT_EPCI::calculate_S_squared (src/ci/epci.cpp:317-332): <S^2> of every state

@@ -764,6 +764,83 @@ __global__ void k_dense_h(ElemArgs a, PermArgs p, int64_t n, int64_t j0, double*
   H[i + (int64_t)blockIdx.y * ldh] = element_at(a, ref_to_int(p, lo), ref_to_int(p, hi));
 }
 
+// (S^2 - e) between two determinants given by the strings of every slot, for the species whose
+// alpha and beta sets are slots ta and tb (create_S_sq_minus_expected_S_sq_matrix_singleshot,
+// s2.cpp:156-255).  Diagonal: n_a + sz (sz - 1) - e, with x = alpha ^ beta the singly occupied
+// orbitals, n_a = |x & alpha| and sz = n_a - |x| / 2.  Off the diagonal: -phase_a phase_b when
+// alpha moves i -> j and beta j -> i with every other string equal, else 0.
+__device__ double s2_pen_elem(const uint64_t* bi, const uint64_t* bj, int nslot, int ta, int tb, double e) {
+  for (int u = 0; u < nslot; u++)
+    if (u != ta && u != tb && bi[u] != bj[u]) return 0.0;
+  const uint64_t ai = bi[ta], aj = bj[ta], Bi = bi[tb], Bj = bj[tb];
+  if (ai == aj && Bi == Bj) {
+    const uint64_t x = ai ^ Bi;
+    const int na = __popcll(x & ai);
+    const double sz = na - 0.5 * __popcll(x);
+    return na + sz * (sz - 1.0) - e;
+  }
+  if (__popcll(ai ^ aj) != 2 || __popcll(Bi ^ Bj) != 2) return 0.0;
+  const int ha = ctz64(ai & ~aj), pa = ctz64(aj & ~ai), hb = ctz64(Bi & ~Bj), pb = ctz64(Bj & ~Bi);
+  if (ha != pb || hb != pa) return 0.0;
+  const int odd = __popcll(ai & between_mask(min(ha, pa), max(ha, pa))) +
+                  __popcll(Bi & between_mask(min(hb, pb), max(hb, pb)));
+  return (odd & 1) ? 1.0 : -1.0;
+}
+
+struct S2PenArgs {
+  int ta, tb, power;
+  double e;
+  const uint8_t* valid;  // truncated spaces: internal address -> in space; nullptr for full CI
+};
+
+// Dense P = S^2 - e (power 1) or P^2 (power 2) of one species in reference address order, from
+// the (min, max) reference pair for both positions (exactly symmetric).  P^2 entry (I, J) is
+// P_II P_IJ + sum over the spin flips K of I that lie in the space of P_IK P_KJ: those are the
+// only nonzero P_IK (the sparse product of s2.cpp:147).
+__global__ void k_dense_s2(ElemArgs a, PermArgs p, S2PenArgs q, int64_t n, int64_t j0, double* __restrict__ S,
+                           int64_t lds) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t j = j0 + blockIdx.y;
+  if (i >= n) return;
+  const int64_t lo = min(i, j), hi = max(i, j);
+  int64_t ri = ref_to_int(p, lo), rj = ref_to_int(p, hi);
+  const int64_t ia = ri;
+  int ii[kSlots];
+  uint64_t bi[kSlots], bj[kSlots];
+  for (int t = a.nslot - 1; t >= 0; --t) {
+    ii[t] = (int)(ri % a.sl[t].nstr);
+    ri /= a.sl[t].nstr;
+    bi[t] = a.sl[t].bits[ii[t]];
+    bj[t] = a.sl[t].bits[rj % a.sl[t].nstr];
+    rj /= a.sl[t].nstr;
+  }
+  double v = s2_pen_elem(bi, bj, a.nslot, q.ta, q.tb, q.e);
+  if (q.power == 2) {
+    double w = s2_pen_elem(bi, bi, a.nslot, q.ta, q.tb, q.e) * v;
+    const uint64_t A = bi[q.ta], B = bi[q.tb];
+    const SlotDev &sa = a.sl[q.ta], &sb = a.sl[q.tb];
+    for (uint64_t xa = A & ~B; xa; xa &= xa - 1) {
+      const int oi = ctz64(xa);
+      for (uint64_t xb = B & ~A; xb; xb &= xb - 1) {
+        const int oj = ctz64(xb);
+        uint64_t bk[kSlots];
+        for (int t = 0; t < a.nslot; t++) bk[t] = bi[t];
+        bk[q.ta] = A ^ (1ull << oi) ^ (1ull << oj);
+        bk[q.tb] = B ^ (1ull << oj) ^ (1ull << oi);
+        if (q.valid) {
+          const int ka = str_index(sa, sa.rtab, bk[q.ta]), kb = str_index(sb, sb.rtab, bk[q.tb]);
+          if (ka < 0 || kb < 0) continue;
+          const int64_t addr = ia + (int64_t)(ka - ii[q.ta]) * a.stride[q.ta] + (int64_t)(kb - ii[q.tb]) * a.stride[q.tb];
+          if (!q.valid[addr]) continue;
+        }
+        w += s2_pen_elem(bi, bk, a.nslot, q.ta, q.tb, q.e) * s2_pen_elem(bk, bj, a.nslot, q.ta, q.tb, q.e);
+      }
+    }
+    v = w;
+  }
+  S[i + (int64_t)blockIdx.y * lds] = v;
+}
+
 // For every string I of a set: its singles (which = 1) or doubles (which = 2) as
 // (excited string internal index, phase bit), in enumeration order.
 __global__ void k_dump_excitations(SlotDev s, int which, int per, int32_t* __restrict__ J, int8_t* __restrict__ odd) {
@@ -1486,6 +1563,29 @@ static ElemArgs elem_args(pqg_ci* ci) {
   return a;
 }
 
+// Fills an N x N reference-order matrix column block by column block: at most 65535 columns per
+// launch (grid.y) and, for a host matrix, a device staging block of at most 512 MiB.
+// launch(j0, ncols, dst, ld) fills columns j0 .. j0 + ncols - 1 into dst.
+template <typename L>
+static void dense_fill(pqg_ci* ci, double* M, int64_t ldm, int on_device, L&& launch) {
+  const int64_t n = ci->n_ref;
+  PQG_REQUIRE(ldm >= n, PQG_ERR_ARG, "leading dimension smaller than N");
+  hipStream_t st = ci->stream;
+  int64_t ncol = std::min<int64_t>(n, 65535);
+  if (!on_device) ncol = std::max<int64_t>(1, std::min<int64_t>(ncol, (int64_t(512) << 20) / (8 * n)));
+  DevBuf<double> stage;
+  if (!on_device) stage.reset(ci->ctx, (size_t)n * ncol);
+  for (int64_t j0 = 0; j0 < n; j0 += ncol) {
+    const int64_t nc = std::min(ncol, n - j0);
+    double* dst = on_device ? M + j0 * ldm : stage.get();
+    launch(j0, nc, dst, on_device ? ldm : n);
+    PQG_HIP(hipGetLastError());
+    if (!on_device)
+      PQG_HIP(hipMemcpy2DAsync(M + j0 * ldm, ldm * 8, stage.get(), n * 8, n * 8, nc, hipMemcpyDeviceToHost, st));
+  }
+  PQG_HIP(hipStreamSynchronize(st));
+}
+
 extern "C" {
 
 pqg_status pqg_ci_create(pqg_ctx* ctx, const pqg_ci_desc* desc, pqg_ci** out) {
@@ -1540,27 +1640,32 @@ static int64_t ref_to_int_host(const pqg_ci* ci, int64_t a) {
 pqg_status pqg_ci_dense_h(pqg_ci* ci, double* H, int64_t ldh, int on_device) {
   return guard([&] {
     PQG_REQUIRE(ci && H, PQG_ERR_ARG, "NULL argument");
-    const int64_t n = ci->n_ref;
-    PQG_REQUIRE(ldh >= n, PQG_ERR_ARG, "leading dimension smaller than N");
-    hipStream_t st = ci->stream;
     const ElemArgs a = elem_args(ci);
     const PermArgs p = perm_args(ci);
-    // column blocks: at most 65535 columns per launch (grid.y) and, for a host matrix, a
-    // staging block of at most 512 MiB
-    int64_t ncol = std::min<int64_t>(n, 65535);
-    if (!on_device) ncol = std::max<int64_t>(1, std::min<int64_t>(ncol, (int64_t(512) << 20) / (8 * n)));
-    DevBuf<double> stage;
-    if (!on_device) stage.reset(ci->ctx, (size_t)n * ncol);
-    for (int64_t j0 = 0; j0 < n; j0 += ncol) {
-      const int64_t nc = std::min(ncol, n - j0);
-      double* dst = on_device ? H + j0 * ldh : stage.get();
-      const int64_t ld = on_device ? ldh : n;
-      k_dense_h<<<dim3(grid1(n, 256), (unsigned)nc), 256, 0, st>>>(a, p, n, j0, dst, ld);
-      PQG_HIP(hipGetLastError());
-      if (!on_device)
-        PQG_HIP(hipMemcpy2DAsync(H + j0 * ldh, ldh * 8, stage.get(), n * 8, n * 8, nc, hipMemcpyDeviceToHost, st));
-    }
-    PQG_HIP(hipStreamSynchronize(st));
+    const int64_t n = ci->n_ref;
+    dense_fill(ci, H, ldh, on_device, [&](int64_t j0, int64_t nc, double* dst, int64_t ld) {
+      k_dense_h<<<dim3(grid1(n, 256), (unsigned)nc), 256, 0, ci->stream>>>(a, p, n, j0, dst, ld);
+    });
+  });
+}
+
+pqg_status pqg_ci_dense_s2(pqg_ci* ci, int part, double expected, int power, double* S, int64_t lds, int on_device) {
+  return guard([&] {
+    PQG_REQUIRE(ci && S, PQG_ERR_ARG, "NULL argument");
+    PQG_REQUIRE(part >= 0 && part < ci->nsp, PQG_ERR_ARG, "species index out of range");
+    PQG_REQUIRE(power == 1 || power == 2, PQG_ERR_ARG, "power must be 1 or 2");
+    const ElemArgs a = elem_args(ci);
+    const PermArgs p = perm_args(ci);
+    S2PenArgs q{};
+    q.ta = 2 * part;
+    q.tb = 2 * part + 1;
+    q.power = power;
+    q.e = expected;
+    q.valid = ci->fci ? nullptr : ci->valid.get();
+    const int64_t n = ci->n_ref;
+    dense_fill(ci, S, lds, on_device, [&](int64_t j0, int64_t nc, double* dst, int64_t ld) {
+      k_dense_s2<<<dim3(grid1(n, 256), (unsigned)nc), 256, 0, ci->stream>>>(a, p, q, n, j0, dst, ld);
+    });
   });
 }
 

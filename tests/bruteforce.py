@@ -147,3 +147,34 @@ def s_squared(prob, dets, species, c):
                 if t is not None:
                     total += g1 * g2 * c[t] * c[j]
     return total
+
+
+def s_squared_matrix(prob, dets, species):
+    """The matrix of S^2 = S- S+ + Sz (Sz + 1) of one species between the determinants of the
+    space (operator algebra as in s_squared; intermediate states unrestricted)."""
+    n = prob.species[species].n_orb
+    sa, sb = 2 * species, 2 * species + 1
+    mask = (1 << n) - 1
+    index = {d: i for i, d in enumerate(dets)}
+    M = np.zeros((len(dets), len(dets)))
+    for j, det in enumerate(dets):
+        comb = det[sa] | (det[sb] << n)
+        nal, nbe = bin(det[sa]).count("1"), bin(det[sb]).count("1")
+        sz = 0.5 * (nal - nbe)
+        M[j, j] += sz * (sz + 1.0)
+        for i in range(n):
+            r1 = apply_E(comb, i, n + i)
+            if r1 is None:
+                continue
+            g1, s1 = r1
+            for k in range(n):
+                r2 = apply_E(s1, n + k, k)
+                if r2 is None:
+                    continue
+                g2, s2 = r2
+                nd = list(det)
+                nd[sa], nd[sb] = s2 & mask, s2 >> n
+                t = index.get(tuple(nd))
+                if t is not None:
+                    M[t, j] += g1 * g2
+    return M

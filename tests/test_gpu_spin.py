@@ -122,3 +122,56 @@ def test_s2_full_size_16o_8a8b(ctx, monkeypatch):
     monkeypatch.setenv("PQG_S2_DIRECT", "1")
     b = ds.evaluate_s2(C)[0, 0]
     assert abs(a - b) < 1e-11, (a, b)
+
+
+PENALTY_CASES = [n for n in CASES if n not in ("fci_open_shell_8o_4a2b", "fci_12o_6a5b")]
+
+
+@pytest.mark.parametrize("name", PENALTY_CASES)
+def test_s2_penalty_matrix_matches_oracle(name, ctx):
+    """create_S_sq_minus_expected_S_sq_matrix_singleshot (s2.cpp:156-255) and its square
+    (s2.cpp:141-147) as dense matrices on the GPU against the oracle, every species; exactly
+    symmetric."""
+    from oracle import refcpu
+    from polyquant_amd.ci import DetSet
+
+    prob = CASES[name]()
+    print(f"[gpu] case s2 penalty {name}", flush=True)
+    ds = DetSet(prob, ctx)
+    oc = refcpu.OracleCI(prob)
+    eye = np.eye(oc.N)
+    for p in range(prob.nspecies):
+        P0 = oc.s2_penalty(p, 0.0)  # the oracle's P(e) is P0 - e I and its square (P0 - e I)^2
+        for power, e in ((1, 0.0), (1, 0.75), (2, 2.0)):
+            got = ds.create_s2_penalty_dense(p, e, power)
+            ref = P0 - e * eye if power == 1 else (P0 - e * eye) @ (P0 - e * eye)
+            scale = max(1.0, np.max(np.abs(ref)))
+            assert np.max(np.abs(got - ref)) < 1e-12 * scale, (name, p, power)
+            assert np.array_equal(got, got.T)
+
+
+def test_exact_diag_with_spin_penalty(ctx):
+    """EPCI exact_diag with the first- and second-order spin penalties (epci.cpp:622-662,
+    s2.cpp:126-153): energies equal the eigenvalues of the oracle's H + a P; the second-order
+    penalty towards S^2 = 0 leaves the lowest roots singlets; bad arguments raise."""
+    from oracle import refcpu
+    from polyquant_amd import _lib
+    from polyquant_amd.ci import DetSet, EPCI
+
+    prob = wl.small_problem(313, 6, 2, 2)
+    print("[gpu] case exact_diag spin penalty fci_restricted_6o_2a2b", flush=True)
+    oc = refcpu.OracleCI(prob)
+    H = oc.dense()
+    P0 = oc.s2_penalty(0, 0.0)
+    for kind, a, P in (("first_order", 0.3, P0), ("second_order", 5.0, P0 @ P0)):
+        ci = EPCI(prob, ctx, num_states=4, exact_diag=True, spin_penalty_type=kind, spin_penalty=[a])
+        E = ci.run()
+        ref = np.linalg.eigvalsh(H + a * P)[:4] + prob.e_shift
+        assert np.max(np.abs(E - ref)) < 1e-10, kind
+        if kind == "second_order":
+            assert np.max(np.abs(ci.calculate_S_squared()[:, 0])) < 1e-8
+    ds = DetSet(prob, ctx)
+    with pytest.raises(_lib.PqgError):
+        ds.create_s2_penalty_dense(1, 0.0, 1)
+    with pytest.raises(_lib.PqgError):
+        ds.create_s2_penalty_dense(0, 0.0, 3)

@@ -375,3 +375,33 @@ def test_s2_of_eigenstates_is_s_times_s_plus_one():
     assert np.max(dist) < 1e-9
     for a in allowed:
         assert np.any(np.abs(S2 - a) < 1e-9)
+
+
+@pytest.mark.parametrize("kind", RDM_KINDS)
+def test_s2_penalty_matrix_matches_bruteforce(kind):
+    """create_S_sq_minus_expected_S_sq_matrix_singleshot (s2.cpp:156-255): P = S^2 - e I between
+    the determinants of the space, against the S^2 matrix from spin-flip operators."""
+    prob = _rdm_problem(kind)
+    oc = refcpu.OracleCI(prob)
+    dets = bruteforce.oracle_dets(oc)
+    for p in range(prob.nspecies):
+        M = bruteforce.s_squared_matrix(prob, dets, p)
+        assert np.max(np.abs(oc.s2_penalty(p, 0.0) - M)) < 1e-12
+        assert np.max(np.abs(oc.s2_penalty(p, 0.75) - (M - 0.75 * np.eye(oc.N)))) < 1e-12
+
+
+def test_s2_penalty_second_order_spectrum():
+    """Restricted full CI is closed under S^2, so (S^2 - e)^2 has the eigenvalues
+    (S(S+1) - e)^2 of the singlets, triplets and quintets of 5 orbitals 2a2b, and H + a (S^2 - 2)^2
+    keeps the triplet eigenpairs of H."""
+    prob = wl.small_problem(48, 5, 2, 2)
+    oc = refcpu.OracleCI(prob)
+    P2 = oc.s2_penalty(0, 2.0, power=2)
+    w = np.linalg.eigvalsh(P2)
+    assert np.max(np.min(np.abs(w[:, None] - np.array([4.0, 0.0, 16.0])[None, :]), axis=1)) < 1e-10
+    H = oc.dense()
+    E, V = np.linalg.eigh(H)
+    trip = np.abs(oc.s2(V)[:, 0] - 2.0) < 1e-9
+    Ep = np.linalg.eigvalsh(H + 0.5 * P2)
+    for e in E[trip]:
+        assert np.min(np.abs(Ep - e)) < 1e-9

@@ -47,3 +47,39 @@ def test_serving_mix_draw_rules():
         assert 1 <= pr.nroots <= 4
         if pr.nspecies == 2:
             assert 6 <= pr.species[1].n_orb <= 12
+
+
+def test_spin_penalty_host_logic_follows_reference_loop():
+    """create_S_sq_penalty (s2.cpp:126-153) host loop: first order adds a S^2 (expected 0),
+    second order a (S^2 - expected)^2, and a zero penalty stops every later species (the species
+    counter only advances past a nonzero penalty)."""
+    import numpy as np
+
+    from polyquant_amd.ci import EPCI
+
+    calls = []
+
+    class FakeDetSet:
+        def create_s2_penalty_dense(self, part, expected, power):
+            calls.append((part, expected, power))
+            return np.full((2, 2), 10.0 ** part)
+
+    class Prob:
+        species = [object(), object()]
+
+    def make(kind, pen, exp):
+        ci = object.__new__(EPCI)
+        ci.prob, ci.detset = Prob(), FakeDetSet()
+        ci.first_order_spin_penalty = kind == "first_order"
+        ci.second_order_spin_penalty = kind == "second_order"
+        ci.spin_penalty, ci.expected_S2 = pen, exp
+        return ci
+
+    H = make("first_order", [2.0, 3.0], [5.0, 6.0]).add_spin_penalty(np.zeros((2, 2)))
+    assert calls == [(0, 0.0, 1), (1, 0.0, 1)] and np.all(H == 2.0 + 30.0)
+    calls.clear()
+    H = make("second_order", [2.0, 3.0], [5.0, 6.0]).add_spin_penalty(np.zeros((2, 2)))
+    assert calls == [(0, 5.0, 2), (1, 6.0, 2)]
+    calls.clear()
+    H = make("second_order", [0.0, 3.0], [5.0, 6.0]).add_spin_penalty(np.zeros((2, 2)))
+    assert calls == [] and np.all(H == 0.0)
