@@ -171,6 +171,19 @@ pqg_status pqg_ci_dump_excitations(pqg_ci* ci, int which, int part, int spin, in
 pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* C, double* out4,
                      int on_device, int mode, double* seconds);
 
+/* compute_frozen_core_ints (integral.cpp:565-698) for one (species a, spin a) operator and one
+ * (species b, spin b) core density, as called by calculate_frozen_core_ints
+ * (integral.cpp:166-205): out (nao_a x nao_a, column-major, host) +=
+ *   cj * J(Dc) - [same species and a_spin == b_spin] * K(Dx)
+ * with J(D)_ij = sum_kl (ij|kl) D_kl, K(D)_ij = sum_kl (ik|jl) D_kl, Dc = 2 D (restricted b) or
+ * D_alpha + D_beta, Dx = D (restricted b) or D_{b_spin}, cj = 1 (same species, restricted b),
+ * 1/2 (same species, unrestricted b: one call per spin block) or charge_prod (two species).
+ * dm_b: the core densities of species b, 1 (restricted) or 2 consecutive nao_b x nao_b blocks
+ * (fc_dm[b]).  eri: same species, the 8-fold packed AO integrals (as pqg_ao2mo); two species,
+ * the npair_a x npair_b column-major block (ij|kl), idx2 packing.                            */
+pqg_status pqg_fc_ints(pqg_ctx* ctx, int nao_a, int nao_b, const double* eri, int same_particle, int a_spin,
+                       int b_spin, int restricted_b, double charge_prod, const double* dm_b, double* out);
+
 /* --- serving (continuous batching of CI solves) ------------------------------------------ */
 typedef struct pqg_server pqg_server;
 typedef struct {

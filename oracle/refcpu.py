@@ -307,6 +307,42 @@ class OracleCI:
         return ev, X, res, it
 
 
+def frozen_core_ints_block(out, eri, fc_dm_b, same_particle, a_spin, b_spin, restricted_b, charge_prod):
+    """compute_frozen_core_ints (integral.cpp:611-698) restated over basis functions, each its
+    own shell: unique pairs j <= i of species a and l <= k of species b (no screening), the
+    pair degeneracy factors, the Coulomb update of (i, j) and (j, i) and, for the same species
+    and spin, the eight exchange updates with weight 1/8.  eri: the full (na, na, nb, nb) AO
+    tensor.  Adds into out and returns it (pure-Python loops, small bases)."""
+    na, nb = eri.shape[0], eri.shape[2]
+    exchange = same_particle and a_spin == b_spin
+    for i in range(na):
+        for j in range(i + 1):
+            for k in range(nb):
+                for l in range(k + 1):
+                    v = eri[i, j, k, l]
+                    deg = (1.0 if i == j else 2.0) * (1.0 if k == l else 2.0)
+                    spinscale = 0.5 if (same_particle and not restricted_b) else 1.0
+                    scaleall = 0.5 * spinscale if same_particle else 0.5 * charge_prod * spinscale
+                    if restricted_b:
+                        d_kl = 2.0 * fc_dm_b[0][k, l]
+                    else:
+                        d_kl = fc_dm_b[0][k, l] + fc_dm_b[1][k, l]
+                    out[i, j] += scaleall * deg * d_kl * v
+                    out[j, i] += scaleall * deg * d_kl * v
+                    if exchange:
+                        D = fc_dm_b[0] if restricted_b else fc_dm_b[b_spin]
+                        w = 0.125 * deg * v
+                        out[i, k] -= w * D[j, l]
+                        out[k, i] -= w * D[j, l]
+                        out[j, l] -= w * D[i, k]
+                        out[l, j] -= w * D[i, k]
+                        out[i, l] -= w * D[j, k]
+                        out[l, i] -= w * D[j, k]
+                        out[j, k] -= w * D[i, l]
+                        out[k, j] -= w * D[i, l]
+    return out
+
+
 def ao2mo(ao_full, Ca, Cb=None):
     """Quarter-transform restatement; ao_full is the (n,n,n,n) AO tensor."""
     if Cb is None:

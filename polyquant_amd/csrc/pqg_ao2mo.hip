@@ -276,9 +276,83 @@ static void ao2mo_impl(pqg_ctx* ctx, int n, const double* eri8, const double* Cd
   }
 }
 
+// Frozen-core operator (compute_frozen_core_ints, integral.cpp:565-698) of species a from the core
+// density of species b, one output element (i, j) per thread:
+//   out(i, j) += cj * sum_kl (ij|kl) Dc(k, l) - cx * sum_kl (ik|jl) Dx(k, l).
+// eri: same species, the 8-fold packed AO integrals (pair index idx2 of the two pair indices);
+// two species, the npair_a x npair_b column-major block (ij|kl) at idx2(i,j) + npair_a * idx2(k,l).
+struct FcArgs {
+  const double* eri;
+  const double *dc, *dx;  // dx: exchange density, nullptr without exchange
+  int na, nb, same;
+  double cj, cx;
+};
+
+__device__ __forceinline__ int64_t fidx2(int64_t p, int64_t q) { return p >= q ? p * (p + 1) / 2 + q : q * (q + 1) / 2 + p; }
+
+__device__ __forceinline__ double fc_eri(const FcArgs& a, int64_t pij, int64_t pkl) {
+  return a.same ? a.eri[fidx2(pij, pkl)] : a.eri[pij + (int64_t)a.na * (a.na + 1) / 2 * pkl];
+}
+
+__global__ void k_fc_jk(FcArgs a, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= a.na) return;
+  const int64_t pij = fidx2(i, j);
+  double J = 0.0, K = 0.0;
+  for (int l = 0; l < a.nb; l++)
+    for (int k = 0; k < a.nb; k++) {
+      J += a.dc[k + (int64_t)l * a.nb] * fc_eri(a, pij, fidx2(k, l));
+      if (a.dx) K += a.dx[k + (int64_t)l * a.nb] * fc_eri(a, fidx2(i, k), fidx2(j, l));
+    }
+  out[i + (int64_t)j * a.na] += a.cj * J - a.cx * K;
+}
+
 }  // namespace pqg
 
 using namespace pqg;
+
+extern "C" pqg_status pqg_fc_ints(pqg_ctx* ctx, int nao_a, int nao_b, const double* eri, int same_particle, int a_spin,
+                                  int b_spin, int restricted_b, double charge_prod, const double* dm_b, double* out) {
+  return guard([&] {
+    PQG_REQUIRE(ctx && eri && dm_b && out, PQG_ERR_ARG, "NULL argument");
+    PQG_REQUIRE(nao_a >= 1 && nao_b >= 1, PQG_ERR_ARG, "empty basis");
+    PQG_REQUIRE(!same_particle || nao_a == nao_b, PQG_ERR_ARG, "same species needs one basis");
+    PQG_REQUIRE(a_spin >= 0 && a_spin < 2 && b_spin >= 0 && b_spin < 2, PQG_ERR_ARG, "spin index out of range");
+    PQG_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int64_t na = nao_a, nb = nao_b;
+    const int64_t pa = na * (na + 1) / 2, pb = nb * (nb + 1) / 2;
+    const int64_t neri = same_particle ? pa * (pa + 1) / 2 : pa * pb;
+    const int64_t nd = nb * nb;
+    // host densities -> the Coulomb density (2 D restricted, D_alpha + D_beta unrestricted) and
+    // the exchange density of spin b
+    std::vector<double> dc(nd), dx;
+    for (int64_t q = 0; q < nd; q++) dc[q] = restricted_b ? 2.0 * dm_b[q] : dm_b[q] + dm_b[nd + q];
+    const bool exchange = same_particle && a_spin == b_spin;
+    if (exchange) dx.assign(dm_b + (restricted_b ? 0 : b_spin * nd), dm_b + (restricted_b ? 0 : b_spin * nd) + nd);
+    DevBuf<double> de(ctx, neri), ddc(ctx, nd), ddx(ctx, exchange ? nd : 1), dout(ctx, na * na);
+    PQG_HIP(hipMemcpyAsync(de.get(), eri, neri * 8, hipMemcpyHostToDevice, st));
+    PQG_HIP(hipMemcpyAsync(ddc.get(), dc.data(), nd * 8, hipMemcpyHostToDevice, st));
+    if (exchange) PQG_HIP(hipMemcpyAsync(ddx.get(), dx.data(), nd * 8, hipMemcpyHostToDevice, st));
+    PQG_HIP(hipMemcpyAsync(dout.get(), out, na * na * 8, hipMemcpyHostToDevice, st));
+    FcArgs a{};
+    a.eri = de.get();
+    a.dc = ddc.get();
+    a.dx = exchange ? ddx.get() : nullptr;
+    a.na = nao_a;
+    a.nb = nao_b;
+    a.same = same_particle ? 1 : 0;
+    // same species: spinscale (1/2 when species b is unrestricted, whose two spin blocks each
+    // add one call); two species: the charge product
+    a.cj = same_particle ? (restricted_b ? 1.0 : 0.5) : charge_prod;
+    a.cx = 1.0;
+    k_fc_jk<<<dim3(gr(na, 64), (unsigned)na), 64, 0, st>>>(a, dout.get());
+    PQG_HIP(hipGetLastError());
+    PQG_HIP(hipMemcpyAsync(out, dout.get(), na * na * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+  });
+}
 
 extern "C" pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* C, double* out4,
                                 int on_device, int mode, double* seconds) {

@@ -53,3 +53,119 @@ def transform_device(ctx: Context, n: int, eri8_ptr: int, C_ptr: int, out_ptr: i
     check(load().pqg_ao2mo(ctx.h, n, n, ct.c_void_p(eri8_ptr), ct.c_void_p(C_ptr), ct.c_void_p(out_ptr), 1, mode,
                            ct.byref(sec)))
     return sec.value
+
+
+# --- frozen core (src/ci/epci.cpp:47-129, src/integral/integral.cpp:166-205, 565-698) ---------
+
+def fc_ints_block(out: np.ndarray, eri: np.ndarray, dm_b, same_particle: bool, a_spin: int, b_spin: int,
+                  restricted_b: bool, charge_prod: float, ctx: Optional[Context] = None) -> np.ndarray:
+    """compute_frozen_core_ints (integral.cpp:565-698) on the GPU: out += the frozen-core
+    operator of one (species b, spin b) core density.  eri: 8-fold packed AO integrals (same
+    species) or the npair_a x npair_b (ij|kl) block (two species); dm_b: fc_dm[b] (1 or 2
+    matrices)."""
+    ctx = ctx or default_context()
+    na = out.shape[0]
+    nb = dm_b[0].shape[0]
+    # block-major, each block column-major
+    dm = np.concatenate([np.asarray(d, dtype=np.float64).ravel(order="F") for d in dm_b])
+    eri = np.ascontiguousarray(eri, dtype=np.float64) if same_particle else np.asfortranarray(eri, dtype=np.float64)
+    res = np.asfortranarray(out, dtype=np.float64).copy(order="F")
+    check(load().pqg_fc_ints(ctx.h, na, nb, eri.ctypes.data_as(ct.c_void_p), int(same_particle), a_spin, b_spin,
+                             int(restricted_b), float(charge_prod), dm.ctypes.data_as(ct.c_void_p),
+                             res.ctypes.data_as(ct.c_void_p)))
+    out[...] = res
+    return out
+
+
+def calculate_frozen_core_ints(species, fc_dm, frozen_core, eri_ao, kinetic, nuclear,
+                               ctx: Optional[Context] = None):
+    """POLYQUANT_INTEGRAL::calculate_frozen_core_ints (integral.cpp:166-205): for every species a
+    and spin block of fc_dm[a], the sum over species b with frozen_core[b] != 0 and over the spin
+    blocks of fc_dm[b] of compute_frozen_core_ints, plus kinetic[a] - charge_a nuclear[a].
+    species: objects with .charge and .restricted; eri_ao[(a, b)]: the 8-fold packed block (a == b)
+    or the npair_a x npair_b block (a < b; a > b uses its transpose)."""
+    out = []
+    for a, spa in enumerate(species):
+        na = kinetic[a].shape[0]
+        per_spin = []
+        for sa in range(len(fc_dm[a])):
+            F = np.zeros((na, na))
+            for b, spb in enumerate(species):
+                for sb in range(len(fc_dm[b])):
+                    if frozen_core[b] != 0:
+                        eri = eri_ao[(a, b)] if a <= b else np.asarray(eri_ao[(b, a)]).T
+                        fc_ints_block(F, eri, fc_dm[b], a == b, sa, sb, spb.restricted, spa.charge * spb.charge, ctx)
+            F += kinetic[a] + (-spa.charge * nuclear[a])
+            per_spin.append(F)
+        out.append(per_spin)
+    return out
+
+
+def frozen_core_density(C, nfc: int) -> np.ndarray:
+    """form_occ_helper_aufbau (the nfc lowest orbitals, occupation 1) then form_DM_helper:
+    D = sum_i occ_i C[:, i] C[:, i]^T (epci.cpp:80-94)."""
+    C = np.asarray(C, dtype=np.float64)
+    Cc = C[:, :nfc]
+    return Cc @ Cc.T
+
+
+def calculate_fc_energy(species, C, frozen_core, eri_ao, kinetic, nuclear, ctx: Optional[Context] = None):
+    """POLYQUANT_EPCI::calculate_fc_energy (epci.cpp:47-129).  species: objects with .charge,
+    .restricted and .num_parts; C[a][s]: MO coefficients (nao x nmo) per spin block.  Returns
+    (fc_dm, frozen_core_ints, frozen_core_energy per species)."""
+    fc_dm = []
+    for a, sp in enumerate(species):
+        nao = kinetic[a].shape[0]
+        two = sp.num_parts > 1 and not sp.restricted
+        blocks = [np.zeros((nao, nao)) for _ in range(2 if two else 1)]
+        if frozen_core[a] != 0:
+            blocks[0] = frozen_core_density(C[a][0], frozen_core[a])
+            if two:
+                blocks[1] = frozen_core_density(C[a][1], frozen_core[a])
+        fc_dm.append(blocks)
+    fci = calculate_frozen_core_ints(species, fc_dm, frozen_core, eri_ao, kinetic, nuclear, ctx)
+    energy = []
+    for a, sp in enumerate(species):
+        e = 0.0
+        if frozen_core[a] != 0:
+            Hc = kinetic[a] + (-sp.charge * nuclear[a])
+            if not sp.restricted:
+                d1 = fc_dm[a][1] if len(fc_dm[a]) > 1 else fc_dm[a][0]
+                f1 = fci[a][1] if len(fci[a]) > 1 else fci[a][0]
+                e = 0.5 * float(np.sum((fc_dm[a][0] + d1) * Hc) + np.sum(fc_dm[a][0] * fci[a][0]) + np.sum(d1 * f1))
+            else:
+                e = float(np.sum(fc_dm[a][0] * (Hc + fci[a][0])))
+        energy.append(e)
+    return fc_dm, fci, energy
+
+
+def calculate_mo_1_body_integrals(C, frozen_core, deleted_virtual, frozen_core_ints):
+    """integral.cpp:565-610 with frozen-core operators: h[a][s] = C_sub^T F[a][s] C_sub,
+    C_sub = the columns frozen_core[a] .. nmo - deleted_virtual[a] - 1 of C[a][s]."""
+    out = []
+    for a, Cs in enumerate(C):
+        per = []
+        for s, Cm in enumerate(Cs):
+            Cm = np.asarray(Cm, dtype=np.float64)
+            nmo = Cm.shape[1] - frozen_core[a] - deleted_virtual[a]
+            sub = Cm[:, frozen_core[a]:frozen_core[a] + nmo]
+            F = frozen_core_ints[a][min(s, len(frozen_core_ints[a]) - 1)]
+            per.append(sub.T @ F @ sub)
+        out.append(per)
+    return out
+
+
+def active_two_body(out4: np.ndarray, nmo: int, first: int, nact: int) -> np.ndarray:
+    """The (ij|kl) block of the orbitals first .. first + nact - 1 from the full 4-fold matrix of
+    nmo orbitals (the transform with the MO subset of integral.cpp:207-477)."""
+    def pairs(n, off):
+        ii, jj = [], []
+        for i in range(n):
+            for j in range(i + 1):
+                ii.append(i + off)
+                jj.append(j + off)
+        return np.array(ii), np.array(jj)
+
+    ia, ja = pairs(nact, first)
+    idx = np.maximum(ia, ja) * (np.maximum(ia, ja) + 1) // 2 + np.minimum(ia, ja)
+    return np.ascontiguousarray(np.asarray(out4)[np.ix_(idx, idx)])

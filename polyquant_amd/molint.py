@@ -306,7 +306,8 @@ WATER_GEOMETRY_ANGSTROM = np.array([[0.0, 0.0, 0.0], [0.7569685, 0.0, -0.5858752
 def water(basis_name: str):
     """RHF water at O-H 0.9572 A, HOH 104.52 deg (the reference's h2o.json geometry).
 
-    Returns dict with E_nuc, E_rhf, C, h_mo (n x n) and eri_mo (n^4, chemists' notation).
+    Returns dict with E_nuc, E_rhf, C, h_mo (n x n), eri_mo (n^4, chemists' notation) and the AO
+    kinetic T, electron-nuclear V and eri_ao (n^4) integrals.
     """
     symbols = ["O", "H", "H"]
     coords = WATER_GEOMETRY_ANGSTROM / BOHR_TO_ANGSTROM
@@ -320,4 +321,5 @@ def water(basis_name: str):
     h_mo = C.T @ Hcore @ C
     eri_mo = np.einsum("pi,qj,rk,sl,pqrs->ijkl", C, C, C, C, eri, optimize=True)
     return {"E_nuc": Enuc, "E_rhf": E_elec + Enuc, "C": C, "eps": eps, "h_mo": h_mo, "eri_mo": eri_mo,
-            "n_orb": len(basis), "n_alpha": (nel + 1) // 2, "n_beta": nel // 2}
+            "n_orb": len(basis), "n_alpha": (nel + 1) // 2, "n_beta": nel // 2,
+            "T": T, "V": V, "eri_ao": eri}

@@ -405,3 +405,66 @@ def test_s2_penalty_second_order_spectrum():
     Ep = np.linalg.eigvalsh(H + 0.5 * P2)
     for e in E[trip]:
         assert np.min(np.abs(Ep - e)) < 1e-9
+
+
+def _sym_ao(rng, na, nb, same):
+    X = rng.standard_normal((na, na, nb, nb))
+    X = X + X.transpose(1, 0, 2, 3)
+    X = X + X.transpose(0, 1, 3, 2)
+    if same:
+        X = X + X.transpose(2, 3, 0, 1)
+    return X
+
+
+@pytest.mark.parametrize("same,restricted,a_spin,b_spin", [(True, True, 0, 0), (True, False, 0, 0),
+                                                           (True, False, 1, 0), (False, True, 0, 0),
+                                                           (False, False, 0, 1)])
+def test_frozen_core_ints_closed_form(same, restricted, a_spin, b_spin):
+    """The restated shell-pair loop of compute_frozen_core_ints equals the closed form
+    cj J(Dc) - [same, same spin] K(Dx): restricted same species 2 J(D) - K(D) (the closed-shell
+    core Fock operator), unrestricted 1/2 J(Da + Db) per call with K(D_spin), two species
+    charge_prod J(Dc)."""
+    rng = np.random.default_rng(5)
+    na = 5
+    nb = na if same else 4
+    X = _sym_ao(rng, na, nb, same)
+    dms = [(lambda M: M @ M.T)(rng.standard_normal((nb, 2))) for _ in range(2)]
+    dm_b = dms[:1] if restricted else dms
+    out = refcpu.frozen_core_ints_block(np.zeros((na, na)), X, dm_b, same, a_spin, b_spin, restricted, -1.0)
+    Dc = 2 * dm_b[0] if restricted else dm_b[0] + dm_b[1]
+    J = np.einsum("ijkl,kl->ij", X, Dc)
+    cj = (1.0 if restricted else 0.5) if same else -1.0
+    ref = cj * J
+    if same and a_spin == b_spin:
+        Dx = dm_b[0] if restricted else dm_b[b_spin]
+        ref = ref - np.einsum("ikjl,kl->ij", X, Dx)
+    assert np.max(np.abs(out - ref)) < 1e-11
+
+
+def test_frozen_core_oracle_water_equals_restricted_full_space():
+    """The restated frozen-core operator with the frozen-core energy of calculate_fc_energy
+    (epci.cpp:118-125, restricted: sum D (H_core + F)) and the active one-body integrals
+    C_act^T F C_act: water STO-3G with the oxygen 1s frozen reproduces the lowest eigenvalues of
+    the full-space Hamiltonian restricted to the determinants with orbital 0 doubly occupied."""
+    from polyquant_amd import molint
+
+    w = molint.water("sto-3g")
+    n, nfc = w["n_orb"], 1
+    C, T, V, X = w["C"], w["T"], w["V"], w["eri_ao"]
+    D = C[:, :nfc] @ C[:, :nfc].T
+    F = refcpu.frozen_core_ints_block(np.zeros((n, n)), X, [D], True, 0, 0, True, -1.0 * -1.0) + T + V
+    efc = float(np.sum(D * (T + V + F)))
+    sub = C[:, nfc:]
+    na = n - nfc
+    vmo = np.einsum("pi,qj,rk,sl,pqrs->ijkl", sub, sub, sub, sub, X, optimize=True)
+    ia, ja = np.array([(i, j) for i in range(na) for j in range(i + 1)]).T
+    v_act = np.ascontiguousarray(vmo[ia[:, None], ja[:, None], ia[None, :], ja[None, :]])
+    prob = wl.CIProblem(species=[wl.Species(na, 4, 4, -1.0, 1, None)], h=[[sub.T @ F @ sub]],
+                        eri={(0, 0, 0, 0): v_act}, nroots=3, e_shift=w["E_nuc"] + efc, name="h2o_fc1")
+    E = np.linalg.eigvalsh(refcpu.OracleCI(prob).dense())[:3] + prob.e_shift
+    full = wl.water_problem("sto-3g")
+    oc = refcpu.OracleCI(full)
+    Sa, Sb, U = oc.strings(0, 0), oc.strings(0, 1), oc.unfold_all()
+    keep = [i for i in range(oc.N) if (int(Sa[U[i, 0]]) & 1) and (int(Sb[U[i, 1]]) & 1)]
+    ref = np.linalg.eigvalsh(oc.dense()[np.ix_(keep, keep)])[:3] + full.e_shift
+    assert np.max(np.abs(E - ref)) < 1e-9
