@@ -119,6 +119,15 @@ pqg_status pqg_ci_elements(pqg_ci* ci, const int64_t* i, const int64_t* j, int64
  * leading dimension ldh, element (i, j) = Slater_Condon(min(i, j), max(i, j)) (diagonal
  * diagonal_Hii), so H is exactly symmetric.  on_device != 0: H is a device pointer.          */
 pqg_status pqg_ci_dense_h(pqg_ci* ci, double* H, int64_t ldh, int on_device);
+/* create_ham (explicit_ham.cpp:51-84 -> single_species_create_ham_singleshot,
+ * explicit_ham_singleparticle.cpp:170-537, two_species_create_ham_singleshot,
+ * explicit_ham_mixedparticle.cpp:186-550) as the reference's compressed row-major upper
+ * triangle with int indices: row r holds the diagonal and every J > r with a nonzero
+ * Slater_Condon element, columns ascending; screening > 0 drops every |v| <= screening
+ * (ham.prune(screening, 1.0), explicit_ham.cpp:78-80).  Host arrays.  Call with cols = vals =
+ * NULL for *nnz (and rowptr[N + 1] when rowptr != NULL), then with cap >= *nnz to fill.      */
+pqg_status pqg_ci_csr(pqg_ci* ci, double screening, int32_t* rowptr, int32_t* cols, double* vals, int64_t cap,
+                      int64_t* nnz);
 /* create_S_sq_minus_expected_S_sq_matrix_singleshot (s2.cpp:156-255) as a dense matrix, and
  * its square for the second-order penalty (s2.cpp:141-147): P = S^2 - expected * I of species
  * part (power 1) or P^2 (power 2, summed over the determinants of the space), N x N reference

@@ -168,6 +168,19 @@ class DetSet:
         check(load().pqg_ci_dense_h(self.h, _ptr(H), n, 0))
         return H
 
+    def create_ham_csr(self, screening: float = 0.0):
+        """create_ham (src/ci/detset/explicit_ham.cpp:51-84): the upper triangle of H as
+        compressed rows (rowptr int32 [N + 1], cols int32, vals), columns ascending, built on the
+        GPU; screening > 0 drops |v| <= screening as ham.prune(screening, 1.0)."""
+        n = self.N_dets
+        rowptr = np.zeros(n + 1, dtype=np.int32)
+        nnz = ct.c_int64()
+        check(load().pqg_ci_csr(self.h, float(screening), _ptr(rowptr), None, None, 0, ct.byref(nnz)))
+        cols = np.zeros(nnz.value, dtype=np.int32)
+        vals = np.zeros(nnz.value)
+        check(load().pqg_ci_csr(self.h, float(screening), None, _ptr(cols), _ptr(vals), nnz.value, ct.byref(nnz)))
+        return rowptr, cols, vals
+
     def create_s2_penalty_dense(self, part: int, expected: float = 0.0, power: int = 1) -> np.ndarray:
         """create_S_sq_minus_expected_S_sq_matrix_singleshot (src/ci/detset/s2.cpp:156-255) as a
         dense N x N matrix P = S^2 - expected I of species part (power 1), or P @ P (power 2, the

@@ -31,6 +31,8 @@
 #include <type_traits>
 #include <unordered_map>
 
+#include <hipcub/hipcub.hpp>
+
 #include "pqg_ci.hpp"
 #include "pqg_strings.hpp"
 
@@ -762,6 +764,108 @@ __global__ void k_dense_h(ElemArgs a, PermArgs p, int64_t n, int64_t j0, double*
   if (i >= n) return;
   const int64_t lo = min(i, j), hi = max(i, j);
   H[i + (int64_t)blockIdx.y * ldh] = element_at(a, ref_to_int(p, lo), ref_to_int(p, hi));
+}
+
+// Upper-triangle CSR of H in reference order (create_ham, explicit_ham.cpp:51-84 with
+// single_species_create_ham_singleshot / two_species_create_ham_singleshot): row r holds the
+// diagonal (always) and every connected determinant J with reference address above r whose
+// Slater_Condon element is nonzero; with a screening threshold s > 0 every entry with |v| <= s is
+// dropped (ham.prune(s, 1.0)).  One thread per row enumerates the singles and doubles of each
+// string and the single x single pairs of two strings.
+struct CsrArgs {
+  ElemArgs e;
+  PermArgs p;
+  const int64_t* inv;             // truncated spaces: internal address -> reference address (-1: not in space)
+  const int32_t* i2r[kSlots];     // full CI: internal -> reference string index per set
+  double screen;
+};
+
+__device__ __forceinline__ int64_t csr_ref_of(const CsrArgs& c, int64_t addr) {
+  if (c.inv) return c.inv[addr];
+  int64_t r = 0;
+  for (int t = c.e.nslot - 1; t >= 0; --t) {
+    const int64_t d = addr % c.e.sl[t].nstr;
+    addr /= c.e.sl[t].nstr;
+    r += (int64_t)c.i2r[t][d] * c.e.stride[t];
+  }
+  return r;
+}
+
+template <class F>
+__device__ void csr_row(const CsrArgs& c, int64_t r, F&& emit) {
+  const ElemArgs& a = c.e;
+  const int64_t ia = ref_to_int(c.p, r);
+  int idx[kSlots];
+  uint64_t bits[kSlots];
+  int64_t rr = ia;
+  for (int t = a.nslot - 1; t >= 0; --t) {
+    idx[t] = (int)(rr % a.sl[t].nstr);
+    rr /= a.sl[t].nstr;
+    bits[t] = a.sl[t].bits[idx[t]];
+  }
+  {
+    const double v = a.Hd[ia];
+    if (c.screen <= 0.0 || fabs(v) > c.screen) emit(r, v);
+  }
+  auto visit = [&](int64_t addr) {
+    const int64_t rj = csr_ref_of(c, addr);
+    if (rj <= r) return;  // lower triangle, or not in the space (-1)
+    const double v = element_at(a, ia, addr);
+    if (v != 0.0 && (c.screen <= 0.0 || fabs(v) > c.screen)) emit(rj, v);
+  };
+  for (int t = 0; t < a.nslot; t++) {
+    const SlotDev& s = a.sl[t];
+    for_singles(bits[t], s.n_orb, [&](int, int, int, int, int, uint64_t J) {
+      const int j = str_index(s, s.rtab, J);
+      if (j >= 0) visit(ia + (int64_t)(j - idx[t]) * a.stride[t]);
+    });
+    for_doubles(bits[t], s.n_orb, [&](int, int, int, int, int, uint64_t J) {
+      const int j = str_index(s, s.rtab, J);
+      if (j >= 0) visit(ia + (int64_t)(j - idx[t]) * a.stride[t]);
+    });
+    for (int u = t + 1; u < a.nslot; u++) {
+      const SlotDev& su = a.sl[u];
+      for_singles(bits[t], s.n_orb, [&](int, int, int, int, int, uint64_t J) {
+        const int j = str_index(s, s.rtab, J);
+        if (j < 0) return;
+        const int64_t base = ia + (int64_t)(j - idx[t]) * a.stride[t];
+        for_singles(bits[u], su.n_orb, [&](int, int, int, int, int, uint64_t K) {
+          const int k = str_index(su, su.rtab, K);
+          if (k >= 0) visit(base + (int64_t)(k - idx[u]) * a.stride[u]);
+        });
+      });
+    }
+  }
+}
+
+__global__ void k_csr_count(CsrArgs c, int64_t n, int64_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t m = 0;
+  csr_row(c, r, [&](int64_t, double) { ++m; });
+  cnt[r] = m;
+}
+
+__global__ void k_csr_fill(CsrArgs c, int64_t n, const int64_t* __restrict__ rowptr, int32_t* __restrict__ cols,
+                           double* __restrict__ vals) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t q = rowptr[r];
+  csr_row(c, r, [&](int64_t j, double v) {
+    cols[q] = (int32_t)j;
+    vals[q] = v;
+    ++q;
+  });
+}
+
+__global__ void k_inv_addr(const int64_t* __restrict__ ref_addr, int64_t nref, int64_t* __restrict__ inv) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nref) inv[ref_addr[r]] = r;
+}
+
+__global__ void k_fill_i64(int64_t* __restrict__ x, int64_t n, int64_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
 }
 
 // (S^2 - e) between two determinants given by the strings of every slot, for the species whose
@@ -1646,6 +1750,71 @@ pqg_status pqg_ci_dense_h(pqg_ci* ci, double* H, int64_t ldh, int on_device) {
     dense_fill(ci, H, ldh, on_device, [&](int64_t j0, int64_t nc, double* dst, int64_t ld) {
       k_dense_h<<<dim3(grid1(n, 256), (unsigned)nc), 256, 0, ci->stream>>>(a, p, n, j0, dst, ld);
     });
+  });
+}
+
+pqg_status pqg_ci_csr(pqg_ci* ci, double screening, int32_t* rowptr, int32_t* cols, double* vals, int64_t cap,
+                      int64_t* nnz) {
+  return guard([&] {
+    PQG_REQUIRE(ci && nnz, PQG_ERR_ARG, "NULL argument");
+    PQG_REQUIRE(ci->n_ref < (int64_t(1) << 31), PQG_ERR_UNSUPPORTED, "N above 2^31 - 1 (int32 CSR indices)");
+    hipStream_t st = ci->stream;
+    const int64_t n = ci->n_ref;
+    CsrArgs c{};
+    c.e = elem_args(ci);
+    c.p = perm_args(ci);
+    c.screen = screening;
+    DevBuf<int64_t> inv;
+    DevBuf<int32_t> i2r[kSlots];
+    if (ci->fci) {
+      for (int t = 0; t < ci->nslot; t++) {
+        const Slot& sl = ci->slot[t];
+        i2r[t].reset(ci->ctx, sl.nstr);
+        PQG_HIP(hipMemcpyAsync(i2r[t].get(), sl.int2ref.data(), (size_t)sl.nstr * 4, hipMemcpyHostToDevice, st));
+        c.i2r[t] = i2r[t].get();
+      }
+    } else {
+      inv.reset(ci->ctx, ci->n_int);
+      k_fill_i64<<<grid1(ci->n_int, 256), 256, 0, st>>>(inv.get(), ci->n_int, -1);
+      k_inv_addr<<<grid1(n, 256), 256, 0, st>>>(ci->ref_addr.get(), n, inv.get());
+      PQG_HIP(hipGetLastError());
+      c.inv = inv.get();
+    }
+    // counts -> row pointers (exclusive scan over n + 1 entries, the last count 0)
+    DevBuf<int64_t> cnt(ci->ctx, n + 1), ptr(ci->ctx, n + 1);
+    k_fill_i64<<<grid1(n + 1, 256), 256, 0, st>>>(cnt.get(), n + 1, 0);
+    k_csr_count<<<grid1(n, 128), 128, 0, st>>>(c, n, cnt.get());
+    PQG_HIP(hipGetLastError());
+    size_t tb = 0;
+    PQG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.get(), ptr.get(), (int)(n + 1), st));
+    DevBuf<uint8_t> tmp(ci->ctx, std::max<size_t>(tb, 1));
+    PQG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tb, cnt.get(), ptr.get(), (int)(n + 1), st));
+    std::vector<int64_t> hptr(n + 1);
+    PQG_HIP(hipMemcpyAsync(hptr.data(), ptr.get(), (n + 1) * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+    const int64_t total = hptr[n];
+    *nnz = total;
+    PQG_REQUIRE(total < (int64_t(1) << 31), PQG_ERR_UNSUPPORTED, "more than 2^31 - 1 nonzeros (int32 CSR indices)");
+    if (rowptr)
+      for (int64_t r = 0; r <= n; r++) rowptr[r] = (int32_t)hptr[r];
+    if (!cols && !vals) return;
+    PQG_REQUIRE(cols && vals, PQG_ERR_ARG, "cols and vals go together");
+    PQG_REQUIRE(cap >= total, PQG_ERR_ARG, "capacity below the number of nonzeros");
+    if (total == 0) return;
+    DevBuf<int32_t> c0(ci->ctx, total), c1(ci->ctx, total);
+    DevBuf<double> v0(ci->ctx, total), v1(ci->ctx, total);
+    k_csr_fill<<<grid1(n, 128), 128, 0, st>>>(c, n, ptr.get(), c0.get(), v0.get());
+    PQG_HIP(hipGetLastError());
+    // columns ascending within each row (the compressed row-major order of the reference)
+    size_t sb = 0;
+    PQG_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, sb, c0.get(), c1.get(), v0.get(), v1.get(), (int)total,
+                                                        (int)n, ptr.get(), ptr.get() + 1, 0, 32, st));
+    DevBuf<uint8_t> stmp(ci->ctx, std::max<size_t>(sb, 1));
+    PQG_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(stmp.get(), sb, c0.get(), c1.get(), v0.get(), v1.get(),
+                                                        (int)total, (int)n, ptr.get(), ptr.get() + 1, 0, 32, st));
+    PQG_HIP(hipMemcpyAsync(cols, c1.get(), total * 4, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipMemcpyAsync(vals, v1.get(), total * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
   });
 }
 

@@ -313,3 +313,48 @@ def test_exact_diag_water_sto3g_cisd(ctx):
     # the eigenvectors feed the same post-processing as the Davidson ones
     s2 = ci.calculate_S_squared()
     assert np.max(np.abs(s2[:, 0] - np.round(s2[:, 0]))) < 1e-8
+
+
+def _csr_dense(rowptr, cols, vals, n):
+    U = np.zeros((n, n))
+    for r in range(n):
+        for q in range(rowptr[r], rowptr[r + 1]):
+            U[r, cols[q]] += vals[q]
+    return U
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_csr_hamiltonian_matches_oracle(name, ctx):
+    """create_ham (explicit_ham.cpp:51-84): compressed upper-triangle rows in reference order,
+    diagonal first, columns strictly ascending, no explicit zeros off the diagonal, values within
+    1e-12 of the oracle's Slater_Condon matrix, every oracle entry above 1e-12 present; the
+    symmetric product with random vectors equals the GPU sigma."""
+    prob, ds, oc = _pair(name, ctx)
+    n = ds.N_dets
+    rowptr, cols, vals = ds.create_ham_csr()
+    assert rowptr[0] == 0 and rowptr[-1] == len(cols) == len(vals)
+    for r in range(n):
+        c = cols[rowptr[r]:rowptr[r + 1]]
+        assert c[0] == r and np.all(np.diff(c) > 0)
+        assert np.all(vals[rowptr[r] + 1:rowptr[r + 1]] != 0.0)
+    Hor = oc.dense()
+    scale = max(1.0, np.max(np.abs(Hor)))
+    U = _csr_dense(rowptr, cols, vals, n)
+    assert np.max(np.abs(U - np.triu(Hor))) < 1e-12 * scale
+    assert np.all(U[np.abs(np.triu(Hor)) > 1e-12 * scale] != 0.0)
+    C = np.random.default_rng(3).standard_normal((n, 2))
+    Hs = U + np.triu(U, 1).T
+    assert np.max(np.abs(Hs @ C - ds.sigma(C))) < 1e-11 * scale * np.max(np.abs(C)) * max(1, n) ** 0.5
+
+
+def test_csr_hamiltonian_screening(ctx):
+    """ham.prune(screening, 1.0) (explicit_ham.cpp:78-80): every kept entry exceeds the threshold
+    and every unscreened entry above it is kept with the same value."""
+    prob, ds, oc = _pair("cisd_h2o_layout_7o_5a5b", ctx)
+    n = ds.N_dets
+    full = _csr_dense(*ds.create_ham_csr(), n)
+    s = 0.05
+    rowptr, cols, vals = ds.create_ham_csr(s)
+    assert np.all(np.abs(vals) > s)
+    kept = _csr_dense(rowptr, cols, vals, n)
+    assert np.array_equal(kept, np.where(np.abs(full) > s, full, 0.0))
