@@ -285,7 +285,19 @@ class EPCI:
     def __init__(self, prob, ctx: Optional[Context] = None, num_states: int = 1, num_subspace_vec: int = 5,
                  iteration_max: int = 500, convergence_E: float = 1e-6, precision: int = _lib.PQG_FP64,
                  exact_diag: bool = False, spin_penalty_type: Optional[str] = None,
-                 spin_penalty: Optional[list] = None, expected_S2: Optional[list] = None):
+                 spin_penalty: Optional[list] = None, expected_S2: Optional[list] = None,
+                 screening_threshold: float = 0.0):
+        # ci_keywords spin_penalty_type / spin_penalty / expected_S2 (calculation.cpp:280-312)
+        if spin_penalty_type not in (None, "first_order", "second_order"):
+            raise ValueError("spin_penalty_type can only be first_order or second_order")
+        if spin_penalty_type is not None and not exact_diag:
+            raise ValueError("the spin penalty is added to the explicit Hamiltonian: use exact_diag")
+        # ci_keywords screening_threshold (determinant_set.hpp:197, calculation.cpp:370-372): the
+        # explicit H drops |H_ij| <= threshold (explicit_ham.cpp:78-80); the matrix-free sigma
+        # has no stored H to screen
+        if float(screening_threshold) != 0.0 and not exact_diag:
+            raise ValueError("screening_threshold applies to the explicit Hamiltonian: use exact_diag "
+                             "(or DetSet.create_ham_csr for the sparse form)")
         self.prob = prob
         self.detset = DetSet(prob, ctx)
         self.num_states = num_states
@@ -294,16 +306,12 @@ class EPCI:
         self.convergence_E = convergence_E
         self.precision = precision
         self.exact_diag = exact_diag
-        # ci_keywords spin_penalty_type / spin_penalty / expected_S2 (calculation.cpp:280-312)
-        if spin_penalty_type not in (None, "first_order", "second_order"):
-            raise ValueError("spin_penalty_type can only be first_order or second_order")
         self.first_order_spin_penalty = spin_penalty_type == "first_order"
         self.second_order_spin_penalty = spin_penalty_type == "second_order"
         nsp = len(prob.species)
         self.spin_penalty = list(spin_penalty) if spin_penalty is not None else [0.0] * nsp
         self.expected_S2 = list(expected_S2) if expected_S2 is not None else [0.0] * nsp
-        if spin_penalty_type is not None and not exact_diag:
-            raise ValueError("the spin penalty is added to the explicit Hamiltonian: use exact_diag")
+        self.screening_threshold = float(screening_threshold)
         self.constant_shift = prob.e_shift  # E_nuc + frozen-core energy (epci.cpp:584-594)
         self.energies = None
         self.C_ci = None
@@ -314,6 +322,8 @@ class EPCI:
             # epci.cpp:622-662: dense H (+ the spin penalty), all eigenpairs on the host, the
             # lowest num_states kept
             H = self.detset.create_ham_dense()
+            if self.screening_threshold != 0.0:
+                H[np.abs(H) <= self.screening_threshold] = 0.0  # ham.prune(threshold, 1.0)
             if self.first_order_spin_penalty or self.second_order_spin_penalty:
                 self.add_spin_penalty(H)
             w, v = np.linalg.eigh(H)

@@ -83,3 +83,42 @@ def test_spin_penalty_host_logic_follows_reference_loop():
     calls.clear()
     H = make("second_order", [0.0, 3.0], [5.0, 6.0]).add_spin_penalty(np.zeros((2, 2)))
     assert calls == [] and np.all(H == 0.0)
+
+
+def test_exact_diag_screening_prunes_before_the_penalty():
+    """create_ham prunes |H_ij| <= screening_threshold (explicit_ham.cpp:78-80) before
+    create_S_sq_penalty adds the penalty (epci.cpp:623-631); the matrix-free path refuses a
+    threshold."""
+    import numpy as np
+    import pytest
+
+    from polyquant_amd.ci import EPCI
+
+    H0 = np.array([[-1.0, 0.01, 0.3], [0.01, -0.5, 0.02], [0.3, 0.02, 0.2]])
+
+    class FakeDetSet:
+        def create_ham_dense(self):
+            return H0.copy()
+
+        def create_s2_penalty_dense(self, part, expected, power):
+            return np.full((3, 3), 0.001)
+
+    class Prob:
+        species = [object()]
+
+    ci = object.__new__(EPCI)
+    ci.prob, ci.detset = Prob(), FakeDetSet()
+    ci.exact_diag, ci.num_states, ci.constant_shift = True, 2, 0.0
+    ci.screening_threshold = 0.05
+    ci.first_order_spin_penalty, ci.second_order_spin_penalty = True, False
+    ci.spin_penalty, ci.expected_S2 = [2.0], [0.0]
+    E = ci.run()
+    Hp = np.where(np.abs(H0) <= 0.05, 0.0, H0) + 2.0 * 0.001
+    assert np.allclose(E, np.linalg.eigvalsh(Hp)[:2], atol=1e-14)
+    # the argument checks run before any GPU object is made
+    with pytest.raises(ValueError, match="screening_threshold"):
+        EPCI(Prob(), screening_threshold=0.1)
+    with pytest.raises(ValueError, match="spin penalty"):
+        EPCI(Prob(), spin_penalty_type="first_order", spin_penalty=[1.0])
+    with pytest.raises(ValueError, match="first_order or second_order"):
+        EPCI(Prob(), exact_diag=True, spin_penalty_type="third_order")
