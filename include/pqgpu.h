@@ -128,6 +128,12 @@ pqg_status pqg_ci_dense_h(pqg_ci* ci, double* H, int64_t ldh, int on_device);
  * NULL for *nnz (and rowptr[N + 1] when rowptr != NULL), then with cap >= *nnz to fill.      */
 pqg_status pqg_ci_csr(pqg_ci* ci, double screening, int32_t* rowptr, int32_t* cols, double* vals, int64_t cap,
                       int64_t* nnz);
+/* build_matrix = true (determinant_set.hpp:196): enable != 0 stores the explicit H of
+ * pqg_ci_csr (same entries, same screening) on the device, both triangles in internal order, and
+ * from then on pqg_ci_sigma and pqg_ci_davidson with PQG_FP64 multiply by it, as the reference's
+ * SparseSymMatProd does (epci.cpp:664-682); PQG_MIXED is refused while it is stored.
+ * enable = 0 frees it and returns to the matrix-free sigma.  nnz (may be NULL): stored entries.  */
+pqg_status pqg_ci_set_explicit(pqg_ci* ci, int enable, double screening, int64_t* nnz);
 /* create_S_sq_minus_expected_S_sq_matrix_singleshot (s2.cpp:156-255) as a dense matrix, and
  * its square for the second-order penalty (s2.cpp:141-147): P = S^2 - expected * I of species
  * part (power 1) or P^2 (power 2, summed over the determinants of the space), N x N reference

@@ -181,6 +181,14 @@ class DetSet:
         check(load().pqg_ci_csr(self.h, float(screening), None, _ptr(cols), _ptr(vals), nnz.value, ct.byref(nnz)))
         return rowptr, cols, vals
 
+    def set_explicit(self, enable: bool = True, screening: float = 0.0) -> int:
+        """build_matrix = true: store the explicit H (create_ham's entries and screening) on the
+        GPU, so sigma and the Davidson solve (FP64) multiply by it; False returns to the
+        matrix-free sigma.  Returns the number of stored entries (both triangles)."""
+        nnz = ct.c_int64()
+        check(load().pqg_ci_set_explicit(self.h, int(enable), float(screening), ct.byref(nnz)))
+        return nnz.value
+
     def create_s2_penalty_dense(self, part: int, expected: float = 0.0, power: int = 1) -> np.ndarray:
         """create_S_sq_minus_expected_S_sq_matrix_singleshot (src/ci/detset/s2.cpp:156-255) as a
         dense N x N matrix P = S^2 - expected I of species part (power 1), or P @ P (power 2, the
@@ -286,7 +294,7 @@ class EPCI:
                  iteration_max: int = 500, convergence_E: float = 1e-6, precision: int = _lib.PQG_FP64,
                  exact_diag: bool = False, spin_penalty_type: Optional[str] = None,
                  spin_penalty: Optional[list] = None, expected_S2: Optional[list] = None,
-                 screening_threshold: float = 0.0):
+                 screening_threshold: float = 0.0, build_matrix: bool = False):
         # ci_keywords spin_penalty_type / spin_penalty / expected_S2 (calculation.cpp:280-312)
         if spin_penalty_type not in (None, "first_order", "second_order"):
             raise ValueError("spin_penalty_type can only be first_order or second_order")
@@ -295,9 +303,10 @@ class EPCI:
         # ci_keywords screening_threshold (determinant_set.hpp:197, calculation.cpp:370-372): the
         # explicit H drops |H_ij| <= threshold (explicit_ham.cpp:78-80); the matrix-free sigma
         # has no stored H to screen
-        if float(screening_threshold) != 0.0 and not exact_diag:
-            raise ValueError("screening_threshold applies to the explicit Hamiltonian: use exact_diag "
-                             "(or DetSet.create_ham_csr for the sparse form)")
+        # explicit H for the Davidson path (build_matrix, or a screening threshold, which only an
+        # explicit H can carry); FP64 only
+        if (build_matrix or float(screening_threshold) != 0.0) and not exact_diag and precision != _lib.PQG_FP64:
+            raise ValueError("the stored explicit Hamiltonian is FP64 only")
         self.prob = prob
         self.detset = DetSet(prob, ctx)
         self.num_states = num_states
@@ -312,6 +321,9 @@ class EPCI:
         self.spin_penalty = list(spin_penalty) if spin_penalty is not None else [0.0] * nsp
         self.expected_S2 = list(expected_S2) if expected_S2 is not None else [0.0] * nsp
         self.screening_threshold = float(screening_threshold)
+        self.build_matrix = bool(build_matrix) or self.screening_threshold != 0.0
+        if self.build_matrix and not exact_diag:
+            self.detset.set_explicit(True, self.screening_threshold)
         self.constant_shift = prob.e_shift  # E_nuc + frozen-core energy (epci.cpp:584-594)
         self.energies = None
         self.C_ci = None

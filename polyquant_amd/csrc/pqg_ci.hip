@@ -791,10 +791,11 @@ __device__ __forceinline__ int64_t csr_ref_of(const CsrArgs& c, int64_t addr) {
   return r;
 }
 
+// Every determinant address connected to the one at internal address ia by a single or double
+// excitation within one string, or by singles in two strings (targets outside a truncated set
+// skipped; targets outside a truncated product space are the caller's to drop).
 template <class F>
-__device__ void csr_row(const CsrArgs& c, int64_t r, F&& emit) {
-  const ElemArgs& a = c.e;
-  const int64_t ia = ref_to_int(c.p, r);
+__device__ void for_connected(const ElemArgs& a, int64_t ia, F&& visit) {
   int idx[kSlots];
   uint64_t bits[kSlots];
   int64_t rr = ia;
@@ -803,16 +804,6 @@ __device__ void csr_row(const CsrArgs& c, int64_t r, F&& emit) {
     rr /= a.sl[t].nstr;
     bits[t] = a.sl[t].bits[idx[t]];
   }
-  {
-    const double v = a.Hd[ia];
-    if (c.screen <= 0.0 || fabs(v) > c.screen) emit(r, v);
-  }
-  auto visit = [&](int64_t addr) {
-    const int64_t rj = csr_ref_of(c, addr);
-    if (rj <= r) return;  // lower triangle, or not in the space (-1)
-    const double v = element_at(a, ia, addr);
-    if (v != 0.0 && (c.screen <= 0.0 || fabs(v) > c.screen)) emit(rj, v);
-  };
   for (int t = 0; t < a.nslot; t++) {
     const SlotDev& s = a.sl[t];
     for_singles(bits[t], s.n_orb, [&](int, int, int, int, int, uint64_t J) {
@@ -835,6 +826,73 @@ __device__ void csr_row(const CsrArgs& c, int64_t r, F&& emit) {
         });
       });
     }
+  }
+}
+
+__device__ __forceinline__ bool csr_keep(const CsrArgs& c, double v, bool diag) {
+  return (diag || v != 0.0) && (c.screen <= 0.0 || fabs(v) > c.screen);
+}
+
+// Row r (reference order) of the upper triangle: (reference column, value).
+template <class F>
+__device__ void csr_row(const CsrArgs& c, int64_t r, F&& emit) {
+  const int64_t ia = ref_to_int(c.p, r);
+  if (csr_keep(c, c.e.Hd[ia], true)) emit(r, c.e.Hd[ia]);
+  for_connected(c.e, ia, [&](int64_t addr) {
+    const int64_t rj = csr_ref_of(c, addr);
+    if (rj <= r) return;  // lower triangle, or not in the space (-1)
+    const double v = element_at(c.e, ia, addr);
+    if (csr_keep(c, v, false)) emit(rj, v);
+  });
+}
+
+// Row ia (internal order) of the symmetric matrix with the same entries: (internal column, value),
+// each value from the lower reference address of the pair as in csr_row.
+template <class F>
+__device__ void xh_row(const CsrArgs& c, int64_t ia, F&& emit) {
+  const int64_t r = csr_ref_of(c, ia);
+  if (r < 0) return;
+  if (csr_keep(c, c.e.Hd[ia], true)) emit(ia, c.e.Hd[ia]);
+  for_connected(c.e, ia, [&](int64_t addr) {
+    const int64_t rj = csr_ref_of(c, addr);
+    if (rj < 0) return;
+    const double v = rj > r ? element_at(c.e, ia, addr) : element_at(c.e, addr, ia);
+    if (csr_keep(c, v, false)) emit(addr, v);
+  });
+}
+
+__global__ void k_xh_count(CsrArgs c, int64_t n, int64_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t m = 0;
+  xh_row(c, r, [&](int64_t, double) { ++m; });
+  cnt[r] = m;
+}
+
+__global__ void k_xh_fill(CsrArgs c, int64_t n, const int64_t* __restrict__ rowptr, int32_t* __restrict__ cols,
+                          double* __restrict__ vals) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t q = rowptr[r];
+  xh_row(c, r, [&](int64_t j, double v) {
+    cols[q] = (int32_t)j;
+    vals[q] = v;
+    ++q;
+  });
+}
+
+// S[:, v] = H C[:, v] over the stored rows, one thread per row, every vector (fixed order).
+__global__ void k_xh_spmv(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                          const double* __restrict__ val, int64_t n, const double* __restrict__ C, double* __restrict__ S,
+                          int k, int64_t ld) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t q0 = ptr[r], q1 = ptr[r + 1];
+  for (int v = 0; v < k; v++) {
+    const double* x = C + (int64_t)v * ld;
+    double s = 0.0;
+    for (int64_t q = q0; q < q1; q++) s += val[q] * x[col[q]];
+    S[(int64_t)v * ld + r] = s;
   }
 }
 
@@ -1521,6 +1579,16 @@ static void transpose(const T* in, int64_t ldi, T* out, int64_t ldo, int64_t row
 template <typename T>
 static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k, int64_t ld, hipStream_t st) {
   ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "sigma_f64" : "sigma_f32", k);
+  if (ci->xh) {
+    if constexpr (std::is_same_v<T, double>) {
+      k_xh_spmv<<<grid1(ci->n_int, 256), 256, 0, st>>>(ci->xh_ptr.get(), ci->xh_col.get(), ci->xh_val.get(), ci->n_int,
+                                                        C, S, k, ld);
+      PQG_HIP(hipGetLastError());
+      return;
+    } else {
+      PQG_REQUIRE(false, PQG_ERR_UNSUPPORTED, "the stored explicit Hamiltonian is FP64 only (PQG_FP64)");
+    }
+  }
   if (ci->tensor.enabled) {
     tensor_sigma(ci, C, S, k, ld, st);
     return;
@@ -1690,6 +1758,36 @@ static void dense_fill(pqg_ci* ci, double* M, int64_t ldm, int on_device, L&& la
   PQG_HIP(hipStreamSynchronize(st));
 }
 
+// Device lookups of CsrArgs (kept alive by the caller): the per-set internal -> reference maps
+// of a full CI, or the inverted address table of a truncated space.
+struct CsrLookups {
+  DevBuf<int64_t> inv;
+  DevBuf<int32_t> i2r[kSlots];
+};
+
+static CsrArgs csr_setup(pqg_ci* ci, double screening, CsrLookups& lk) {
+  hipStream_t st = ci->stream;
+  CsrArgs c{};
+  c.e = elem_args(ci);
+  c.p = perm_args(ci);
+  c.screen = screening;
+  if (ci->fci) {
+    for (int t = 0; t < ci->nslot; t++) {
+      const Slot& sl = ci->slot[t];
+      lk.i2r[t].reset(ci->ctx, sl.nstr);
+      PQG_HIP(hipMemcpyAsync(lk.i2r[t].get(), sl.int2ref.data(), (size_t)sl.nstr * 4, hipMemcpyHostToDevice, st));
+      c.i2r[t] = lk.i2r[t].get();
+    }
+  } else {
+    lk.inv.reset(ci->ctx, ci->n_int);
+    k_fill_i64<<<grid1(ci->n_int, 256), 256, 0, st>>>(lk.inv.get(), ci->n_int, -1);
+    k_inv_addr<<<grid1(ci->n_ref, 256), 256, 0, st>>>(ci->ref_addr.get(), ci->n_ref, lk.inv.get());
+    PQG_HIP(hipGetLastError());
+    c.inv = lk.inv.get();
+  }
+  return c;
+}
+
 extern "C" {
 
 pqg_status pqg_ci_create(pqg_ctx* ctx, const pqg_ci_desc* desc, pqg_ci** out) {
@@ -1753,33 +1851,51 @@ pqg_status pqg_ci_dense_h(pqg_ci* ci, double* H, int64_t ldh, int on_device) {
   });
 }
 
+pqg_status pqg_ci_set_explicit(pqg_ci* ci, int enable, double screening, int64_t* nnz) {
+  return guard([&] {
+    PQG_REQUIRE(ci, PQG_ERR_ARG, "NULL argument");
+    ci->xh = false;
+    ci->xh_ptr.reset(ci->ctx, 0);
+    ci->xh_col.reset(ci->ctx, 0);
+    ci->xh_val.reset(ci->ctx, 0);
+    if (nnz) *nnz = 0;
+    if (!enable) return;
+    PQG_REQUIRE(ci->n_int < (int64_t(1) << 31) - 1, PQG_ERR_UNSUPPORTED, "N above 2^31 - 2 (int32 indices)");
+    hipStream_t st = ci->stream;
+    const int64_t n = ci->n_int;
+    CsrLookups lk;
+    const CsrArgs c = csr_setup(ci, screening, lk);
+    DevBuf<int64_t> cnt(ci->ctx, n + 1);
+    ci->xh_ptr.reset(ci->ctx, n + 1);
+    k_fill_i64<<<grid1(n + 1, 256), 256, 0, st>>>(cnt.get(), n + 1, 0);
+    k_xh_count<<<grid1(n, 128), 128, 0, st>>>(c, n, cnt.get());
+    PQG_HIP(hipGetLastError());
+    size_t tb = 0;
+    PQG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.get(), ci->xh_ptr.get(), (int)(n + 1), st));
+    DevBuf<uint8_t> tmp(ci->ctx, std::max<size_t>(tb, 1));
+    PQG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tb, cnt.get(), ci->xh_ptr.get(), (int)(n + 1), st));
+    int64_t total = 0;
+    PQG_HIP(hipMemcpyAsync(&total, ci->xh_ptr.get() + n, 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+    ci->xh_col.reset(ci->ctx, std::max<int64_t>(total, 1));
+    ci->xh_val.reset(ci->ctx, std::max<int64_t>(total, 1));
+    k_xh_fill<<<grid1(n, 128), 128, 0, st>>>(c, n, ci->xh_ptr.get(), ci->xh_col.get(), ci->xh_val.get());
+    PQG_HIP(hipGetLastError());
+    PQG_HIP(hipStreamSynchronize(st));
+    ci->xh = true;
+    if (nnz) *nnz = total;
+  });
+}
+
 pqg_status pqg_ci_csr(pqg_ci* ci, double screening, int32_t* rowptr, int32_t* cols, double* vals, int64_t cap,
                       int64_t* nnz) {
   return guard([&] {
     PQG_REQUIRE(ci && nnz, PQG_ERR_ARG, "NULL argument");
-    PQG_REQUIRE(ci->n_ref < (int64_t(1) << 31), PQG_ERR_UNSUPPORTED, "N above 2^31 - 1 (int32 CSR indices)");
+    PQG_REQUIRE(ci->n_ref < (int64_t(1) << 31) - 1, PQG_ERR_UNSUPPORTED, "N above 2^31 - 2 (int32 CSR indices)");
     hipStream_t st = ci->stream;
     const int64_t n = ci->n_ref;
-    CsrArgs c{};
-    c.e = elem_args(ci);
-    c.p = perm_args(ci);
-    c.screen = screening;
-    DevBuf<int64_t> inv;
-    DevBuf<int32_t> i2r[kSlots];
-    if (ci->fci) {
-      for (int t = 0; t < ci->nslot; t++) {
-        const Slot& sl = ci->slot[t];
-        i2r[t].reset(ci->ctx, sl.nstr);
-        PQG_HIP(hipMemcpyAsync(i2r[t].get(), sl.int2ref.data(), (size_t)sl.nstr * 4, hipMemcpyHostToDevice, st));
-        c.i2r[t] = i2r[t].get();
-      }
-    } else {
-      inv.reset(ci->ctx, ci->n_int);
-      k_fill_i64<<<grid1(ci->n_int, 256), 256, 0, st>>>(inv.get(), ci->n_int, -1);
-      k_inv_addr<<<grid1(n, 256), 256, 0, st>>>(ci->ref_addr.get(), n, inv.get());
-      PQG_HIP(hipGetLastError());
-      c.inv = inv.get();
-    }
+    CsrLookups lk;
+    const CsrArgs c = csr_setup(ci, screening, lk);
     // counts -> row pointers (exclusive scan over n + 1 entries, the last count 0)
     DevBuf<int64_t> cnt(ci->ctx, n + 1), ptr(ci->ctx, n + 1);
     k_fill_i64<<<grid1(n + 1, 256), 256, 0, st>>>(cnt.get(), n + 1, 0);

@@ -115,6 +115,12 @@ struct pqg_ci {
   pqg::DevBuf<char> scratch;
   // boundary staging
   pqg::DevBuf<double> stage_in, stage_out;
+  // stored explicit H (pqg_ci_set_explicit): both triangles, internal order, rows over every
+  // internal address (empty outside the space); sigma in FP64 is then its product
+  bool xh = false;
+  pqg::DevBuf<int64_t> xh_ptr;
+  pqg::DevBuf<int32_t> xh_col;
+  pqg::DevBuf<double> xh_val;
 };
 
 namespace pqg {

@@ -358,3 +358,48 @@ def test_csr_hamiltonian_screening(ctx):
     assert np.all(np.abs(vals) > s)
     kept = _csr_dense(rowptr, cols, vals, n)
     assert np.array_equal(kept, np.where(np.abs(full) > s, full, 0.0))
+
+
+@pytest.mark.parametrize("name", ["fci_open_shell_8o_4a2b", "fci_unrestricted_6o_3a2b", "cisd_h2o_layout_7o_5a5b",
+                                  "ep_fci_5e_2a2b_4p_2a1b", "ep_truncated_psh_layout", "fci_1p_strings_11o_1a1b"])
+def test_stored_explicit_h_sigma_and_davidson(name, ctx):
+    """build_matrix = true: the stored explicit H (pqg_ci_set_explicit) holds both triangles of
+    the CSR entries; its sigma equals the matrix-free sigma, its Davidson roots the matrix-free
+    ones; mixed precision is refused while it is stored, and switching it off restores the
+    matrix-free sigma."""
+    from polyquant_amd import _lib
+
+    prob, ds, oc = _pair(name, ctx)
+    n = ds.N_dets
+    C = np.random.default_rng(9).standard_normal((n, 3))
+    S_free = ds.sigma(C)
+    E_free, _, _ = ds.davidson(2, tol=1e-9)
+    rowptr, cols, vals = ds.create_ham_csr()
+    nnz = ds.set_explicit(True)
+    assert nnz == 2 * len(cols) - n
+    scale = max(1.0, np.max(np.abs(S_free)))
+    assert np.max(np.abs(ds.sigma(C) - S_free)) < 1e-12 * scale
+    E, _, rep = ds.davidson(2, tol=1e-9)
+    assert rep.converged == 1 and np.max(np.abs(E - E_free)) < 1e-8
+    with pytest.raises(_lib.PqgError):
+        ds.sigma(C, precision=_lib.PQG_MIXED)
+    ds.set_explicit(False)
+    assert np.max(np.abs(ds.sigma(C, precision=_lib.PQG_MIXED) - S_free)) < 2e-5 * scale
+
+
+def test_screened_explicit_h_davidson_matches_pruned_oracle(ctx):
+    """screening_threshold with build_matrix (explicit_ham.cpp:78-80, epci.cpp:664-682): the
+    Davidson roots on the stored, screened H equal the lowest eigenvalues of the oracle's H with
+    |H_ij| <= threshold removed, and the exact_diag path with the same threshold."""
+    from polyquant_amd.ci import EPCI
+
+    prob, ds, oc = _pair("cisd_h2o_layout_7o_5a5b", ctx)
+    s = 0.05
+    Hor = oc.dense()
+    ref = np.linalg.eigvalsh(np.where(np.abs(Hor) <= s, 0.0, Hor))[:3] + prob.e_shift
+    E = EPCI(prob, ctx, num_states=3, num_subspace_vec=10, convergence_E=1e-10, screening_threshold=s).run()
+    assert np.max(np.abs(E - ref)) < 1e-8
+    Ex = EPCI(prob, ctx, num_states=3, exact_diag=True, screening_threshold=s).run()
+    assert np.max(np.abs(Ex - ref)) < 1e-9
+    Hfull = np.linalg.eigvalsh(Hor)[:3] + prob.e_shift
+    assert np.max(np.abs(Hfull - ref)) > 1e-6  # the threshold changes the answer

@@ -87,8 +87,8 @@ def test_spin_penalty_host_logic_follows_reference_loop():
 
 def test_exact_diag_screening_prunes_before_the_penalty():
     """create_ham prunes |H_ij| <= screening_threshold (explicit_ham.cpp:78-80) before
-    create_S_sq_penalty adds the penalty (epci.cpp:623-631); the matrix-free path refuses a
-    threshold."""
+    create_S_sq_penalty adds the penalty (epci.cpp:623-631); the stored explicit H of the
+    Davidson path is FP64 only."""
     import numpy as np
     import pytest
 
@@ -116,8 +116,8 @@ def test_exact_diag_screening_prunes_before_the_penalty():
     Hp = np.where(np.abs(H0) <= 0.05, 0.0, H0) + 2.0 * 0.001
     assert np.allclose(E, np.linalg.eigvalsh(Hp)[:2], atol=1e-14)
     # the argument checks run before any GPU object is made
-    with pytest.raises(ValueError, match="screening_threshold"):
-        EPCI(Prob(), screening_threshold=0.1)
+    with pytest.raises(ValueError, match="FP64 only"):
+        EPCI(Prob(), screening_threshold=0.1, precision=1)
     with pytest.raises(ValueError, match="spin penalty"):
         EPCI(Prob(), spin_penalty_type="first_order", spin_penalty=[1.0])
     with pytest.raises(ValueError, match="first_order or second_order"):
