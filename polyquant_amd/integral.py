@@ -169,3 +169,32 @@ def active_two_body(out4: np.ndarray, nmo: int, first: int, nact: int) -> np.nda
     ia, ja = pairs(nact, first)
     idx = np.maximum(ia, ja) * (np.maximum(ia, ja) + 1) // 2 + np.minimum(ia, ja)
     return np.ascontiguousarray(np.asarray(out4)[np.ix_(idx, idx)])
+
+
+def frozen_core_problem(species, C, kinetic, nuclear, eri_ao, e_nuc: float, frozen_core, deleted_virtual,
+                        nroots: int = 1, ctx: Optional[Context] = None):
+    """calculate_fc_energy, calculate_mo_1_body_integrals and the MO two-body transform with
+    frozen core and deleted virtuals (epci.cpp:47-129, 575-594; integral.cpp:166-205, 207-477,
+    565-610) for same-species AO integrals: returns the active-space CIProblem whose e_shift is
+    e_nuc plus the frozen-core energies, and those energies.  species: objects with .charge,
+    .restricted, .n_alpha, .n_beta (all particles, core included) and .ex_level; C[a][s]: the
+    MO coefficients (one restricted block); eri_ao[(0, 0)]: 8-fold packed AO integrals."""
+    from .workloads import CIProblem, Species
+
+    if len(species) != 1:
+        raise ValueError("frozen_core_problem takes one species (two-species AO blocks: fc_ints_block)")
+    if len(C[0]) != 1:
+        raise ValueError("unrestricted frozen core needs the mixed-spin transform (not built)")
+    sp = species[0]
+    sp.num_parts = sp.n_alpha + sp.n_beta
+    fc_dm, fci, e_fc = calculate_fc_energy(species, C, frozen_core, eri_ao, kinetic, nuclear, ctx)
+    h = calculate_mo_1_body_integrals(C, frozen_core, deleted_virtual, fci)
+    nmo = np.asarray(C[0][0]).shape[1]
+    nact = nmo - frozen_core[0] - deleted_virtual[0]
+    out4, _ = transform_mo_2_body_integrals(eri_ao[(0, 0)], np.asarray(C[0][0]), ctx=ctx)
+    eri = {(0, 0, 0, 0): active_two_body(out4, nmo, frozen_core[0], nact)}
+    act = Species(nact, sp.n_alpha - frozen_core[0], sp.n_beta - frozen_core[0], sp.charge, 1,
+                  getattr(sp, "ex_level", None))
+    prob = CIProblem(species=[act], h=[h[0]], eri=eri, nroots=nroots, e_shift=e_nuc + sum(e_fc),
+                     name=f"frozen_core_{frozen_core[0]}_deleted_{deleted_virtual[0]}")
+    return prob, e_fc

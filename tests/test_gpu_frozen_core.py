@@ -95,3 +95,32 @@ def test_frozen_core_water_equals_restricted_full_space(ctx):
     assert np.max(np.abs(E - ref)) < 1e-9, (E, ref)
     # the frozen-core energy is the energy of the doubly occupied core alone
     assert e_fc[0] < 0.0
+
+
+def test_frozen_core_and_deleted_virtual_problem(ctx):
+    """integral.frozen_core_problem with the oxygen 1s frozen and the highest orbital deleted:
+    the 5-orbital 4a4b full CI equals the 7-orbital full-CI Hamiltonian restricted to the
+    determinants with orbital 0 doubly occupied and orbital 6 empty in both strings."""
+    from types import SimpleNamespace
+
+    from oracle import refcpu
+    from polyquant_amd import integral, molint
+    from polyquant_amd import workloads as wl
+    from polyquant_amd.ci import EPCI
+
+    print("[gpu] case frozen core + deleted virtual water sto-3g", flush=True)
+    w = molint.water("sto-3g")
+    sp = [SimpleNamespace(charge=-1.0, restricted=True, n_alpha=w["n_alpha"], n_beta=w["n_beta"], ex_level=None)]
+    prob, e_fc = integral.frozen_core_problem(sp, [[w["C"]]], [w["T"]], [w["V"]],
+                                              {(0, 0): integral.pack8(w["eri_ao"])}, w["E_nuc"], [1], [1],
+                                              nroots=3, ctx=ctx)
+    assert prob.species[0].n_orb == 5 and prob.species[0].n_alpha == 4
+    E = EPCI(prob, ctx, num_states=3, exact_diag=True).run()
+    full = wl.water_problem("sto-3g")
+    oc = refcpu.OracleCI(full)
+    Sa, Sb, U = oc.strings(0, 0), oc.strings(0, 1), oc.unfold_all()
+    ok = lambda x: (int(x) & 1) and not (int(x) >> 6) & 1  # noqa: E731
+    keep = [i for i in range(oc.N) if ok(Sa[U[i, 0]]) and ok(Sb[U[i, 1]])]
+    assert len(keep) == 25
+    ref = np.linalg.eigvalsh(oc.dense()[np.ix_(keep, keep)])[:3] + full.e_shift
+    assert np.max(np.abs(E - ref)) < 1e-9, (E, ref)
