@@ -134,6 +134,11 @@ pqg_status pqg_ci_csr(pqg_ci* ci, double screening, int32_t* rowptr, int32_t* co
  * SparseSymMatProd does (epci.cpp:664-682); PQG_MIXED is refused while it is stored.
  * enable = 0 frees it and returns to the matrix-free sigma.  nnz (may be NULL): stored entries.  */
 pqg_status pqg_ci_set_explicit(pqg_ci* ci, int enable, double screening, int64_t* nnz);
+/* create_S_sq_penalty (s2.cpp:126-153) on the stored explicit H (pqg_ci_set_explicit first):
+ * sigma then adds alpha (S^2 - expected) C of species part (power 1, first order) or
+ * alpha (S^2 - expected)^2 C (power 2, second order, applied as two products).  Penalties
+ * accumulate over calls; pqg_ci_set_explicit clears them.                                     */
+pqg_status pqg_ci_add_spin_penalty(pqg_ci* ci, int part, double expected, int power, double alpha);
 /* create_S_sq_minus_expected_S_sq_matrix_singleshot (s2.cpp:156-255) as a dense matrix, and
  * its square for the second-order penalty (s2.cpp:141-147): P = S^2 - expected * I of species
  * part (power 1) or P^2 (power 2, summed over the determinants of the space), N x N reference

@@ -189,6 +189,11 @@ class DetSet:
         check(load().pqg_ci_set_explicit(self.h, int(enable), float(screening), ct.byref(nnz)))
         return nnz.value
 
+    def add_spin_penalty(self, part: int, expected: float, power: int, alpha: float):
+        """Adds alpha (S^2 - expected)^power of species part to the stored explicit H
+        (set_explicit first); sigma and the Davidson solve then include it."""
+        check(load().pqg_ci_add_spin_penalty(self.h, part, float(expected), power, float(alpha)))
+
     def create_s2_penalty_dense(self, part: int, expected: float = 0.0, power: int = 1) -> np.ndarray:
         """create_S_sq_minus_expected_S_sq_matrix_singleshot (src/ci/detset/s2.cpp:156-255) as a
         dense N x N matrix P = S^2 - expected I of species part (power 1), or P @ P (power 2, the
@@ -298,8 +303,8 @@ class EPCI:
         # ci_keywords spin_penalty_type / spin_penalty / expected_S2 (calculation.cpp:280-312)
         if spin_penalty_type not in (None, "first_order", "second_order"):
             raise ValueError("spin_penalty_type can only be first_order or second_order")
-        if spin_penalty_type is not None and not exact_diag:
-            raise ValueError("the spin penalty is added to the explicit Hamiltonian: use exact_diag")
+        if spin_penalty_type is not None and not exact_diag and precision != _lib.PQG_FP64:
+            raise ValueError("the stored explicit Hamiltonian is FP64 only")
         # ci_keywords screening_threshold (determinant_set.hpp:197, calculation.cpp:370-372): the
         # explicit H drops |H_ij| <= threshold (explicit_ham.cpp:78-80); the matrix-free sigma
         # has no stored H to screen
@@ -321,9 +326,12 @@ class EPCI:
         self.spin_penalty = list(spin_penalty) if spin_penalty is not None else [0.0] * nsp
         self.expected_S2 = list(expected_S2) if expected_S2 is not None else [0.0] * nsp
         self.screening_threshold = float(screening_threshold)
-        self.build_matrix = bool(build_matrix) or self.screening_threshold != 0.0
+        # the penalty is added to the explicit H (epci.cpp:623-631), so it implies build_matrix
+        self.build_matrix = bool(build_matrix) or self.screening_threshold != 0.0 or spin_penalty_type is not None
         if self.build_matrix and not exact_diag:
             self.detset.set_explicit(True, self.screening_threshold)
+            if self.first_order_spin_penalty or self.second_order_spin_penalty:
+                self.add_spin_penalty(None)
         self.constant_shift = prob.e_shift  # E_nuc + frozen-core energy (epci.cpp:584-594)
         self.energies = None
         self.C_ci = None
@@ -352,10 +360,11 @@ class EPCI:
         self.report = rep
         return self.energies
 
-    def add_spin_penalty(self, H: np.ndarray):
+    def add_spin_penalty(self, H: Optional[np.ndarray]):
         """create_S_sq_penalty (src/ci/detset/s2.cpp:126-153): H += a (S^2) (first order,
         expected 0) or H += a (S^2 - expected)^2 (second order) per species, a = spin_penalty of
-        that species.  As in the reference, the species counter advances only past a nonzero
+        that species, into the dense H (exact_diag) or, for H None, onto the stored explicit H
+        of the Davidson path.  As in the reference, the species counter advances only past a nonzero
         penalty, so a zero penalty also switches off every later species."""
         power = 1 if self.first_order_spin_penalty else 2
         idx = 0
@@ -364,7 +373,10 @@ class EPCI:
             if alpha == 0.0:
                 continue
             expected = 0.0 if power == 1 else self.expected_S2[idx]
-            H += alpha * self.detset.create_s2_penalty_dense(idx, expected, power)
+            if H is None:  # onto the stored explicit H of the Davidson path
+                self.detset.add_spin_penalty(idx, expected, power, alpha)
+            else:
+                H += alpha * self.detset.create_s2_penalty_dense(idx, expected, power)
             idx += 1
         return H
 

@@ -1003,6 +1003,75 @@ __global__ void k_dense_s2(ElemArgs a, PermArgs p, S2PenArgs q, int64_t n, int64
   S[i + (int64_t)blockIdx.y * lds] = v;
 }
 
+// Row ia (internal order) of P = S^2 - e of one species: the diagonal and the spin flips of I
+// that lie in the space, the only nonzero entries (s2_pen_elem).
+template <class F>
+__device__ void xp_row(const ElemArgs& a, const S2PenArgs& q, int64_t ia, F&& emit) {
+  if (q.valid && !q.valid[ia]) return;
+  int ii[kSlots];
+  uint64_t bi[kSlots];
+  int64_t rr = ia;
+  for (int t = a.nslot - 1; t >= 0; --t) {
+    ii[t] = (int)(rr % a.sl[t].nstr);
+    rr /= a.sl[t].nstr;
+    bi[t] = a.sl[t].bits[ii[t]];
+  }
+  emit(ia, s2_pen_elem(bi, bi, a.nslot, q.ta, q.tb, q.e));
+  const uint64_t A = bi[q.ta], B = bi[q.tb];
+  const SlotDev &sa = a.sl[q.ta], &sb = a.sl[q.tb];
+  for (uint64_t xa = A & ~B; xa; xa &= xa - 1) {
+    const int oi = ctz64(xa);
+    for (uint64_t xb = B & ~A; xb; xb &= xb - 1) {
+      const int oj = ctz64(xb);
+      uint64_t bk[kSlots];
+      for (int t = 0; t < a.nslot; t++) bk[t] = bi[t];
+      bk[q.ta] = A ^ (1ull << oi) ^ (1ull << oj);
+      bk[q.tb] = B ^ (1ull << oj) ^ (1ull << oi);
+      const int ka = str_index(sa, sa.rtab, bk[q.ta]), kb = str_index(sb, sb.rtab, bk[q.tb]);
+      if (ka < 0 || kb < 0) continue;
+      const int64_t addr = ia + (int64_t)(ka - ii[q.ta]) * a.stride[q.ta] + (int64_t)(kb - ii[q.tb]) * a.stride[q.tb];
+      if (q.valid && !q.valid[addr]) continue;
+      emit(addr, s2_pen_elem(bi, bk, a.nslot, q.ta, q.tb, q.e));
+    }
+  }
+}
+
+__global__ void k_xp_count(ElemArgs a, S2PenArgs q, int64_t n, int64_t* __restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t m = 0;
+  xp_row(a, q, r, [&](int64_t, double) { ++m; });
+  cnt[r] = m;
+}
+
+__global__ void k_xp_fill(ElemArgs a, S2PenArgs q, int64_t n, const int64_t* __restrict__ rowptr,
+                          int32_t* __restrict__ cols, double* __restrict__ vals) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int64_t p = rowptr[r];
+  xp_row(a, q, r, [&](int64_t j, double v) {
+    cols[p] = (int32_t)j;
+    vals[p] = v;
+    ++p;
+  });
+}
+
+// Y[:, v] = s * M X[:, v] (+ Y[:, v] when acc) over stored rows, one thread per row.
+__global__ void k_csr_apply(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                            const double* __restrict__ val, int64_t n, const double* __restrict__ X, double* __restrict__ Y,
+                            int k, int64_t ld, double s, int acc) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t q0 = ptr[r], q1 = ptr[r + 1];
+  for (int v = 0; v < k; v++) {
+    const double* x = X + (int64_t)v * ld;
+    double t = 0.0;
+    for (int64_t q = q0; q < q1; q++) t += val[q] * x[col[q]];
+    double* y = Y + (int64_t)v * ld + r;
+    *y = acc ? *y + s * t : s * t;
+  }
+}
+
 // For every string I of a set: its singles (which = 1) or doubles (which = 2) as
 // (excited string internal index, phase bit), in enumeration order.
 __global__ void k_dump_excitations(SlotDev s, int which, int per, int32_t* __restrict__ J, int8_t* __restrict__ odd) {
@@ -1581,9 +1650,24 @@ static void sigma_impl(pqg_ci* ci, const Tables<T>& tb, const T* C, T* S, int k,
   ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "sigma_f64" : "sigma_f32", k);
   if (ci->xh) {
     if constexpr (std::is_same_v<T, double>) {
-      k_xh_spmv<<<grid1(ci->n_int, 256), 256, 0, st>>>(ci->xh_ptr.get(), ci->xh_col.get(), ci->xh_val.get(), ci->n_int,
-                                                        C, S, k, ld);
+      const int64_t n = ci->n_int;
+      k_xh_spmv<<<grid1(n, 256), 256, 0, st>>>(ci->xh_ptr.get(), ci->xh_col.get(), ci->xh_val.get(), n, C, S, k, ld);
       PQG_HIP(hipGetLastError());
+      for (const auto& pen : ci->xpen) {
+        if (pen.power == 1) {
+          k_csr_apply<<<grid1(n, 256), 256, 0, st>>>(pen.ptr.get(), pen.col.get(), pen.val.get(), n, C, S, k, ld,
+                                                      pen.alpha, 1);
+        } else {
+          // T1 = P C with the leading dimension of C and S (the kernel takes one for both)
+          grow(ci->scratch, ci->ctx, (size_t)ld * k * sizeof(double), st);
+          double* T1 = reinterpret_cast<double*>(ci->scratch.get());
+          k_csr_apply<<<grid1(n, 256), 256, 0, st>>>(pen.ptr.get(), pen.col.get(), pen.val.get(), n, C, T1, k, ld,
+                                                      1.0, 0);
+          k_csr_apply<<<grid1(n, 256), 256, 0, st>>>(pen.ptr.get(), pen.col.get(), pen.val.get(), n, T1, S, k, ld,
+                                                      pen.alpha, 1);
+        }
+        PQG_HIP(hipGetLastError());
+      }
       return;
     } else {
       PQG_REQUIRE(false, PQG_ERR_UNSUPPORTED, "the stored explicit Hamiltonian is FP64 only (PQG_FP64)");
@@ -1855,6 +1939,7 @@ pqg_status pqg_ci_set_explicit(pqg_ci* ci, int enable, double screening, int64_t
   return guard([&] {
     PQG_REQUIRE(ci, PQG_ERR_ARG, "NULL argument");
     ci->xh = false;
+    ci->xpen.clear();
     ci->xh_ptr.reset(ci->ctx, 0);
     ci->xh_col.reset(ci->ctx, 0);
     ci->xh_val.reset(ci->ctx, 0);
@@ -1884,6 +1969,45 @@ pqg_status pqg_ci_set_explicit(pqg_ci* ci, int enable, double screening, int64_t
     PQG_HIP(hipStreamSynchronize(st));
     ci->xh = true;
     if (nnz) *nnz = total;
+  });
+}
+
+pqg_status pqg_ci_add_spin_penalty(pqg_ci* ci, int part, double expected, int power, double alpha) {
+  return guard([&] {
+    PQG_REQUIRE(ci, PQG_ERR_ARG, "NULL argument");
+    PQG_REQUIRE(ci->xh, PQG_ERR_ARG, "the spin penalty is added to the stored explicit H: pqg_ci_set_explicit first");
+    PQG_REQUIRE(part >= 0 && part < ci->nsp, PQG_ERR_ARG, "species index out of range");
+    PQG_REQUIRE(power == 1 || power == 2, PQG_ERR_ARG, "power must be 1 or 2");
+    hipStream_t st = ci->stream;
+    const int64_t n = ci->n_int;
+    const ElemArgs a = elem_args(ci);
+    S2PenArgs q{};
+    q.ta = 2 * part;
+    q.tb = 2 * part + 1;
+    q.power = power;
+    q.e = expected;
+    q.valid = ci->fci ? nullptr : ci->valid.get();
+    pqg_ci::XPen pen;
+    pen.alpha = alpha;
+    pen.power = power;
+    DevBuf<int64_t> cnt(ci->ctx, n + 1);
+    pen.ptr.reset(ci->ctx, n + 1);
+    k_fill_i64<<<grid1(n + 1, 256), 256, 0, st>>>(cnt.get(), n + 1, 0);
+    k_xp_count<<<grid1(n, 256), 256, 0, st>>>(a, q, n, cnt.get());
+    PQG_HIP(hipGetLastError());
+    size_t tb = 0;
+    PQG_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt.get(), pen.ptr.get(), (int)(n + 1), st));
+    DevBuf<uint8_t> tmp(ci->ctx, std::max<size_t>(tb, 1));
+    PQG_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tb, cnt.get(), pen.ptr.get(), (int)(n + 1), st));
+    int64_t total = 0;
+    PQG_HIP(hipMemcpyAsync(&total, pen.ptr.get() + n, 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+    pen.col.reset(ci->ctx, std::max<int64_t>(total, 1));
+    pen.val.reset(ci->ctx, std::max<int64_t>(total, 1));
+    k_xp_fill<<<grid1(n, 256), 256, 0, st>>>(a, q, n, pen.ptr.get(), pen.col.get(), pen.val.get());
+    PQG_HIP(hipGetLastError());
+    PQG_HIP(hipStreamSynchronize(st));
+    ci->xpen.push_back(std::move(pen));
   });
 }
 

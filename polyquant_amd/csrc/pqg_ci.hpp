@@ -121,6 +121,16 @@ struct pqg_ci {
   pqg::DevBuf<int64_t> xh_ptr;
   pqg::DevBuf<int32_t> xh_col;
   pqg::DevBuf<double> xh_val;
+  // spin penalties added to the stored H: per species P = S^2 - e (both triangles, internal
+  // order); sigma += alpha P C (power 1) or alpha P (P C) (power 2)
+  struct XPen {
+    pqg::DevBuf<int64_t> ptr;
+    pqg::DevBuf<int32_t> col;
+    pqg::DevBuf<double> val;
+    double alpha = 0.0;
+    int power = 1;
+  };
+  std::vector<XPen> xpen;
 };
 
 namespace pqg {

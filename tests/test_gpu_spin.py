@@ -175,3 +175,34 @@ def test_exact_diag_with_spin_penalty(ctx):
         ds.create_s2_penalty_dense(1, 0.0, 1)
     with pytest.raises(_lib.PqgError):
         ds.create_s2_penalty_dense(0, 0.0, 3)
+
+
+def test_davidson_with_stored_spin_penalty(ctx):
+    """create_S_sq_penalty on the explicit H of the Davidson path (s2.cpp:126-153,
+    epci.cpp:623-682): roots equal the exact_diag ones with the same penalty (one and two species,
+    both orders); the stored second-order penalty applies (H + a (S^2 - e)^2) C as the oracle's
+    dense matrices do; a penalty without the stored H is refused."""
+    from oracle import refcpu
+    from polyquant_amd import _lib
+    from polyquant_amd.ci import DetSet, EPCI
+
+    print("[gpu] case davidson stored spin penalty", flush=True)
+    cases = [(wl.small_problem(313, 6, 2, 2), "first_order", [0.3]),
+             (wl.small_problem(313, 6, 2, 2), "second_order", [5.0]),
+             (CASES["ep_fci_5e_2a2b_4p_2a1b"](), "first_order", [0.3, 0.2])]
+    for prob, kind, pen in cases:
+        Ex = EPCI(prob, ctx, num_states=3, exact_diag=True, spin_penalty_type=kind, spin_penalty=pen).run()
+        Ed = EPCI(prob, ctx, num_states=3, num_subspace_vec=10, convergence_E=1e-10, spin_penalty_type=kind,
+                  spin_penalty=pen).run()
+        assert np.max(np.abs(Ex - Ed)) < 1e-8, (kind, Ex, Ed)
+    prob = wl.small_problem(313, 6, 2, 2)
+    oc = refcpu.OracleCI(prob)
+    Q = oc.s2_penalty(0, 0.75)
+    ds = DetSet(prob, ctx)
+    C = np.random.default_rng(4).standard_normal((oc.N, 3))
+    with pytest.raises(_lib.PqgError):
+        ds.add_spin_penalty(0, 0.75, 2, 0.4)
+    ds.set_explicit(True)
+    ds.add_spin_penalty(0, 0.75, 2, 0.4)
+    ref = (oc.dense() + 0.4 * Q @ Q) @ C
+    assert np.max(np.abs(ds.sigma(C) - ref)) < 1e-11 * np.max(np.abs(ref))

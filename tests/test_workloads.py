@@ -118,7 +118,7 @@ def test_exact_diag_screening_prunes_before_the_penalty():
     # the argument checks run before any GPU object is made
     with pytest.raises(ValueError, match="FP64 only"):
         EPCI(Prob(), screening_threshold=0.1, precision=1)
-    with pytest.raises(ValueError, match="spin penalty"):
-        EPCI(Prob(), spin_penalty_type="first_order", spin_penalty=[1.0])
+    with pytest.raises(ValueError, match="FP64 only"):
+        EPCI(Prob(), spin_penalty_type="first_order", spin_penalty=[1.0], precision=1)
     with pytest.raises(ValueError, match="first_order or second_order"):
         EPCI(Prob(), exact_diag=True, spin_penalty_type="third_order")
