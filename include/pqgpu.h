@@ -191,6 +191,12 @@ pqg_status pqg_ci_dump_excitations(pqg_ci* ci, int which, int part, int spin, in
 pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* C, double* out4,
                      int on_device, int mode, double* seconds);
 
+/* transform_mo_2_body_integrals with two coefficient matrices (integral.cpp:207-477, mo_coeffs_a
+ * and mo_coeffs_b of one basis, e.g. the alpha-beta block of an unrestricted species):
+ * out4(idx2(i,j), idx2(k,l)) = (ij|kl) with i, j over the columns of Ca and k, l over those of
+ * Cb, npair x npair column-major.  Arguments otherwise as pqg_ao2mo.                        */
+pqg_status pqg_ao2mo_pair(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* Ca, const double* Cb,
+                          double* out4, int on_device, int mode, double* seconds);
 /* compute_frozen_core_ints (integral.cpp:565-698) for one (species a, spin a) operator and one
  * (species b, spin b) core density, as called by calculate_frozen_core_ints
  * (integral.cpp:166-205): out (nao_a x nao_a, column-major, host) +=

@@ -235,8 +235,12 @@ static void ctmc(const T* M, const T* Cm, T* U, T* Tout, int n, int nb, hipStrea
   bgemm<T>(U, Cm, Tout, n, nb, true, st);
 }
 
+// out4(idx2(i,j), idx2(k,l)) = (ij|kl) with i, j over the columns of Cd_row and k, l over those of
+// Cd_col (the same matrix for one spin block): half 1 transforms the column pair with Cd_col, half 2
+// the row pair with Cd_row.
 template <typename T>
-static void ao2mo_impl(pqg_ctx* ctx, int n, const double* eri8, const double* Cd, double* out4, hipStream_t st) {
+static void ao2mo_impl(pqg_ctx* ctx, int n, const double* eri8, const double* Cd_row, const double* Cd_col,
+                       double* out4, hipStream_t st) {
   const int64_t np = (int64_t)n * (n + 1) / 2;
   const int64_t n2 = (int64_t)n * n;
   // batch size: keep the three n x n x nb work arrays within ~1/8 of the budget
@@ -245,32 +249,34 @@ static void ao2mo_impl(pqg_ctx* ctx, int n, const double* eri8, const double* Cd
   PQG_REQUIRE(avail > interm + (64ull << 20), PQG_ERR_OOM, "AO->MO intermediate does not fit the HBM budget");
   int nb = (int)std::min<int64_t>(np, std::max<int64_t>(1, (int64_t)((avail - interm) / 4 / (3 * n2 * sizeof(T)))));
   nb = std::min(nb, 4096);
-  DevBuf<T> Cm(ctx, n2), X(ctx, n2 * nb), U(ctx, n2 * nb), Tm(ctx, n2 * nb), B(ctx, np * np);
+  DevBuf<T> Cr(ctx, n2), Cc(ctx, n2), X(ctx, n2 * nb), U(ctx, n2 * nb), Tm(ctx, n2 * nb), B(ctx, np * np);
   // C in row-major [mu][i] (input is column-major nao x nmo)
-  {
+  for (int w = 0; w < 2; w++) {
     std::vector<double> h(n2);
-    PQG_HIP(hipMemcpyAsync(h.data(), Cd, n2 * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipMemcpyAsync(h.data(), w ? Cd_col : Cd_row, n2 * 8, hipMemcpyDeviceToHost, st));
     PQG_HIP(hipStreamSynchronize(st));
     std::vector<T> r(n2);
     for (int mu = 0; mu < n; mu++)
       for (int i = 0; i < n; i++) r[(size_t)mu * n + i] = (T)h[mu + (size_t)i * n];
-    PQG_HIP(hipMemcpyAsync(Cm.get(), r.data(), n2 * sizeof(T), hipMemcpyHostToDevice, st));
+    PQG_HIP(hipMemcpyAsync((w ? Cc : Cr).get(), r.data(), n2 * sizeof(T), hipMemcpyHostToDevice, st));
+    PQG_HIP(hipStreamSynchronize(st));
   }
   // half 1: rows Q of B
   for (int64_t q0 = 0; q0 < np; q0 += nb) {
     const int b = (int)std::min<int64_t>(nb, np - q0);
     k_unpack8<T><<<gr(n2 * b, 256), 256, 0, st>>>(eri8, n, q0, b, X.get());
     PQG_HIP(hipGetLastError());
-    ctmc<T>(X.get(), Cm.get(), U.get(), Tm.get(), n, b, st);
+    ctmc<T>(X.get(), Cc.get(), U.get(), Tm.get(), n, b, st);
     k_pack<T, T><<<gr(np * b, 256), 256, 0, st>>>(Tm.get(), n, b, B.get() + q0 * np, np);
     PQG_HIP(hipGetLastError());
   }
-  // half 2: columns (ij) of B; output column (ij) = row (ij) by symmetry
+  // half 2: column (kl) of B holds (rho sigma|kl); its transform over the row pair is output
+  // column (kl)
   for (int64_t c0 = 0; c0 < np; c0 += nb) {
     const int b = (int)std::min<int64_t>(nb, np - c0);
     k_unpack_half<T><<<gr(n2 * b, 256), 256, 0, st>>>(B.get(), np, n, c0, b, X.get());
     PQG_HIP(hipGetLastError());
-    ctmc<T>(X.get(), Cm.get(), U.get(), Tm.get(), n, b, st);
+    ctmc<T>(X.get(), Cr.get(), U.get(), Tm.get(), n, b, st);
     k_pack<T, double><<<gr(np * b, 256), 256, 0, st>>>(Tm.get(), n, b, out4 + c0 * np, np);
     PQG_HIP(hipGetLastError());
   }
@@ -354,49 +360,60 @@ extern "C" pqg_status pqg_fc_ints(pqg_ctx* ctx, int nao_a, int nao_b, const doub
   });
 }
 
+static void ao2mo_call(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* Ca, const double* Cb,
+                       double* out4, int on_device, int mode, double* seconds) {
+  PQG_REQUIRE(ctx && eri8 && Ca && Cb && out4, PQG_ERR_ARG, "NULL argument");
+  PQG_REQUIRE(nao >= 1 && nmo == nao, PQG_ERR_UNSUPPORTED, "pqg_ao2mo needs nmo == nao");
+  PQG_REQUIRE(mode == PQG_AO2MO_FP64 || mode == PQG_AO2MO_BF16X3, PQG_ERR_ARG, "unknown mode");
+  PQG_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const int n = nao;
+  const int64_t np = (int64_t)n * (n + 1) / 2;
+  const int64_t n8 = np * (np + 1) / 2;
+  DevBuf<double> e8, ca, cb, o4;
+  const double *de8 = eri8, *dA = Ca, *dB = Cb;
+  double* dout = out4;
+  if (!on_device) {
+    e8.reset(ctx, n8);
+    ca.reset(ctx, (size_t)n * n);
+    cb.reset(ctx, (size_t)n * n);
+    o4.reset(ctx, (size_t)np * np);
+    PQG_HIP(hipMemcpyAsync(e8.get(), eri8, n8 * 8, hipMemcpyHostToDevice, st));
+    PQG_HIP(hipMemcpyAsync(ca.get(), Ca, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
+    PQG_HIP(hipMemcpyAsync(cb.get(), Cb, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
+    de8 = e8.get();
+    dA = ca.get();
+    dB = cb.get();
+    dout = o4.get();
+  }
+  hipEvent_t t0, t1;
+  PQG_HIP(hipEventCreate(&t0));
+  PQG_HIP(hipEventCreate(&t1));
+  PQG_HIP(hipEventRecord(t0, st));
+  g_prof_ctx = ctx;
+  if (mode == PQG_AO2MO_FP64)
+    ao2mo_impl<double>(ctx, n, de8, dA, dB, dout, st);
+  else
+    ao2mo_impl<float>(ctx, n, de8, dA, dB, dout, st);
+  PQG_HIP(hipEventRecord(t1, st));
+  PQG_HIP(hipEventSynchronize(t1));
+  float ms = 0.f;
+  PQG_HIP(hipEventElapsedTime(&ms, t0, t1));
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (seconds) *seconds = ms * 1e-3;
+  if (!on_device) {
+    PQG_HIP(hipMemcpyAsync(out4, dout, (size_t)np * np * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+  }
+}
+
 extern "C" pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* C, double* out4,
                                 int on_device, int mode, double* seconds) {
-  return guard([&] {
-    PQG_REQUIRE(ctx && eri8 && C && out4, PQG_ERR_ARG, "NULL argument");
-    PQG_REQUIRE(nao >= 1 && nmo == nao, PQG_ERR_UNSUPPORTED, "pqg_ao2mo needs nmo == nao");
-    PQG_REQUIRE(mode == PQG_AO2MO_FP64 || mode == PQG_AO2MO_BF16X3, PQG_ERR_ARG, "unknown mode");
-    PQG_HIP(hipSetDevice(ctx->device));
-    hipStream_t st = ctx->stream;
-    const int n = nao;
-    const int64_t np = (int64_t)n * (n + 1) / 2;
-    const int64_t n8 = np * (np + 1) / 2;
-    DevBuf<double> e8, cd, o4;
-    const double *de8 = eri8, *dC = C;
-    double* dout = out4;
-    if (!on_device) {
-      e8.reset(ctx, n8);
-      cd.reset(ctx, (size_t)n * n);
-      o4.reset(ctx, (size_t)np * np);
-      PQG_HIP(hipMemcpyAsync(e8.get(), eri8, n8 * 8, hipMemcpyHostToDevice, st));
-      PQG_HIP(hipMemcpyAsync(cd.get(), C, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
-      de8 = e8.get();
-      dC = cd.get();
-      dout = o4.get();
-    }
-    hipEvent_t t0, t1;
-    PQG_HIP(hipEventCreate(&t0));
-    PQG_HIP(hipEventCreate(&t1));
-    PQG_HIP(hipEventRecord(t0, st));
-    g_prof_ctx = ctx;
-    if (mode == PQG_AO2MO_FP64)
-      ao2mo_impl<double>(ctx, n, de8, dC, dout, st);
-    else
-      ao2mo_impl<float>(ctx, n, de8, dC, dout, st);
-    PQG_HIP(hipEventRecord(t1, st));
-    PQG_HIP(hipEventSynchronize(t1));
-    float ms = 0.f;
-    PQG_HIP(hipEventElapsedTime(&ms, t0, t1));
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    if (seconds) *seconds = ms * 1e-3;
-    if (!on_device) {
-      PQG_HIP(hipMemcpyAsync(out4, dout, (size_t)np * np * 8, hipMemcpyDeviceToHost, st));
-      PQG_HIP(hipStreamSynchronize(st));
-    }
-  });
+  return guard([&] { ao2mo_call(ctx, nao, nmo, eri8, C, C, out4, on_device, mode, seconds); });
+}
+
+extern "C" pqg_status pqg_ao2mo_pair(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* Ca,
+                                     const double* Cb, double* out4, int on_device, int mode, double* seconds) {
+  return guard([&] { ao2mo_call(ctx, nao, nmo, eri8, Ca, Cb, out4, on_device, mode, seconds); });
 }

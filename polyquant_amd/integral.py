@@ -47,6 +47,25 @@ def transform_mo_2_body_integrals(eri8: np.ndarray, C: np.ndarray, mode: int = F
     return out, sec.value
 
 
+def transform_mo_2_body_integrals_pair(eri8: np.ndarray, Ca: np.ndarray, Cb: np.ndarray, mode: int = FP64,
+                                       ctx: Optional[Context] = None):
+    """transform_mo_2_body_integrals with mo_coeffs_a != mo_coeffs_b (integral.cpp:207-477) in
+    one basis: (ij|kl) with the pair ij over Ca (rows) and kl over Cb (columns)."""
+    ctx = ctx or default_context()
+    n = Ca.shape[0]
+    npair = n * (n + 1) // 2
+    eri8 = np.ascontiguousarray(eri8, dtype=np.float64)
+    assert eri8.size == npair * (npair + 1) // 2 and Cb.shape == Ca.shape
+    Af = np.asfortranarray(Ca, dtype=np.float64)
+    Bf = np.asfortranarray(Cb, dtype=np.float64)
+    out = np.zeros((npair, npair), order="F")
+    sec = ct.c_double()
+    check(load().pqg_ao2mo_pair(ctx.h, n, Ca.shape[1], eri8.ctypes.data_as(ct.c_void_p), Af.ctypes.data_as(ct.c_void_p),
+                                Bf.ctypes.data_as(ct.c_void_p), out.ctypes.data_as(ct.c_void_p), 0, mode,
+                                ct.byref(sec)))
+    return out, sec.value
+
+
 def transform_device(ctx: Context, n: int, eri8_ptr: int, C_ptr: int, out_ptr: int, mode: int = FP64) -> float:
     """Same on device buffers (C column-major n x n, out column-major npair x npair)."""
     sec = ct.c_double()
@@ -178,22 +197,31 @@ def frozen_core_problem(species, C, kinetic, nuclear, eri_ao, e_nuc: float, froz
     565-610) for same-species AO integrals: returns the active-space CIProblem whose e_shift is
     e_nuc plus the frozen-core energies, and those energies.  species: objects with .charge,
     .restricted, .n_alpha, .n_beta (all particles, core included) and .ex_level; C[a][s]: the
-    MO coefficients (one restricted block); eri_ao[(0, 0)]: 8-fold packed AO integrals."""
+    MO coefficients (one block, or alpha and beta when unrestricted); eri_ao[(0, 0)]: 8-fold
+    packed AO integrals."""
     from .workloads import CIProblem, Species
 
     if len(species) != 1:
         raise ValueError("frozen_core_problem takes one species (two-species AO blocks: fc_ints_block)")
-    if len(C[0]) != 1:
-        raise ValueError("unrestricted frozen core needs the mixed-spin transform (not built)")
     sp = species[0]
     sp.num_parts = sp.n_alpha + sp.n_beta
     fc_dm, fci, e_fc = calculate_fc_energy(species, C, frozen_core, eri_ao, kinetic, nuclear, ctx)
     h = calculate_mo_1_body_integrals(C, frozen_core, deleted_virtual, fci)
     nmo = np.asarray(C[0][0]).shape[1]
     nact = nmo - frozen_core[0] - deleted_virtual[0]
-    out4, _ = transform_mo_2_body_integrals(eri_ao[(0, 0)], np.asarray(C[0][0]), ctx=ctx)
-    eri = {(0, 0, 0, 0): active_two_body(out4, nmo, frozen_core[0], nact)}
-    act = Species(nact, sp.n_alpha - frozen_core[0], sp.n_beta - frozen_core[0], sp.charge, 1,
+    nspin = len(C[0])
+    eri = {}
+    for s1 in range(nspin):
+        for s2 in range(s1, nspin):
+            Ca, Cb = np.asarray(C[0][s1]), np.asarray(C[0][s2])
+            if s1 == s2:
+                out4, _ = transform_mo_2_body_integrals(eri_ao[(0, 0)], Ca, ctx=ctx)
+            else:
+                out4, _ = transform_mo_2_body_integrals_pair(eri_ao[(0, 0)], Ca, Cb, ctx=ctx)
+            eri[(0, s1, 0, s2)] = active_two_body(out4, nmo, frozen_core[0], nact)
+    if nspin == 2:
+        eri[(0, 1, 0, 0)] = np.ascontiguousarray(eri[(0, 0, 0, 1)].T)
+    act = Species(nact, sp.n_alpha - frozen_core[0], sp.n_beta - frozen_core[0], sp.charge, nspin,
                   getattr(sp, "ex_level", None))
     prob = CIProblem(species=[act], h=[h[0]], eri=eri, nroots=nroots, e_shift=e_nuc + sum(e_fc),
                      name=f"frozen_core_{frozen_core[0]}_deleted_{deleted_virtual[0]}")

@@ -124,3 +124,45 @@ def test_frozen_core_and_deleted_virtual_problem(ctx):
     assert len(keep) == 25
     ref = np.linalg.eigvalsh(oc.dense()[np.ix_(keep, keep)])[:3] + full.e_shift
     assert np.max(np.abs(E - ref)) < 1e-9, (E, ref)
+
+
+def test_ao2mo_pair_matches_oracle(ctx):
+    """pqg_ao2mo_pair: (ij|kl) with ij over Ca and kl over Cb equals the oracle's transform."""
+    from oracle import refcpu
+    from polyquant_amd import integral
+
+    print("[gpu] case ao2mo_pair n=8", flush=True)
+    rng = np.random.default_rng(21)
+    n = 8
+    X = _sym_ao(rng, n, n, True)
+    Ca, Cb = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    got, _ = integral.transform_mo_2_body_integrals_pair(integral.pack8(X), Ca, Cb, ctx=ctx)
+    ref = refcpu.ao2mo(X, Ca, Cb)
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+def test_unrestricted_frozen_core_equals_restricted(ctx):
+    """frozen_core_problem with separate alpha and beta blocks (the beta orbitals a rotation of
+    the active ones): the unrestricted frozen-core full CI has the restricted one's energies
+    (full CI is invariant under separate rotations of the active alpha and beta orbitals)."""
+    from types import SimpleNamespace
+
+    from polyquant_amd import integral, molint
+    from polyquant_amd.ci import EPCI
+
+    print("[gpu] case unrestricted frozen core water sto-3g", flush=True)
+    w = molint.water("sto-3g")
+    C = w["C"]
+    R, _ = np.linalg.qr(np.random.default_rng(3).standard_normal((6, 6)))
+    Cb = C.copy()
+    Cb[:, 1:] = C[:, 1:] @ R
+    eri = {(0, 0): integral.pack8(w["eri_ao"])}
+    args = ([w["T"]], [w["V"]], eri, w["E_nuc"], [1], [0])
+    sr = [SimpleNamespace(charge=-1.0, restricted=True, n_alpha=5, n_beta=5, ex_level=None)]
+    su = [SimpleNamespace(charge=-1.0, restricted=False, n_alpha=5, n_beta=5, ex_level=None)]
+    pr, er = integral.frozen_core_problem(sr, [[C]], *args, nroots=3, ctx=ctx)
+    pu, eu = integral.frozen_core_problem(su, [[C, Cb]], *args, nroots=3, ctx=ctx)
+    assert pu.species[0].nspin_blocks == 2 and abs(er[0] - eu[0]) < 1e-10
+    Er = EPCI(pr, ctx, num_states=3, exact_diag=True).run()
+    Eu = EPCI(pu, ctx, num_states=3, exact_diag=True).run()
+    assert np.max(np.abs(Er - Eu)) < 1e-9, (Er, Eu)

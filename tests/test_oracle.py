@@ -468,3 +468,19 @@ def test_frozen_core_oracle_water_equals_restricted_full_space():
     keep = [i for i in range(oc.N) if (int(Sa[U[i, 0]]) & 1) and (int(Sb[U[i, 1]]) & 1)]
     ref = np.linalg.eigvalsh(oc.dense()[np.ix_(keep, keep)])[:3] + full.e_shift
     assert np.max(np.abs(E - ref)) < 1e-9
+
+
+def test_ao2mo_two_coefficient_matrices_closed_form():
+    """The transform with mo_coeffs_a != mo_coeffs_b (integral.cpp:207-477): row pair ij over Ca,
+    column pair kl over Cb, against the einsum of the quarter transforms."""
+    rng = np.random.default_rng(12)
+    n = 5
+    X = rng.standard_normal((n, n, n, n))
+    X = X + X.transpose(1, 0, 2, 3)
+    X = X + X.transpose(0, 1, 3, 2)
+    X = X + X.transpose(2, 3, 0, 1)
+    Ca, Cb = rng.standard_normal((n, n)), rng.standard_normal((n, n))
+    ref = np.einsum("pi,qj,rk,sl,pqrs->ijkl", Ca, Ca, Cb, Cb, X, optimize=True)
+    ii, jj = np.array([(i, j) for i in range(n) for j in range(i + 1)]).T
+    got = refcpu.ao2mo(X, Ca, Cb)
+    assert np.max(np.abs(got - ref[ii[:, None], jj[:, None], ii[None, :], jj[None, :]])) < 1e-11
