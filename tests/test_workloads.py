@@ -122,3 +122,40 @@ def test_exact_diag_screening_prunes_before_the_penalty():
         EPCI(Prob(), spin_penalty_type="first_order", spin_penalty=[1.0], precision=1)
     with pytest.raises(ValueError, match="first_order or second_order"):
         EPCI(Prob(), exact_diag=True, spin_penalty_type="third_order")
+
+
+def test_frozen_core_ints_host_loop_follows_reference():
+    """calculate_frozen_core_ints (integral.cpp:166-205): for each species a and spin block of
+    fc_dm[a], one compute_frozen_core_ints call per species b with a frozen core and per spin
+    block of fc_dm[b] (so an unrestricted core species contributes once per block also to another
+    species), then kinetic - charge * nuclear; a > b blocks use the transposed (b, a) integrals."""
+    from types import SimpleNamespace
+
+    import numpy as np
+
+    from polyquant_amd import integral
+
+    calls = []
+
+    def fake(out, eri, dm_b, same, sa, sb, rb, cp, ctx=None):
+        calls.append((out.shape[0], eri.shape, len(dm_b), same, sa, sb, rb, cp))
+        out += 1.0
+        return out
+
+    saved = integral.fc_ints_block
+    integral.fc_ints_block = fake
+    try:
+        sp = [SimpleNamespace(charge=-1.0, restricted=False), SimpleNamespace(charge=1.0, restricted=True)]
+        fc_dm = [[np.zeros((3, 3)), np.zeros((3, 3))], [np.zeros((2, 2))]]
+        eri = {(0, 0): np.zeros(21), (0, 1): np.zeros((6, 3)), (1, 1): np.zeros(6)}
+        kin = [np.eye(3), 2 * np.eye(2)]
+        nuc = [np.ones((3, 3)), np.ones((2, 2))]
+        out = integral.calculate_frozen_core_ints(sp, fc_dm, [1, 0], eri, kin, nuc)
+    finally:
+        integral.fc_ints_block = saved
+    # species 1 has no frozen core: only species 0 contributes, twice (two spin blocks)
+    assert calls == [(3, (21,), 2, True, 0, 0, False, 1.0), (3, (21,), 2, True, 0, 1, False, 1.0),
+                     (3, (21,), 2, True, 1, 0, False, 1.0), (3, (21,), 2, True, 1, 1, False, 1.0),
+                     (2, (3, 6), 2, False, 0, 0, False, -1.0), (2, (3, 6), 2, False, 0, 1, False, -1.0)]
+    assert np.allclose(out[0][0], 2.0 + np.eye(3) + np.ones((3, 3)))
+    assert np.allclose(out[1][0], 2.0 + 2 * np.eye(2) - np.ones((2, 2)))
