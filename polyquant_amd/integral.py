@@ -203,9 +203,12 @@ def frozen_core_problem(species, C, kinetic, nuclear, eri_ao, e_nuc: float, froz
 
     if len(species) != 1:
         raise ValueError("frozen_core_problem takes one species (two-species AO blocks: fc_ints_block)")
+    from types import SimpleNamespace
+
     sp = species[0]
-    sp.num_parts = sp.n_alpha + sp.n_beta
-    fc_dm, fci, e_fc = calculate_fc_energy(species, C, frozen_core, eri_ao, kinetic, nuclear, ctx)
+    # the caller's object is left as it is; calculate_fc_energy reads num_parts
+    spc = SimpleNamespace(charge=sp.charge, restricted=sp.restricted, num_parts=sp.n_alpha + sp.n_beta)
+    fc_dm, fci, e_fc = calculate_fc_energy([spc], C, frozen_core, eri_ao, kinetic, nuclear, ctx)
     h = calculate_mo_1_body_integrals(C, frozen_core, deleted_virtual, fci)
     nmo = np.asarray(C[0][0]).shape[1]
     nact = nmo - frozen_core[0] - deleted_virtual[0]
