@@ -156,6 +156,20 @@ pqg_status pqg_ci_sigma(pqg_ci* ci, const double* C, int64_t ldc, int k, double*
 pqg_status pqg_ci_davidson(pqg_ci* ci, const pqg_dav_params* params, double* E, double* X, int64_t ldx,
                            pqg_dav_report* report);
 
+/* The same solve one iteration at a time (no counterpart in the reference, whose Spectra solver
+ * runs its iterations inside compute(), epci.cpp:606-608): for callers that interleave solves
+ * with other work or time single iterations.  pqg_dav_create builds the start vectors;
+ * pqg_dav_step runs one iteration (the sigma of the pending block, Rayleigh-Ritz, residuals and
+ * the next block; one host synchronisation) and sets *done once the solve has converged or
+ * stopped; pqg_dav_result then gives what pqg_ci_davidson gives (E, X and report may be NULL,
+ * PQG_ERR_NOCONV after filling them when the solve stopped unconverged).  The handle belongs to
+ * its pqg_ci and must be destroyed before it; it is used from one host thread at a time.    */
+typedef struct pqg_dav pqg_dav;
+pqg_status pqg_dav_create(pqg_ci* ci, const pqg_dav_params* params, pqg_dav** out);
+pqg_status pqg_dav_step(pqg_dav* dav, int* done);
+pqg_status pqg_dav_result(pqg_dav* dav, double* E, double* X, int64_t ldx, pqg_dav_report* report);
+pqg_status pqg_dav_destroy(pqg_dav* dav);
+
 /* create_1rdm (src/ci/detset/rdm1.cpp:6-79) for k states at once: the one-particle reduced
  * density matrix of the (part, spin) string set for each column of C (N x k, reference order,
  * column-major).  D receives k consecutive n_orb x n_orb column-major blocks, element (k, l) =
@@ -204,8 +218,9 @@ pqg_status pqg_ao2mo_pair(pqg_ctx* ctx, int nao, int nmo, const double* eri8, co
  * with J(D)_ij = sum_kl (ij|kl) D_kl, K(D)_ij = sum_kl (ik|jl) D_kl, Dc = 2 D (restricted b) or
  * D_alpha + D_beta, Dx = D (restricted b) or D_{b_spin}, cj = 1 (same species, restricted b),
  * 1/2 (same species, unrestricted b: one call per spin block) or charge_prod (two species).
- * dm_b: the core densities of species b, 1 (restricted) or 2 consecutive nao_b x nao_b blocks
- * (fc_dm[b]).  eri: same species, the 8-fold packed AO integrals (as pqg_ao2mo); two species,
+ * dm_b: the core densities of species b, nao_b x nao_b blocks: exactly 1 is read when
+ * restricted_b != 0, exactly 2 consecutive blocks (alpha, beta) when restricted_b == 0, so an
+ * unrestricted species with one spin block passes a zero beta block.  eri: same species, the 8-fold packed AO integrals (as pqg_ao2mo); two species,
  * the npair_a x npair_b column-major block (ij|kl), idx2 packing.                            */
 pqg_status pqg_fc_ints(pqg_ctx* ctx, int nao_a, int nao_b, const double* eri, int same_particle, int a_spin,
                        int b_spin, int restricted_b, double charge_prod, const double* dm_b, double* out);

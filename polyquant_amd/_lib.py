@@ -101,6 +101,10 @@ SIGNATURES = [
     ("pqg_ci_s2", ct.c_int, [_vp, _vp, ct.c_int64, ct.c_int, _vp, ct.c_int]),
     ("pqg_ci_rdm1", ct.c_int, [_vp, _vp, ct.c_int64, ct.c_int, ct.c_int, ct.c_int, _vp, ct.c_int]),
     ("pqg_ci_davidson", ct.c_int, [_vp, ct.POINTER(DavParams), _vp, _vp, ct.c_int64, ct.POINTER(DavReport)]),
+    ("pqg_dav_create", ct.c_int, [_vp, ct.POINTER(DavParams), ct.POINTER(_vp)]),
+    ("pqg_dav_step", ct.c_int, [_vp, ct.POINTER(ct.c_int)]),
+    ("pqg_dav_result", ct.c_int, [_vp, _vp, _vp, ct.c_int64, ct.POINTER(DavReport)]),
+    ("pqg_dav_destroy", ct.c_int, [_vp]),
     ("pqg_ci_dump_strings", ct.c_int, [_vp, ct.c_int, ct.c_int, _vp]),
     ("pqg_ci_dump_unfold", ct.c_int, [_vp, _vp]),
     ("pqg_ci_dump_excitations", ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, _vp, _vp, _vp]),

@@ -290,6 +290,75 @@ class DetSet:
         check(st)
         return E, X, rep
 
+    def davidson_solver(self, nev: int, nvec_init: int = 5, nvec_max: int = 0, maxit: int = 500, tol: float = 1e-6,
+                        precision: int = _lib.PQG_FP64, want_vectors: bool = True,
+                        perturb: float = 0.0) -> DavidsonSolver:
+        """The same solve as davidson(), run an iteration at a time through DavidsonSolver.step()."""
+        return DavidsonSolver(self, DavParams(nev, nvec_init, nvec_max, maxit, tol, precision, perturb), want_vectors)
+
+
+def spin_penalty_defaults(prob, spin_penalty_type, spin_penalty=None, expected_S2=None):
+    """spin_penalty and expected_S2 as POLYQUANT_EPCI::setup fills them (src/ci/epci.cpp:15-26):
+    with a penalty type, an empty spin_penalty becomes default_spin_penalty = 0.1 per species
+    (epci.hpp:130) and an empty expected_S2 the SCF <S^2> of each species (input_epscf->S_squared,
+    here ``prob.scf_S2``).  A problem without an SCF value needs expected_S2 from the caller for the
+    second-order penalty; the first order uses 0 (s2.cpp:136-140).  Returns both lists."""
+    nsp = len(prob.species)
+    if spin_penalty is not None:
+        pen = [float(x) for x in spin_penalty]
+    else:
+        pen = [0.1] * nsp if spin_penalty_type is not None else [0.0] * nsp
+    if expected_S2 is not None:
+        exp = [float(x) for x in expected_S2]
+    elif spin_penalty_type is not None and getattr(prob, "scf_S2", None) is not None:
+        exp = [float(x) for x in prob.scf_S2]
+    elif spin_penalty_type == "second_order":
+        raise ValueError("second_order spin penalty needs expected_S2 (the problem carries no SCF <S^2>)")
+    else:
+        exp = [0.0] * nsp
+    if len(pen) < nsp or len(exp) < nsp:
+        raise ValueError("spin_penalty and expected_S2 need one value per species")
+    return pen, exp
+
+
+class DavidsonSolver:
+    """One Davidson solve driven an iteration at a time (pqg_dav_*): step() runs one iteration
+    and returns True once the solve has finished; result() then returns (E, X, report) as
+    DetSet.davidson does."""
+
+    def __init__(self, ds: "DetSet", params: DavParams, want_vectors: bool = True):
+        self.ds = ds
+        self.nev = params.nev
+        self.want_vectors = want_vectors
+        self.h = ct.c_void_p()
+        check(load().pqg_dav_create(ds.h, ct.byref(params), ct.byref(self.h)))
+        self.done = False
+
+    def step(self) -> bool:
+        d = ct.c_int()
+        check(load().pqg_dav_step(self.h, ct.byref(d)))
+        self.done = bool(d.value)
+        return self.done
+
+    def result(self):
+        E = np.zeros(self.nev)
+        X = np.zeros((self.ds.N_dets, self.nev), order="F") if self.want_vectors else None
+        rep = DavReport()
+        check(load().pqg_dav_result(self.h, _ptr(E), _ptr(X) if X is not None else None, self.ds.N_dets,
+                                    ct.byref(rep)))
+        return E, X, rep
+
+    def close(self):
+        if self.h:
+            load().pqg_dav_destroy(self.h)
+            self.h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 class EPCI:
     """Solve flow of POLYQUANT_EPCI::run (src/ci/epci.cpp:575-662): the matrix-free Davidson path
@@ -312,6 +381,8 @@ class EPCI:
         # explicit H can carry); FP64 only
         if (build_matrix or float(screening_threshold) != 0.0) and not exact_diag and precision != _lib.PQG_FP64:
             raise ValueError("the stored explicit Hamiltonian is FP64 only")
+        self.spin_penalty, self.expected_S2 = spin_penalty_defaults(prob, spin_penalty_type, spin_penalty,
+                                                                    expected_S2)
         self.prob = prob
         self.detset = DetSet(prob, ctx)
         self.num_states = num_states
@@ -322,9 +393,6 @@ class EPCI:
         self.exact_diag = exact_diag
         self.first_order_spin_penalty = spin_penalty_type == "first_order"
         self.second_order_spin_penalty = spin_penalty_type == "second_order"
-        nsp = len(prob.species)
-        self.spin_penalty = list(spin_penalty) if spin_penalty is not None else [0.0] * nsp
-        self.expected_S2 = list(expected_S2) if expected_S2 is not None else [0.0] * nsp
         self.screening_threshold = float(screening_threshold)
         # the penalty is added to the explicit H (epci.cpp:623-631), so it implies build_matrix
         self.build_matrix = bool(build_matrix) or self.screening_threshold != 0.0 or spin_penalty_type is not None

@@ -1072,6 +1072,34 @@ __global__ void k_csr_apply(const int64_t* __restrict__ ptr, const int32_t* __re
   }
 }
 
+// Diagonal of the stored explicit H (entries outside the space keep diagonal_Hii's 1e30).
+__global__ void k_xdiag_init(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                             const double* __restrict__ val, const double* __restrict__ hd, int64_t n,
+                             double* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double d = 0.0;
+  for (int64_t q = ptr[r]; q < ptr[r + 1]; q++)
+    if (col[q] == r) d += val[q];
+  out[r] = hd[r] >= 1e29 ? hd[r] : d;
+}
+// out += alpha diag(P) (power 1) or alpha diag(P^2) = alpha sum_k P_rk^2 (power 2, P symmetric)
+__global__ void k_xdiag_pen(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                            const double* __restrict__ val, int64_t n, double alpha, int power,
+                            double* __restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || out[r] >= 1e29) return;
+  double d = 0.0;
+  for (int64_t q = ptr[r]; q < ptr[r + 1]; q++) {
+    if (power == 1) {
+      if (col[q] == r) d += val[q];
+    } else {
+      d += val[q] * val[q];
+    }
+  }
+  out[r] += alpha * d;
+}
+
 // For every string I of a set: its singles (which = 1) or doubles (which = 2) as
 // (excited string internal index, phase bit), in enumeration order.
 __global__ void k_dump_excitations(SlotDev s, int which, int per, int32_t* __restrict__ J, int8_t* __restrict__ odd) {
@@ -1943,6 +1971,7 @@ pqg_status pqg_ci_set_explicit(pqg_ci* ci, int enable, double screening, int64_t
     ci->xh_ptr.reset(ci->ctx, 0);
     ci->xh_col.reset(ci->ctx, 0);
     ci->xh_val.reset(ci->ctx, 0);
+    ci->xdiag.reset(ci->ctx, 0);
     if (nnz) *nnz = 0;
     if (!enable) return;
     PQG_REQUIRE(ci->n_int < (int64_t(1) << 31) - 1, PQG_ERR_UNSUPPORTED, "N above 2^31 - 2 (int32 indices)");
@@ -1965,6 +1994,10 @@ pqg_status pqg_ci_set_explicit(pqg_ci* ci, int enable, double screening, int64_t
     ci->xh_col.reset(ci->ctx, std::max<int64_t>(total, 1));
     ci->xh_val.reset(ci->ctx, std::max<int64_t>(total, 1));
     k_xh_fill<<<grid1(n, 128), 128, 0, st>>>(c, n, ci->xh_ptr.get(), ci->xh_col.get(), ci->xh_val.get());
+    PQG_HIP(hipGetLastError());
+    ci->xdiag.reset(ci->ctx, n);
+    k_xdiag_init<<<grid1(n, 256), 256, 0, st>>>(ci->xh_ptr.get(), ci->xh_col.get(), ci->xh_val.get(), ci->hdiag.get(),
+                                                n, ci->xdiag.get());
     PQG_HIP(hipGetLastError());
     PQG_HIP(hipStreamSynchronize(st));
     ci->xh = true;
@@ -2005,6 +2038,9 @@ pqg_status pqg_ci_add_spin_penalty(pqg_ci* ci, int part, double expected, int po
     pen.col.reset(ci->ctx, std::max<int64_t>(total, 1));
     pen.val.reset(ci->ctx, std::max<int64_t>(total, 1));
     k_xp_fill<<<grid1(n, 256), 256, 0, st>>>(a, q, n, pen.ptr.get(), pen.col.get(), pen.val.get());
+    PQG_HIP(hipGetLastError());
+    k_xdiag_pen<<<grid1(n, 256), 256, 0, st>>>(pen.ptr.get(), pen.col.get(), pen.val.get(), n, alpha, power,
+                                               ci->xdiag.get());
     PQG_HIP(hipGetLastError());
     PQG_HIP(hipStreamSynchronize(st));
     ci->xpen.push_back(std::move(pen));

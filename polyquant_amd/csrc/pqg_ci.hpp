@@ -131,10 +131,16 @@ struct pqg_ci {
     int power = 1;
   };
   std::vector<XPen> xpen;
+  // diagonal of the stored operator (stored H plus the penalties): the Davidson's
+  // preconditioner and start selection use it while the explicit H is stored
+  pqg::DevBuf<double> xdiag;
 };
 
 namespace pqg {
 
+// Diagonal of the operator sigma applies: the stored explicit H with its penalties when
+// one is stored, else diagonal_Hii (internal order, +1e30 outside the space).
+inline const double* operator_diag(const pqg_ci* ci) { return ci->xh ? ci->xdiag.get() : ci->hdiag.get(); }
 // Sigma on internal-order device vectors: S = H C for k vectors (leading dimension ld).
 void sigma_internal(pqg_ci* ci, const double* C, double* S, int k, int64_t ld, hipStream_t st);
 void sigma_internal(pqg_ci* ci, const float* C, float* S, int k, int64_t ld, hipStream_t st);

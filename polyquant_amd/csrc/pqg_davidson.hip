@@ -542,7 +542,7 @@ void Davidson::ts_gemm_nn2(hipStream_t st, const double* A0, const double* A1, i
 void Davidson::initial_guess(hipStream_t st) {
   // unit vectors at the ninit smallest diagonal entries (ties: lower index first)
   std::vector<double> hd(N_);
-  PQG_HIP(hipMemcpyAsync(hd.data(), ci_->hdiag.get(), N_ * 8, hipMemcpyDeviceToHost, st));
+  PQG_HIP(hipMemcpyAsync(hd.data(), operator_diag(ci_), N_ * 8, hipMemcpyDeviceToHost, st));
   PQG_HIP(hipStreamSynchronize(st));
   std::vector<int64_t> idx(N_);
   std::iota(idx.begin(), idx.end(), 0);
@@ -713,7 +713,7 @@ void Davidson::post_sigma(hipStream_t st) {
       for (int k = 0; k < nev_; k++) theta_h_[k] = q[k + k * 64];
       PQG_HIP(hipMemcpyAsync(theta_.get(), theta_h_.data(), nev_ * 8, hipMemcpyHostToDevice, st));
     }
-    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), W64_.get(), theta_.get(), ci_->hdiag.get(), Rp(), Tp(),
+    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), W64_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(),
                                             N_, nev_);
     PQG_HIP(hipGetLastError());
     ts_gemm_tn(st, Rp(), N_, nev_, Rp(), N_, nev_, small_.get(), 64);
@@ -783,7 +783,7 @@ void Davidson::post_sigma(hipStream_t st) {
   rayleigh_ritz(st);
   // Ritz vectors and residuals
   ts_gemm_nn2(st, V_.get(), W_.get(), N_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), N_);
-  k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), ci_->hdiag.get(), Rp(), Tp(), N_,
+  k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_,
                                           nev_);
   PQG_HIP(hipGetLastError());
   // one Gram of [R T]: R^T R (residual norms) and T^T T (drop thresholds of the corrections)
@@ -867,29 +867,89 @@ pqg_dav_report Davidson::report() const {
 
 using namespace pqg;
 
+struct pqg_dav {
+  pqg_ci* ci;
+  pqg::Davidson dav;
+  pqg_dav(pqg_ci* c, const pqg_dav_params& p) : ci(c), dav(c, p) {}
+};
+
+namespace {
+
+void dav_check_params(const pqg_ci* ci, const pqg_dav_params* params) {
+  PQG_REQUIRE(ci && params, PQG_ERR_ARG, "NULL argument");
+  PQG_REQUIRE(params->precision == PQG_FP64 || params->precision == PQG_MIXED, PQG_ERR_ARG, "unknown precision");
+  PQG_REQUIRE(params->tol > 0 && params->maxit > 0, PQG_ERR_ARG, "tol and maxit must be positive");
+}
+
+// energies, Ritz vectors (reference order, host) and report of a finished solve; raises the
+// reference's "did not converge" error after filling them
+void dav_finish(pqg_ci* ci, Davidson& dav, double* E, double* X, int64_t ldx, pqg_dav_report* report) {
+  hipStream_t st = ci->stream;
+  PQG_HIP(hipStreamSynchronize(st));
+  const auto& ev = dav.energies();
+  if (E)
+    for (int k = 0; k < dav.nev(); k++) E[k] = ev[k];
+  if (X) {
+    PQG_REQUIRE(ldx >= ci->n_ref, PQG_ERR_ARG, "ldx smaller than N");
+    DevBuf<double> xr(ci->ctx, (size_t)ci->n_ref * dav.nev());
+    to_reference(ci, dav.ritz_vectors(), ci->n_int, xr.get(), ci->n_ref, dav.nev(), st);
+    PQG_HIP(hipMemcpy2DAsync(X, ldx * 8, xr.get(), ci->n_ref * 8, ci->n_ref * 8, dav.nev(), hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipStreamSynchronize(st));
+  }
+  if (report) *report = dav.report();
+  if (dav.status() != PQG_OK) throw Error{dav.status(), "CI Calculation did not converge!"};
+}
+
+}  // namespace
+
 extern "C" pqg_status pqg_ci_davidson(pqg_ci* ci, const pqg_dav_params* params, double* E, double* X, int64_t ldx,
                                       pqg_dav_report* report) {
   return guard([&] {
-    PQG_REQUIRE(ci && params && E, PQG_ERR_ARG, "NULL argument");
-    PQG_REQUIRE(params->precision == PQG_FP64 || params->precision == PQG_MIXED, PQG_ERR_ARG, "unknown precision");
-    PQG_REQUIRE(params->tol > 0 && params->maxit > 0, PQG_ERR_ARG, "tol and maxit must be positive");
+    dav_check_params(ci, params);
+    PQG_REQUIRE(E, PQG_ERR_ARG, "NULL argument");
     hipStream_t st = ci->stream;
     Davidson dav(ci, *params);
     while (!dav.done()) {
       dav.run_sigma(st);
       dav.post_sigma(st);
     }
-    PQG_HIP(hipStreamSynchronize(st));
-    const auto& ev = dav.energies();
-    for (int k = 0; k < dav.nev(); k++) E[k] = ev[k];
-    if (X) {
-      PQG_REQUIRE(ldx >= ci->n_ref, PQG_ERR_ARG, "ldx smaller than N");
-      DevBuf<double> xr(ci->ctx, (size_t)ci->n_ref * dav.nev());
-      to_reference(ci, dav.ritz_vectors(), ci->n_int, xr.get(), ci->n_ref, dav.nev(), st);
-      PQG_HIP(hipMemcpy2DAsync(X, ldx * 8, xr.get(), ci->n_ref * 8, ci->n_ref * 8, dav.nev(), hipMemcpyDeviceToHost, st));
-      PQG_HIP(hipStreamSynchronize(st));
+    dav_finish(ci, dav, E, X, ldx, report);
+  });
+}
+
+extern "C" pqg_status pqg_dav_create(pqg_ci* ci, const pqg_dav_params* params, pqg_dav** out) {
+  return guard([&] {
+    dav_check_params(ci, params);
+    PQG_REQUIRE(out, PQG_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    *out = new pqg_dav(ci, *params);
+  });
+}
+
+extern "C" pqg_status pqg_dav_step(pqg_dav* d, int* done) {
+  return guard([&] {
+    PQG_REQUIRE(d && done, PQG_ERR_ARG, "NULL argument");
+    if (!d->dav.done()) {
+      hipStream_t st = d->ci->stream;
+      d->dav.run_sigma(st);
+      d->dav.post_sigma(st);
     }
-    if (report) *report = dav.report();
-    if (dav.status() != PQG_OK) throw Error{dav.status(), "CI Calculation did not converge!"};
+    *done = d->dav.done() ? 1 : 0;
+  });
+}
+
+extern "C" pqg_status pqg_dav_result(pqg_dav* d, double* E, double* X, int64_t ldx, pqg_dav_report* report) {
+  return guard([&] {
+    PQG_REQUIRE(d, PQG_ERR_ARG, "NULL argument");
+    PQG_REQUIRE(d->dav.done(), PQG_ERR_ARG, "the solve has not finished: call pqg_dav_step until done");
+    dav_finish(d->ci, d->dav, E, X, ldx, report);
+  });
+}
+
+extern "C" pqg_status pqg_dav_destroy(pqg_dav* d) {
+  return guard([&] {
+    if (!d) return;
+    (void)hipStreamSynchronize(d->ci->stream);
+    delete d;
   });
 }

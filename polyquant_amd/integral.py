@@ -66,6 +66,24 @@ def transform_mo_2_body_integrals_pair(eri8: np.ndarray, Ca: np.ndarray, Cb: np.
     return out, sec.value
 
 
+def cholesky_eri8_device(L, block: int = 2048):
+    """8-fold packed AO integrals of Cholesky-like vectors, built on the GPU (torch):
+    eri8[idx2(P, Q)] = sum_k L[k, P] L[k, Q] for pair indices Q <= P.  L: (nvec, npair) float64,
+    host array or device tensor.  Returns (eri8, L_device)."""
+    import torch
+
+    Lg = torch.as_tensor(L, dtype=torch.float64, device="cuda")
+    npr = Lg.shape[1]
+    eri8 = torch.empty(npr * (npr + 1) // 2, dtype=torch.float64, device=Lg.device)
+    for p0 in range(0, npr, block):
+        p1 = min(npr, p0 + block)
+        M = Lg[:, p0:p1].T.contiguous() @ Lg[:, :p1]
+        mask = torch.arange(p1, device=Lg.device)[None, :] <= torch.arange(p0, p1, device=Lg.device)[:, None]
+        eri8[p0 * (p0 + 1) // 2: p1 * (p1 + 1) // 2] = M[mask]
+        del M, mask
+    return eri8, Lg
+
+
 def transform_device(ctx: Context, n: int, eri8_ptr: int, C_ptr: int, out_ptr: int, mode: int = FP64) -> float:
     """Same on device buffers (C column-major n x n, out column-major npair x npair)."""
     sec = ct.c_double()
@@ -85,6 +103,13 @@ def fc_ints_block(out: np.ndarray, eri: np.ndarray, dm_b, same_particle: bool, a
     ctx = ctx or default_context()
     na = out.shape[0]
     nb = dm_b[0].shape[0]
+    if len(dm_b) not in (1, 2):
+        raise ValueError("dm_b holds one (restricted) or two (unrestricted) core densities")
+    if not restricted_b and len(dm_b) == 1:
+        # an unrestricted species with one particle has only its alpha block (calculate_fc_energy
+        # makes a second block only for num_parts > 1, epci.cpp:47-129): its beta core density is
+        # zero, and pqg_fc_ints reads two blocks for an unrestricted species
+        dm_b = [dm_b[0], np.zeros_like(np.asarray(dm_b[0], dtype=np.float64))]
     # block-major, each block column-major
     dm = np.concatenate([np.asarray(d, dtype=np.float64).ravel(order="F") for d in dm_b])
     eri = np.ascontiguousarray(eri, dtype=np.float64) if same_particle else np.asfortranarray(eri, dtype=np.float64)

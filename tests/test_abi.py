@@ -46,3 +46,63 @@ def test_no_gpu_means_loud_failure():
         pytest.skip("libpqgpu.so not built")
     with pytest.raises(_lib.PqgError):
         Context()
+
+
+LAYOUT_C = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "pqgpu.h"
+#define F(T, m) printf(#T " " #m " %zu %zu\n", offsetof(T, m), sizeof(((T*)0)->m))
+#define S(T) printf(#T " sizeof %zu %zu\n", sizeof(T), _Alignof(T))
+int main(void) {
+  S(pqg_ci_desc); F(pqg_ci_desc, nspecies); F(pqg_ci_desc, n_orb); F(pqg_ci_desc, n_alpha);
+  F(pqg_ci_desc, n_beta); F(pqg_ci_desc, charge); F(pqg_ci_desc, nspin_blocks); F(pqg_ci_desc, ex_level);
+  F(pqg_ci_desc, max_collective); F(pqg_ci_desc, h); F(pqg_ci_desc, eri);
+  S(pqg_dav_params); F(pqg_dav_params, nev); F(pqg_dav_params, nvec_init); F(pqg_dav_params, nvec_max);
+  F(pqg_dav_params, maxit); F(pqg_dav_params, tol); F(pqg_dav_params, precision);
+  F(pqg_dav_params, guess_perturb);
+  S(pqg_dav_report); F(pqg_dav_report, iterations); F(pqg_dav_report, converged);
+  F(pqg_dav_report, sigma_calls); F(pqg_dav_report, sigma_calls_fp64); F(pqg_dav_report, restarts);
+  F(pqg_dav_report, residual); F(pqg_dav_report, seconds);
+  S(pqg_result); F(pqg_result, request_id); F(pqg_result, status); F(pqg_result, nev);
+  F(pqg_result, energies); F(pqg_result, t_submit); F(pqg_result, t_admit); F(pqg_result, t_done);
+  F(pqg_result, iterations); F(pqg_result, message);
+  return 0;
+}
+"""
+
+
+def test_struct_layouts_match_the_ctypes_mirror(tmp_path):
+    """sizeof/offsetof of every struct in include/pqgpu.h, from a C program compiled against the
+    header, equal the ctypes mirror in polyquant_amd/_lib.py field by field."""
+    import ctypes
+    import shutil
+    import subprocess
+
+    from polyquant_amd import _lib
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text(LAYOUT_C)
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    mirror = {"pqg_ci_desc": _lib.CiDesc, "pqg_dav_params": _lib.DavParams, "pqg_dav_report": _lib.DavReport,
+              "pqg_result": _lib.Result}
+    seen = set()
+    for line in filter(None, out):
+        name, field, off, size = line.split()
+        cls = mirror[name]
+        if field == "sizeof":
+            assert ctypes.sizeof(cls) == int(off), name
+            continue
+        desc = getattr(cls, field)
+        assert (desc.offset, desc.size) == (int(off), int(size)), (name, field)
+        seen.add((name, field))
+    # every ctypes field was checked against the header
+    for name, cls in mirror.items():
+        for f in cls._fields_:
+            assert (name, f[0]) in seen, (name, f[0])

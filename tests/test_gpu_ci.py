@@ -403,3 +403,26 @@ def test_screened_explicit_h_davidson_matches_pruned_oracle(ctx):
     assert np.max(np.abs(Ex - ref)) < 1e-9
     Hfull = np.linalg.eigvalsh(Hor)[:3] + prob.e_shift
     assert np.max(np.abs(Hfull - ref)) > 1e-6  # the threshold changes the answer
+
+
+@pytest.mark.parametrize("name", ["fci_restricted_7o_3a3b", "ep_fci_6e_2a1b_5p_1"])
+def test_stepwise_davidson_equals_whole_solve(name, ctx):
+    """pqg_dav_create / pqg_dav_step / pqg_dav_result run the same solve as pqg_ci_davidson: same
+    iterations, bit-identical energies and vectors, in FP64 and mixed precision; the result is
+    refused before the solve has finished."""
+    from polyquant_amd import _lib
+    from polyquant_amd._lib import PQG_FP64, PQG_MIXED
+
+    prob, ds, oc = _pair(name, ctx)
+    for prec in (PQG_FP64, PQG_MIXED):
+        E, X, rep = ds.davidson(2, nvec_init=5, tol=1e-7, precision=prec)
+        sv = ds.davidson_solver(2, nvec_init=5, tol=1e-7, precision=prec)
+        with pytest.raises(_lib.PqgError):
+            sv.result()
+        steps = 0
+        while not sv.step():
+            steps += 1
+        E2, X2, rep2 = sv.result()
+        sv.close()
+        assert rep2.iterations == rep.iterations and steps >= 1
+        assert np.array_equal(E, E2) and np.array_equal(X, X2)

@@ -164,7 +164,8 @@ def test_exact_diag_with_spin_penalty(ctx):
     H = oc.dense()
     P0 = oc.s2_penalty(0, 0.0)
     for kind, a, P in (("first_order", 0.3, P0), ("second_order", 5.0, P0 @ P0)):
-        ci = EPCI(prob, ctx, num_states=4, exact_diag=True, spin_penalty_type=kind, spin_penalty=[a])
+        ci = EPCI(prob, ctx, num_states=4, exact_diag=True, spin_penalty_type=kind, spin_penalty=[a],
+                  expected_S2=[0.0])
         E = ci.run()
         ref = np.linalg.eigvalsh(H + a * P)[:4] + prob.e_shift
         assert np.max(np.abs(E - ref)) < 1e-10, kind
@@ -191,9 +192,11 @@ def test_davidson_with_stored_spin_penalty(ctx):
              (wl.small_problem(313, 6, 2, 2), "second_order", [5.0]),
              (CASES["ep_fci_5e_2a2b_4p_2a1b"](), "first_order", [0.3, 0.2])]
     for prob, kind, pen in cases:
-        Ex = EPCI(prob, ctx, num_states=3, exact_diag=True, spin_penalty_type=kind, spin_penalty=pen).run()
+        e0 = [0.0] * len(pen)
+        Ex = EPCI(prob, ctx, num_states=3, exact_diag=True, spin_penalty_type=kind, spin_penalty=pen,
+                  expected_S2=e0).run()
         Ed = EPCI(prob, ctx, num_states=3, num_subspace_vec=10, convergence_E=1e-10, spin_penalty_type=kind,
-                  spin_penalty=pen).run()
+                  spin_penalty=pen, expected_S2=e0).run()
         assert np.max(np.abs(Ex - Ed)) < 1e-8, (kind, Ex, Ed)
     prob = wl.small_problem(313, 6, 2, 2)
     oc = refcpu.OracleCI(prob)

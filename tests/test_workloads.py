@@ -159,3 +159,31 @@ def test_frozen_core_ints_host_loop_follows_reference():
                      (2, (3, 6), 2, False, 0, 0, False, -1.0), (2, (3, 6), 2, False, 0, 1, False, -1.0)]
     assert np.allclose(out[0][0], 2.0 + np.eye(3) + np.ones((3, 3)))
     assert np.allclose(out[1][0], 2.0 + 2 * np.eye(2) - np.ones((2, 2)))
+
+
+def test_spin_penalty_defaults_follow_epci_setup():
+    """POLYQUANT_EPCI::setup (epci.cpp:15-26): with a penalty type and no list, spin_penalty is
+    default_spin_penalty = 0.1 per species (epci.hpp:130) and expected_S2 the SCF <S^2> per
+    species; without an SCF value the second order needs expected_S2 from the caller.  The checks
+    run before any GPU object is made."""
+    import pytest
+
+    from polyquant_amd.ci import EPCI, spin_penalty_defaults
+
+    class Prob:
+        species = [object(), object()]
+
+    class ProbScf(Prob):
+        scf_S2 = [0.75, 2.0]
+
+    assert spin_penalty_defaults(Prob(), None) == ([0.0, 0.0], [0.0, 0.0])
+    assert spin_penalty_defaults(Prob(), "first_order") == ([0.1, 0.1], [0.0, 0.0])
+    assert spin_penalty_defaults(ProbScf(), "first_order") == ([0.1, 0.1], [0.75, 2.0])
+    assert spin_penalty_defaults(ProbScf(), "second_order") == ([0.1, 0.1], [0.75, 2.0])
+    assert spin_penalty_defaults(Prob(), "second_order", [0.5, 0.0], [1.0, 2.0]) == ([0.5, 0.0], [1.0, 2.0])
+    with pytest.raises(ValueError, match="expected_S2"):
+        spin_penalty_defaults(Prob(), "second_order")
+    with pytest.raises(ValueError, match="one value per species"):
+        spin_penalty_defaults(Prob(), "first_order", [0.3])
+    with pytest.raises(ValueError, match="expected_S2"):
+        EPCI(Prob(), exact_diag=True, spin_penalty_type="second_order")
