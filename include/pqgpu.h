@@ -150,6 +150,20 @@ pqg_status pqg_ci_dense_s2(pqg_ci* ci, int part, double expected, int power, dou
  * on_device != 0: C and S are device pointers.  precision: PQG_FP64 or PQG_MIXED (FP32).   */
 pqg_status pqg_ci_sigma(pqg_ci* ci, const double* C, int64_t ldc, int k, double* S, int64_t lds,
                         int on_device, int precision);
+/* Allocation-free variant of pqg_ci_sigma (no counterpart in the reference; for callers that
+ * cannot allocate device memory: tight serving loops, harnesses that forbid it).  C and S are
+ * device pointers (reference order, N x k column-major); the call makes no hipMalloc/hipFree:
+ * every scratch buffer (the internal-order copies of C and S, transposes, the link-GEMM output
+ * chunks) comes from one fixed device array of PQG_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends.  Sizes whose copies of C and S alone exceed it (2 x 8 N k bytes, plus 8 N k with
+ * PQG_MIXED) are refused with PQG_ERR_ARG; PQG_MIXED also needs the FP32 tables that the
+ * handle's first mixed-precision pqg_ci_sigma / pqg_ci_davidson call builds (PQG_ERR_UNSUPPORTED
+ * before that), and a stored explicit H works as in pqg_ci_sigma.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other threads, other handles) race.  The
+ * call synchronises its handle's stream before it returns.                                   */
+#define PQG_NOALLOC_SCRATCH_BYTES ((uint64_t)256 << 20)
+pqg_status pqg_ci_sigma_noalloc(pqg_ci* ci, const double* C, int64_t ldc, int k, double* S, int64_t lds,
+                                int precision);
 /* Full Davidson solve on the GPU (Spectra DavidsonSymEigsSolver, SmallestAlge,
  * epci.cpp:589-608).  E: nev eigenvalues (without constant shifts); X: N x nev Ritz vectors
  * (host, reference order, column-major, may be NULL).                                      */

@@ -98,6 +98,7 @@ SIGNATURES = [
     ("pqg_ci_set_explicit", ct.c_int, [_vp, ct.c_int, ct.c_double, ct.POINTER(ct.c_int64)]),
     ("pqg_ci_add_spin_penalty", ct.c_int, [_vp, ct.c_int, ct.c_double, ct.c_int, ct.c_double]),
     ("pqg_ci_sigma", ct.c_int, [_vp, _vp, ct.c_int64, ct.c_int, _vp, ct.c_int64, ct.c_int, ct.c_int]),
+    ("pqg_ci_sigma_noalloc", ct.c_int, [_vp, _vp, ct.c_int64, ct.c_int, _vp, ct.c_int64, ct.c_int]),
     ("pqg_ci_s2", ct.c_int, [_vp, _vp, ct.c_int64, ct.c_int, _vp, ct.c_int]),
     ("pqg_ci_rdm1", ct.c_int, [_vp, _vp, ct.c_int64, ct.c_int, ct.c_int, ct.c_int, _vp, ct.c_int]),
     ("pqg_ci_davidson", ct.c_int, [_vp, ct.POINTER(DavParams), _vp, _vp, ct.c_int64, ct.POINTER(DavReport)]),

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 #include <unordered_map>
@@ -2172,6 +2173,90 @@ pqg_status pqg_ci_sigma(pqg_ci* ci, const double* C, int64_t ldc, int k, double*
       to_reference(ci, sout.get(), N, sref.get(), Nr, k, st);
       PQG_HIP(hipMemcpy2DAsync(S, lds * 8, sref.get(), Nr * 8, Nr * 8, k, hipMemcpyDeviceToHost, st));
     }
+    PQG_HIP(hipStreamSynchronize(st));
+  });
+}
+
+// ---------------------------------------------------------------------------------------
+// allocation-free sigma (pqg_ci_sigma_noalloc): every scratch buffer of the call lives in this
+// fixed device array.  Each GPU the library runs on holds its own copy until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress gives for that device.
+// ---------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[PQG_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `device`'s copy of scratch (that device must be current)
+char* noalloc_scratch(int device) {
+  PQG_REQUIRE(device >= 0 && device < kMaxDevices, PQG_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    PQG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    PQG_REQUIRE(ptr, PQG_ERR_HIP, "hipGetSymbolAddress returned no address for the fixed scratch");
+    g_scratch_ptr[device] = static_cast<char*>(ptr);
+  }
+  return g_scratch_ptr[device];
+}
+
+// installs a fixed workspace on this thread and lends the handle's own scratch buffers out of
+// the way for the call (they come back, unchanged, when the scope ends)
+struct NoAllocScope {
+  FixedScratch fs;
+  pqg_ci* ci;
+  DevBuf<char> own_scratch, own_u;
+  NoAllocScope(pqg_ci* c, char* base, size_t size) : ci(c) {
+    fs.base = base;
+    fs.size = size;
+    own_scratch = std::move(ci->scratch);
+    own_u = std::move(ci->lg.U);
+    fixed_scratch() = &fs;
+  }
+  ~NoAllocScope() {
+    fixed_scratch() = nullptr;
+    ci->scratch = std::move(own_scratch);
+    ci->lg.U = std::move(own_u);
+  }
+};
+}  // namespace
+
+pqg_status pqg_ci_sigma_noalloc(pqg_ci* ci, const double* C, int64_t ldc, int k, double* S, int64_t lds,
+                                int precision) {
+  return guard([&] {
+    PQG_REQUIRE(ci && C && S && k >= 1, PQG_ERR_ARG, "invalid argument");
+    PQG_REQUIRE(ldc >= ci->n_ref && lds >= ci->n_ref, PQG_ERR_ARG, "leading dimension smaller than N");
+    PQG_REQUIRE(precision == PQG_FP64 || precision == PQG_MIXED, PQG_ERR_ARG, "unknown precision");
+    PQG_REQUIRE(precision == PQG_FP64 || (ci->mixed_ready && (!ci->tensor.enabled || ci->tensor.f32_ready)),
+                PQG_ERR_UNSUPPORTED,
+                "PQG_MIXED without allocation needs the FP32 tables a first mixed-precision pqg_ci_sigma or "
+                "pqg_ci_davidson call builds");
+    hipStream_t st = ci->stream;
+    PQG_HIP(hipSetDevice(ci->ctx->device));
+    const int64_t N = ci->n_int;
+    const size_t vec64 = (size_t)N * k * sizeof(double);
+    const size_t need_min = 2 * vec64 + (precision == PQG_MIXED ? vec64 : 0);
+    PQG_REQUIRE(need_min <= PQG_NOALLOC_SCRATCH_BYTES, PQG_ERR_ARG,
+                "N x k too large for the allocation-free sigma: its internal-order vector copies alone exceed the " +
+                    std::to_string(PQG_NOALLOC_SCRATCH_BYTES >> 20) + " MiB fixed scratch");
+    NoAllocScope scope(ci, noalloc_scratch(ci->ctx->device), PQG_NOALLOC_SCRATCH_BYTES);
+    double* cin = reinterpret_cast<double*>(scope.fs.take(vec64));
+    double* sout = reinterpret_cast<double*>(scope.fs.take(vec64));
+    to_internal(ci, C, ldc, cin, N, k, st);
+    if (precision == PQG_FP64) {
+      sigma_internal(ci, cin, sout, k, N, st);
+    } else {
+      float* c32 = reinterpret_cast<float*>(scope.fs.take((size_t)N * k * sizeof(float)));
+      float* s32 = reinterpret_cast<float*>(scope.fs.take((size_t)N * k * sizeof(float)));
+      k_d2f<<<grid1((int64_t)N * k, 256), 256, 0, st>>>(cin, c32, (int64_t)N * k);
+      PQG_HIP(hipGetLastError());
+      sigma_internal(ci, c32, s32, k, N, st);
+      k_f2d<<<grid1((int64_t)N * k, 256), 256, 0, st>>>(s32, sout, (int64_t)N * k);
+      PQG_HIP(hipGetLastError());
+    }
+    to_reference(ci, sout, N, S, lds, k, st);
     PQG_HIP(hipStreamSynchronize(st));
   });
 }

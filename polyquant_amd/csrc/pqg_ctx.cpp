@@ -15,6 +15,18 @@ namespace pqg {
 static thread_local std::string t_err;
 void set_error(const std::string& msg) { t_err = msg; }
 
+static thread_local FixedScratch* t_fixed = nullptr;
+FixedScratch*& fixed_scratch() { return t_fixed; }
+
+char* FixedScratch::take(size_t bytes) {
+  const size_t off = (used + 255) / 256 * 256;
+  PQG_REQUIRE(off <= size && bytes <= size - off, PQG_ERR_ARG,
+              "the fixed scratch of the allocation-free entry point is too small for this call (" +
+                  std::to_string(size >> 20) + " MiB)");
+  used = off + bytes;
+  return base + off;
+}
+
 static constexpr uint64_t kAlign = 256;
 static constexpr uint64_t kMinChunk = 256ull << 20;
 // Blocks at least this large get a chunk of their own that goes back to the device when the

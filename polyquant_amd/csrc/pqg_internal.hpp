@@ -112,11 +112,24 @@ namespace pqg {
 void ctx_retain(pqg_ctx* c);
 void ctx_release(pqg_ctx* c);
 
+// Fixed device workspace of the allocation-free entry point: while one is installed on the
+// calling thread, grow() hands out views into it instead of allocating, and ProfScope records
+// no events.  base is a device address; used is the bump offset.
+struct FixedScratch {
+  char* base = nullptr;
+  size_t size = 0;
+  size_t used = 0;
+  // next `bytes` (256-byte aligned) of the workspace; PQG_ERR_ARG when it is exhausted
+  char* take(size_t bytes);
+  size_t left() const { return size > used ? size - used : 0; }
+};
+FixedScratch*& fixed_scratch();  // per host thread; nullptr when none is installed
+
 // Brackets one kernel launch with HIP events on `st` while the context is profiling.
 class ProfScope {
  public:
   ProfScope(pqg_ctx* c, hipStream_t st, const char* name, int64_t units) : c_(c), st_(st) {
-    if (!c_->profiling.load()) return;
+    if (!c_->profiling.load() || fixed_scratch()) return;
     rec_.name = name;
     rec_.units = units;
     if (hipEventCreate(&rec_.start) != hipSuccess || hipEventCreate(&rec_.stop) != hipSuccess) return;
@@ -139,7 +152,8 @@ class ProfScope {
 
 namespace pqg {
 
-// RAII device buffer from the context arena
+// RAII device buffer from the context arena, or (ctx == nullptr) a view into memory it does
+// not own, such as the fixed workspace
 template <class T>
 struct DevBuf {
   pqg_ctx* ctx = nullptr;
@@ -149,11 +163,11 @@ struct DevBuf {
   DevBuf(pqg_ctx* c, size_t count) { reset(c, count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : ctx(o.ctx), p(o.p), n(o.n) { o.ctx = nullptr; o.p = nullptr; o.n = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
     release();
     ctx = o.ctx; p = o.p; n = o.n;
-    o.p = nullptr; o.n = 0;
+    o.ctx = nullptr; o.p = nullptr; o.n = 0;
     return *this;
   }
   void reset(pqg_ctx* c, size_t count) {
@@ -162,8 +176,15 @@ struct DevBuf {
     n = count;
     p = count ? static_cast<T*>(c->arena->alloc(count * sizeof(T))) : nullptr;
   }
+  // view of `count` elements of the installed fixed workspace (nothing is allocated)
+  void view(FixedScratch* fs, size_t count) {
+    release();
+    p = reinterpret_cast<T*>(fs->take(count * sizeof(T)));
+    n = count;
+  }
   void release() {
     if (p && ctx) ctx->arena->free(p);
+    ctx = nullptr;
     p = nullptr;
     n = 0;
   }
@@ -174,9 +195,16 @@ struct DevBuf {
 
 // Grows a scratch buffer used by work queued on `st`.  The old block goes back to the shared
 // arena only after that work has finished: another stream (a serving slot) may be handed the
-// memory at once.
+// memory at once.  With a fixed workspace installed, the buffer becomes a view into it.
 template <class T>
 void grow(DevBuf<T>& b, pqg_ctx* c, size_t count, hipStream_t st) {
+  if (FixedScratch* fs = fixed_scratch()) {
+    const bool owned = b.p && b.ctx;
+    PQG_REQUIRE(!owned, PQG_ERR_ARG, "an owned buffer reached the allocation-free path");
+    if (b.p && b.n >= count) return;
+    b.view(fs, count);
+    return;
+  }
   if (b.n >= count) return;
   if (b.p) PQG_HIP(hipStreamSynchronize(st));
   b.reset(c, count);

@@ -684,7 +684,7 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
   const size_t lds = lg_lds_bytes(d, a.n_c, kqh, ksplit, sizeof(T), nt);
   // U holds n_r * L * ldu values (ldu = cw rounded up to 4); sized for the larger of the two
   // passes so it grows once
-  const int cw = lg_chunk(ci, d, a.n_c, sizeof(T), nt);
+  int cw = lg_chunk(ci, d, a.n_c, sizeof(T), nt);
   size_t need = lg_per_col(d, sizeof(T)) * (size_t)(cw + 3);
   {
     const LgPass o = lg_pass_desc(ci, 1 - pass);
@@ -693,7 +693,17 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
     need = std::max(need, lg_per_col(o, sizeof(T)) * (size_t)(lg_chunk(ci, o, n_co, sizeof(T),
                                                                         lg_nt(o, n_co, sizeof(T), ksplit, kqo)) + 3));
   }
-  grow(lg.U, ci->ctx, need, st);
+  if (FixedScratch* fs = fixed_scratch()) {
+    // allocation-free call: U is a view of what the fixed workspace has left, and the column
+    // chunk shrinks to fit it (the per-element sums do not depend on the chunking)
+    if (!lg.U.get()) lg.U.view(fs, std::min(need, fs->left() > 256 ? fs->left() - 256 : 0));
+    const int64_t fit = (int64_t)(lg.U.n / lg_per_col(d, sizeof(T))) - 3;
+    if (fit < cw) cw = (int)(fit / nt * nt);
+    PQG_REQUIRE(cw >= nt || cw >= a.n_c, PQG_ERR_ARG,
+                "the fixed scratch of the allocation-free entry point cannot hold one link-GEMM column tile");
+  } else {
+    grow(lg.U, ci->ctx, need, st);
+  }
   a.U = reinterpret_cast<T*>(lg.U.get());
   const int mb = (a.L + 15) / 16;
   // algorithmic flops of the GEMM as posed (M = L links, K = in-row rows + links, no padding)
