@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 OUT = os.path.join(HERE, "libpqgpu.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["pqg_ctx.cpp", "pqg_ci.hip", "pqg_tensor.hip", "pqg_linkgemm.hip", "pqg_davidson.hip", "pqg_ao2mo.hip", "pqg_rdm.hip",
+SOURCES = ["pqg_ctx.cpp", "pqg_ci.hip", "pqg_tensor.hip", "pqg_linkgemm.hip", "pqg_linkgemm_b3.hip", "pqg_davidson.hip", "pqg_ao2mo.hip", "pqg_rdm.hip",
            "pqg_spin.hip", "pqg_serve.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -2208,17 +2208,20 @@ struct NoAllocScope {
   FixedScratch fs;
   pqg_ci* ci;
   DevBuf<char> own_scratch, own_u;
+  DevBuf<uint32_t> own_p;
   NoAllocScope(pqg_ci* c, char* base, size_t size) : ci(c) {
     fs.base = base;
     fs.size = size;
     own_scratch = std::move(ci->scratch);
     own_u = std::move(ci->lg.U);
+    own_p = std::move(ci->lg.P);
     fixed_scratch() = &fs;
   }
   ~NoAllocScope() {
     fixed_scratch() = nullptr;
     ci->scratch = std::move(own_scratch);
     ci->lg.U = std::move(own_u);
+    ci->lg.P = std::move(own_p);
   }
 };
 }  // namespace

@@ -59,11 +59,13 @@ struct LinkTab {
   int nin = 0;                // rows of the column-set map (electron sets)
   DevBuf<int2> links;         // [nstr][L]: {J | odd << 31, Q << 16 | rev}
   DevBuf<int32_t> tbm;        // [nin][nstr]: (column index << 1 | odd) or -1 (electron sets)
+  DevBuf<int32_t> tbmT;       // off-diagonal pair rows of tbm transposed, [nstr][nb] (BF16x3 kernel)
 };
 struct LinkGemm {
   bool enabled = false;
   LinkTab tab[kSlots];
   DevBuf<char> U;             // GEMM output for one column chunk
+  DevBuf<uint32_t> P;         // BF16x3 kernel: the vector of a pass as bf16 (hi, lo) pairs
   DevBuf<float> V32s[kSlots]; // FP32 copies of the same-set coupling blocks
 };
 

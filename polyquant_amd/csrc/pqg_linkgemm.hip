@@ -34,89 +34,10 @@
 #include <type_traits>
 
 #include "pqg_ci.hpp"
+#include "pqg_linkgemm.hpp"
 #include "pqg_mfma.hpp"
 
 namespace pqg {
-
-constexpr int LG_NT = 32;  // columns per tile (FP64; FP32 takes LG_NT32 when its LDS allows)
-constexpr int LG_NT32 = 64;
-constexpr int LG_MAXL = 64;     // links per row tuple (four 16-row M blocks)
-constexpr int LG_MAXIN = 160;   // in-row rows
-constexpr int LG_MAXBO = 128;   // off-diagonal pairs of the column set (those go through the map)
-constexpr int LG_MAXSETS = 3;
-
-// Row tuples range over up to three string sets (electron alpha or beta, positron alpha,
-// positron beta): row index = sum_u i_u * rstride[u].  The "active" sets (act[] = their
-// position in the row tuple) contribute links, numbered from loff[s] in set order.
-template <typename T>
-struct LgArgs {
-  const T* C;          // this vector, n_r x n_c row-major
-  T* U;                // chunk scratch [n_r][L][ldu]
-  int nrow;            // sets in the row tuple
-  int64_t rstride[LG_MAXSETS];
-  int rns[LG_MAXSETS];
-  int nsets;           // active sets
-  int act[LG_MAXSETS];
-  const int2* links[LG_MAXSETS];  // per active set [n_s][L_s]: {J | odd << 31, Q << 16 | rev}
-  int Ls[LG_MAXSETS], nvs[LG_MAXSETS], loff[LG_MAXSETS], np[LG_MAXSETS];
-  const T* Vrc[LG_MAXSETS];              // (P|Q): P a pair of active set s, Q a pair of the column set
-  const T* Vrr[LG_MAXSETS][LG_MAXSETS];  // (P|Q): P of active set s, Q of active set t (charges folded in)
-  const T* Aone[LG_MAXSETS];             // one-string part of singles of active set s, [L_s][n_s]
-  const T* Xrow[LG_MAXSETS][LG_MAXSETS]; // spectator sums of row set u for singles of active set s, [n_u][np_s]
-  const int32_t* map;  // column-set map [nin][n_c]: (column index << 1 | odd) or -1
-  const uint64_t* cbits;  // column strings (bit j = orbital j occupied)
-  int nbo, ncd, nin;   // in-row rows: off-diagonal pairs, diagonal pairs, + the own row
-  int np_c;
-  int n_r, n_c, L, K;
-  int c0, cw, gcols;
-  int ldu;             // U row stride: cw rounded up to 4 elements, so rows start 16-byte aligned
-  int accumulate;      // gather: add to S (1) or overwrite it (0)
-  int lab_off, kbf_off, crow_off, tile_off;  // LDS byte offsets (links at 0)
-};
-
-// kernel-argument arrays indexed by a run-time set number, without a copy to scratch
-template <class X>
-__device__ __forceinline__ X pick3(const X (&v)[LG_MAXSETS], int s) {
-  return s == 0 ? v[0] : (s == 1 ? v[1] : v[2]);
-}
-template <class X>
-__device__ __forceinline__ X pick33(const X (&v)[LG_MAXSETS][LG_MAXSETS], int s, int t) {
-  return s == 0 ? pick3(v[0], t) : (s == 1 ? pick3(v[1], t) : pick3(v[2], t));
-}
-
-template <typename T>
-__device__ __forceinline__ int row_index(const LgArgs<T>& a, int64_t J, int u) {
-  return (int)((J / pick3(a.rstride, u)) % pick3(a.rns, u));
-}
-
-// Links of row J into LDS: lq[e] = {target row | odd << 31, Q << 16 | rev}, lab[e] =
-// set << 16 | hole number << 8 | particle number.
-template <typename T>
-__device__ void lg_row_links(const LgArgs<T>& a, int64_t J, int2* lq, int* lab, int tid, int nth) {
-  for (int e = tid; e < a.L; e += nth) {
-    int s = 0;
-    if (a.nsets > 1 && e >= a.loff[1]) s = 1;
-    if (a.nsets > 2 && e >= a.loff[2]) s = 2;
-    const int el = e - pick3(a.loff, s);
-    const int u = pick3(a.act, s);
-    const int64_t stride = pick3(a.rstride, u);
-    const int64_t i = row_index(a, J, u);
-    const int2 l = pick3(a.links, s)[(size_t)i * pick3(a.Ls, s) + el];
-    const int64_t tgt = J + ((int64_t)(l.x & 0x7fffffff) - i) * stride;
-    lq[e] = make_int2((int)tgt | (l.x & (int)0x80000000), (l.y & ~0xffff) | (pick3(a.loff, s) + (l.y & 0xffff)));
-    const int nv = pick3(a.nvs, s);
-    const int ae = el / nv;
-    lab[e] = (s << 16) | (ae << 8) | (el - ae * nv);
-  }
-}
-
-// p of the off-diagonal pair number k = p (p - 1) / 2 + q, q < p
-__device__ __forceinline__ int offpair_hi(int k) {
-  int p = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)k)) * 0.5f);
-  while (p * (p - 1) / 2 > k) --p;
-  while (p * (p + 1) / 2 <= k) ++p;
-  return p;
-}
 
 // Block = 4 x KSPLIT waves.  Wave w owns "slot" w & 3 (an M block, or with NSPLIT = 2 half the
 // columns of one); with KSPLIT = 2 waves w and w + 4 split the K steps in two halves whose
@@ -162,42 +83,11 @@ __global__ __launch_bounds__(256 * KSPLIT, 3 - KSPLIT) void k_lg_u(const LgArgs<
     const int abp = rowok ? lab[ep] : 0;
     const int sp = abp >> 16;
     // own row: one-string part of the single plus the other row strings' spectator sums
-    T own = T(0);
-    if (rowok) {
-      const int el = ep - pick3(a.loff, sp);
-      const int us = pick3(a.act, sp);
-      own = pick3(a.Aone, sp)[(size_t)el * pick3(a.rns, us) + row_index(a, J, us)];
-      for (int u = 0; u < a.nrow; u++) {
-        const T* X = pick33(a.Xrow, sp, u);
-        if (X) own += X[(size_t)row_index(a, J, u) * pick3(a.np, sp) + Qp];
-      }
-    }
+    const T own = rowok ? lg_own_entry(a, J, ep, Qp, sp) : T(0);
 #pragma unroll
     for (int q = 0; q < KQH; q++) {
       const int k = 4 * (kh * KQH + q) + (lane >> 4);
-      T v = T(0);
-      if (rowok && k < K) {
-        if (k < nin - 1) {
-          v = pick3(a.Vrc, sp)[(size_t)Qp * a.np_c + kbf[k]];
-        } else if (k == nin - 1) {
-          v = own;
-        } else {
-          const int e = k - nin;
-          const int abe = lab[e];
-          const int se = abe >> 16;
-          const int2 l = lq[e];
-          if (se == sp) {
-            // same-set doubles: the two links share neither hole nor particle
-            if (((abe ^ abp) & 0xff00) && ((abe ^ abp) & 0xff))
-              v = (l.x < 0 ? T(-0.5) : T(0.5)) * pick33(a.Vrr, sp, sp)[(size_t)Qp * pick3(a.np, sp) + (l.y >> 16)];
-          } else if (sp < se) {
-            // one excitation in each of two row sets: only the path whose first link lies in
-            // the lower set carries the coupling
-            v = (l.x < 0 ? T(-1) : T(1)) * pick33(a.Vrr, sp, se)[(size_t)Qp * pick3(a.np, se) + (l.y >> 16)];
-          }
-        }
-      }
-      af[q] = v;
+      af[q] = rowok ? lg_a_entry(a, lq, lab, kbf, Qp, abp, sp, own, k) : T(0);
     }
   }
 
@@ -551,6 +441,15 @@ void link_gemm_setup(pqg_ci* ci) {
       }
       lt.tbm.reset(ci->ctx, map.size());
       PQG_HIP(hipMemcpy(lt.tbm.get(), map.data(), map.size() * 4, hipMemcpyHostToDevice));
+      if (nbo > 0) {
+        // rows padded to a multiple of 4 pairs, and one tile of -1 rows past the last string
+        const int nbo4 = (nbo + 3) / 4 * 4;
+        std::vector<int32_t> mt((size_t)(s.nstr + LG_B3_NT) * nbo4, -1);
+        for (int m = 0; m < s.nstr; m++)
+          for (int kb = 0; kb < nbo; kb++) mt[(size_t)m * nbo4 + kb] = map[(size_t)kb * s.nstr + m];
+        lt.tbmT.reset(ci->ctx, mt.size());
+        PQG_HIP(hipMemcpy(lt.tbmT.get(), mt.data(), mt.size() * 4, hipMemcpyHostToDevice));
+      }
     }
   }
 }
@@ -586,6 +485,43 @@ static void lg_go_k1(const LgArgs<T>& a, int kqh, dim3 grid, size_t lds, hipStre
     case 48: lg_go<T, NSPLIT, 1, 48, LG_NT>(a, grid, lds, st); break;
     default: lg_go<T, NSPLIT, 1, 52, LG_NT>(a, grid, lds, st); break;
   }
+}
+
+// BF16x3 kernel shape for a pass: MB 32-link M blocks, KS K parts of KQ steps of 16 (0: none)
+static LgB3Shape lg_b3_shape(const LgPass& d, int n_c) {
+  LgB3Shape sh;
+  if (d.L <= 0 || d.L > 64) return sh;
+  sh.mb = d.L > 32 ? 2 : 1;
+  sh.ks = 4 / sh.mb;
+  // tile rows: excited rows, off-diagonal pairs, diagonal pairs + own row, each padded to 4
+  const int ndg = d.nin - d.nbo;
+  if (ndg > 32 || d.nbo > LG_MAXBO) return LgB3Shape{};  // one diagonal group per thread
+  const int K = (d.L + 3) / 4 * 4 + (d.nbo + 3) / 4 * 4 + (ndg + 3) / 4 * 4;
+  sh.kq = (K + 16 * sh.ks - 1) / (16 * sh.ks);
+  if (sh.kq > (sh.ks == 2 ? 7 : 4)) return LgB3Shape{};
+  sh.stride = 16 * sh.ks * sh.kq + 8;  // bf16 elements per tile column (16-byte rows)
+  auto r16 = [](size_t x) { return (x + 15) / 16 * 16; };
+  size_t off = r16((size_t)d.L * 8);
+  sh.lab_off = (int)off;
+  off = r16(off + (size_t)d.L * 4);
+  sh.kbf_off = (int)off;
+  off = r16(off + (size_t)d.nin * 4);
+  sh.crow_off = (int)off;
+  off = r16(off + (size_t)n_c * 4);
+  sh.tile_off = (int)off;
+  off = r16(off + (size_t)2 * 2 * (LG_B3_NT + 1) * sh.stride * 2);  // 2 groups x (hi, lo) planes
+  sh.part_off = (int)off;
+  off += (size_t)2 * (sh.ks - 1) * sh.mb * 1024 * 4;
+  sh.lds = off;
+  if (sh.lds > 160 * 1024) return LgB3Shape{};
+  return sh;
+}
+
+// mixed precision takes the BF16x3 kernel where its LDS fits and its 32-bit load offsets reach
+// the whole vector (PQG_LG_FP32=1: the FP32 MFMA kernel)
+static bool lg_use_b3(const LgPass& d, int n_c) {
+  static const bool fp32 = std::getenv("PQG_LG_FP32") != nullptr;
+  return !fp32 && lg_b3_shape(d, n_c).mb > 0 && ((int64_t)d.n_r * n_c + LG_B3_NT) * 4 < (int64_t(1) << 32);
 }
 
 // the 64-column tile exists only for the kernel with the K split over two wave groups
@@ -680,8 +616,25 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
   a.kbf_off = (int)r16(a.lab_off + (size_t)a.L * 4);
   a.crow_off = (int)r16(a.kbf_off + (size_t)a.nin * 4);
   a.tile_off = (int)r16(a.crow_off + (size_t)a.n_c * sizeof(T));
-  const int nt = lg_nt(d, a.n_c, sizeof(T), ksplit, kqh);
+  const bool b3 = std::is_same_v<T, float> && lg_use_b3(d, a.n_c);
+  const int nt = b3 ? LG_B3_NT : lg_nt(d, a.n_c, sizeof(T), ksplit, kqh);
   const size_t lds = lg_lds_bytes(d, a.n_c, kqh, ksplit, sizeof(T), nt);
+  LgB3Shape sh;
+  if (b3) {
+    // the vector as bf16 pairs, and the BF16x3 kernel's LDS layout
+    sh = lg_b3_shape(d, a.n_c);
+    const int64_t nv = (int64_t)d.n_r * a.n_c;
+    grow(lg.P, ci->ctx, (size_t)nv + LG_B3_NT, st);  // tile loads may run one tile past the end
+    lg_to_pairs(reinterpret_cast<const float*>(C), lg.P.get(), nv, st);
+    a.Cp = lg.P.get();
+    a.mapT = lg.tab[d.col].tbmT.get();
+    a.lab_off = sh.lab_off;
+    a.kbf_off = sh.kbf_off;
+    a.crow_off = sh.crow_off;
+    a.tile_off = sh.tile_off;
+    a.part_off = sh.part_off;
+    a.bstride = sh.stride;
+  }
   // U holds n_r * L * ldu values (ldu = cw rounded up to 4); sized for the larger of the two
   // passes so it grows once
   int cw = lg_chunk(ci, d, a.n_c, sizeof(T), nt);
@@ -719,7 +672,9 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
     {
       ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "lg_gemm_f64" : "lg_gemm_f32", flops_per_col * a.cw);
       if constexpr (sizeof(T) == 4) {
-        if (nt == LG_NT32) {
+        if (b3) {
+          lg_b3_launch(reinterpret_cast<const LgArgs<float>&>(a), sh, grid, st);
+        } else if (nt == LG_NT32) {
           if (mb <= 2) lg_go_kq<T, 2, LG_NT32>(a, ksplit, kqh, grid, lds, st);
           else lg_go_kq<T, 1, LG_NT32>(a, ksplit, kqh, grid, lds, st);
         } else {

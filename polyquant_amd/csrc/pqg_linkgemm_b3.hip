@@ -1,0 +1,311 @@
+// BF16x3 link GEMM for the mixed-precision sigma (see pqg_linkgemm.hip for the formulation).
+#include <hip/hip_runtime.h>
+
+#include "pqg_linkgemm.hpp"
+
+namespace pqg {
+
+// ---------------------------------------------------------------------------------------
+// BF16x3 link GEMM (mixed precision).  Same U as k_lg_u<float>, but each product is formed
+// from bf16 halves on the bf16 matrix cores: with x = x_hi + x_lo (x_hi = bf16(x),
+// x_lo = bf16(x - x_hi)), a b ~ a_lo b_hi + a_hi b_lo + a_hi b_hi (the dropped a_lo b_lo is
+// about 2^-16 of a b), FP32 accumulation, on v_mfma_f32_32x32x16_bf16.
+//
+// The vector enters as packed pairs (x_hi in the low 16 bits, x_lo in the high 16 bits), so the
+// tile build copies 32-bit words and negates by flipping both sign bits.  Block = 4 waves; wave
+// w owns the 32-link M block w % MB and the K part w / MB (KS parts of KQ steps of 16); the
+// partial sums of the K parts meet in LDS, added in a fixed order.  The tile is 32 columns,
+// stored [column][k] so that a lane's 8 consecutive k are two 16-byte LDS reads, double
+// buffered: the MFMAs on tile t and the build of tile t + 1 are independent, one barrier per
+// tile.  A stays in registers as bf16 halves for all tiles of a row tuple.
+// ---------------------------------------------------------------------------------------
+typedef __bf16 lg_bf8 __attribute__((ext_vector_type(8)));
+typedef float lg_f16v __attribute__((ext_vector_type(16)));
+typedef uint32_t lg_u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t lg_pair(float x) {
+  const __bf16 hi = (__bf16)x;
+  const __bf16 lo = (__bf16)(x - (float)hi);
+  return (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
+
+__global__ void k_lg_to_pairs(const float* __restrict__ x, uint32_t* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = lg_pair(x[i]);
+}
+
+// Tile rows of the BF16x3 kernel, in an order that keeps every group 16-byte aligned: the
+// excited rows (links e < L, padded to L4 = 4 ceil(L/4)), the column set's off-diagonal pairs
+// (nbo, padded to a multiple of 4), then its diagonal pairs and the own row (nin - nbo, padded);
+// the A fragments map each tile row back to k_lg_u's order (lg_a_entry).
+// HASMAP: the column set has off-diagonal pair rows (pass A); without them (pass B, nbo = 0) the
+// map rows and their loads do not exist.
+template <int MB, int KS, int KQ, bool HASMAP>
+__global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
+  constexpr int NT = LG_B3_NT;
+  constexpr int NMAP = (LG_MAXBO / 4 * NT + 255) / 256;  // 4-pair groups of the map rows per thread
+  constexpr int NEXC = (LG_MAXL / 4 * NT + 255) / 256;   // 4-link groups of the excited rows
+  static_assert(MB * KS == 4, "four waves per group");
+  extern __shared__ __align__(16) char smem[];
+  int2* lq = reinterpret_cast<int2*>(smem);
+  int* lab = reinterpret_cast<int*>(smem + a.lab_off);
+  int* kbf = reinterpret_cast<int*>(smem + a.kbf_off);
+  uint32_t* crow = reinterpret_cast<uint32_t*>(smem + a.crow_off);
+  // tile planes: [2 groups][hi, lo][NT + 1 columns][S bf16]; column NT takes the stores of idle
+  // threads, so no store is conditional
+  __bf16* Bt = reinterpret_cast<__bf16*>(smem + a.tile_off);
+  float* Pt = reinterpret_cast<float*>(smem + a.part_off);  // [2 groups][(KS - 1) * MB][16][64]
+  const int S = a.bstride;
+  const int plane = (NT + 1) * S;
+  const int64_t J = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int grp = w >> 2, wl = w & 3;  // the two groups take alternate tiles
+  const int mw = wl % MB, kh = wl / MB;
+  const int gt = tid & 255;            // thread within the group
+  const int L = a.L, nin = a.nin, nbo = a.nbo, n_c = a.n_c;
+  const int ndg = nin - nbo;           // diagonal-pair rows and the own row
+  const int L4 = (L + 3) / 4 * 4, nbo4 = (nbo + 3) / 4 * 4;
+  const int kdg = L4 + nbo4;           // first diagonal-pair row of the tile
+  lg_row_links(a, J, lq, lab, tid, 512);
+  for (int k = tid; k < nin - 1; k += 512) {
+    const int j = k - nbo;
+    kbf[k] = k < nbo ? k + offpair_hi(k) : j * (j + 1) / 2 + j;
+  }
+  for (int m = tid; m < n_c; m += 512) crow[m] = a.Cp[(size_t)J * n_c + m];
+  {
+    uint32_t* z = reinterpret_cast<uint32_t*>(Bt);
+    for (int i = tid; i < 2 * plane; i += 512) z[i] = 0u;  // both groups, both planes
+  }
+  __syncthreads();
+
+  // A fragments: lane holds A[32 mw + lane % 32][k'] for the tile rows
+  // k' = 16 (kh KQ + q) + 8 (lane / 32) + e, mapped back to k_lg_u's row order
+  lg_bf8 ahi[KQ], alo[KQ];
+  {
+    const int ep = 32 * mw + (lane & 31);
+    const bool rowok = ep < L;
+    const int Qp = rowok ? (lq[ep].y >> 16) : 0;
+    const int abp = rowok ? lab[ep] : 0;
+    const int sp = abp >> 16;
+    const float own = rowok ? lg_own_entry(a, J, ep, Qp, sp) : 0.f;
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int kk = 16 * (kh * KQ + q) + 8 * (lane >> 5) + e;
+        int k = -1;
+        if (kk < L4) {
+          if (kk < L) k = nin + kk;
+        } else if (kk < kdg) {
+          if (kk - L4 < nbo) k = kk - L4;
+        } else if (kk - kdg < ndg) {
+          k = nbo + (kk - kdg);
+        }
+        const float v = (rowok && k >= 0) ? lg_a_entry(a, lq, lab, kbf, Qp, abp, sp, own, k) : 0.f;
+        const __bf16 hi = (__bf16)v;
+        ahi[q][e] = hi;
+        alo[q][e] = (__bf16)(v - (float)hi);
+      }
+    }
+  }
+
+  // per-thread work of a tile, fixed for the block: byte offsets of its loads (relative to the
+  // tile's first column) and element offsets of its LDS stores (idle threads: column NT)
+  //   map groups: 4 consecutive off-diagonal pairs of one column (16-byte loads of mapT)
+  //   excited groups: one column of 4 consecutive links (4 loads of 4 bytes)
+  //   diagonal groups: 4 consecutive diagonal-pair/own rows of one column (from crow)
+  const int nmg = nbo4 / 4, nmgt = nmg * NT;
+  const int nxg = L4 / 4, nxgt = nxg * NT;
+  const int ndg4 = (ndg + 3) / 4, ndgt = ndg4 * NT;
+  uint32_t moff[NMAP];
+  int mst[NMAP], mcol[NMAP];
+#pragma unroll
+  for (int i = 0; i < NMAP; i++) {
+    const int g = min(gt + 256 * i, max(nmgt - 1, 0));
+    const int col = g / max(nmg, 1), kb4 = g - col * max(nmg, 1);
+    moff[i] = (uint32_t)((col * nbo4 + 4 * kb4) * 4);
+    const bool ok = gt + 256 * i < nmgt;
+    mst[i] = ok ? col * S + L4 + 4 * kb4 : NT * S;
+    mcol[i] = ok ? col : NT;
+  }
+  uint32_t xoff[NEXC][4];
+  int xst[NEXC], xcol[NEXC], xn[NEXC];
+#pragma unroll
+  for (int i = 0; i < NEXC; i++) {
+    const int g = min(gt + 256 * i, nxgt - 1);
+    const int col = g & (NT - 1), e4 = g / NT;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int e = min(4 * e4 + j, L - 1);
+      xoff[i][j] = (uint32_t)(((lq[e].x & 0x7fffffff) * (int64_t)n_c + col) * 4);
+    }
+    const bool ok = gt + 256 * i < nxgt;
+    xst[i] = ok ? col * S + 4 * e4 : NT * S;
+    xcol[i] = ok ? col : NT;
+    xn[i] = min(4, L - 4 * e4);  // rows past L in the last group are padding
+  }
+  const bool dg = gt < ndgt;
+  const int dcol = dg ? gt / ndg4 : NT, dj4 = dg ? gt - dcol * ndg4 : 0;
+  const int dst = dcol * S + kdg + 4 * dj4;
+
+  const int64_t gbeg = (int64_t)a.c0 + (int64_t)blockIdx.y * a.gcols;
+  const int64_t gend = std::min<int64_t>((int64_t)a.c0 + a.cw, gbeg + a.gcols);
+  const int ntiles = gend > gbeg ? (int)((gend - gbeg + NT - 1) / NT) : 0;
+  const int32_t* mapT = a.mapT;
+  const uint32_t* Cp = a.Cp;
+  const uint64_t* cbits = a.cbits;
+  lg_u4 tm[NMAP];
+  uint32_t tx[NEXC][4];
+  uint64_t cbd = 0;
+  auto prefetch = [&](int64_t m0) {
+    const char* mb = reinterpret_cast<const char*>(mapT + (size_t)m0 * nbo4);
+    const char* cb = reinterpret_cast<const char*>(Cp + m0);
+    if constexpr (HASMAP) {
+#pragma unroll
+      for (int i = 0; i < NMAP; i++) tm[i] = *reinterpret_cast<const lg_u4*>(mb + moff[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < NEXC; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) tx[i][j] = *reinterpret_cast<const uint32_t*>(cb + xoff[i][j]);
+    cbd = cbits[min<int64_t>(m0 + min(dcol, NT - 1), (int64_t)n_c - 1)];
+  };
+  // four pairs -> hi and lo planes (8 bytes each)
+  auto put4 = [&](__bf16* T, int off, lg_u4 v) {
+    uint2 h, l;
+    h.x = __builtin_amdgcn_perm(v[1], v[0], 0x05040100u);
+    h.y = __builtin_amdgcn_perm(v[3], v[2], 0x05040100u);
+    l.x = __builtin_amdgcn_perm(v[1], v[0], 0x07060302u);
+    l.y = __builtin_amdgcn_perm(v[3], v[2], 0x07060302u);
+    *reinterpret_cast<uint2*>(T + off) = h;
+    *reinterpret_cast<uint2*>(T + plane + off) = l;
+  };
+  // branch-free: every load is in bounds, masks select what is stored
+  auto store = [&](int64_t m0, __bf16* T) {
+    const int ncol = (int)std::min<int64_t>(NT, gend - m0);  // columns of this tile in the chunk
+    if constexpr (HASMAP) {
+#pragma unroll
+      for (int i = 0; i < NMAP; i++) {
+        const uint32_t in = mcol[i] < ncol ? 0xffffffffu : 0u;
+        lg_u4 v;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int x = (int)tm[i][j];  // -1 (no pair) or (column index << 1 | odd), index < n_c
+          const uint32_t keep = (x >= 0) ? in : 0u;
+          const uint32_t c = crow[max(x, 0) >> 1];
+          v[j] = (c ^ ((uint32_t)(x & 1) * 0x80008000u)) & keep;
+        }
+        put4(T, mst[i], v);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NEXC; i++) {
+      const uint32_t in = xcol[i] < ncol ? 0xffffffffu : 0u;
+      lg_u4 v;
+#pragma unroll
+      for (int j = 0; j < 4; j++) v[j] = tx[i][j] & (j < xn[i] ? in : 0u);
+      put4(T, xst[i], v);
+    }
+    {
+      const uint32_t c = crow[m0 + min(dcol, ncol - 1)];
+      const uint32_t in = dcol < ncol ? 0xffffffffu : 0u;
+      lg_u4 v;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int jj = 4 * dj4 + j;
+        const bool on = jj == ndg - 1 || (jj < ndg - 1 && ((cbd >> jj) & 1));
+        v[j] = c & (on ? in : 0u);
+      }
+      put4(T, dst, v);
+    }
+  };
+
+  const bool active = 32 * mw < L;
+  const bool full_rows = 32 * mw + 32 <= L;
+  // lane's output offset in U relative to the row block (32-bit), rows 8 (r / 4) + r % 4 added
+  // per register as a uniform offset
+  const uint32_t uoff = (uint32_t)((4 * (lane >> 5)) * a.ldu + (lane & 31));
+  __bf16* Tg = Bt + 2 * grp * plane;  // this group's buffer
+  float* P = Pt + grp * (KS - 1) * MB * 1024;
+  // Tile i belongs to group i % 2.  Phase p: the owner of tile p builds it (and stores the output
+  // of its tile p - 2); the other group runs the MFMAs of tile p - 1.  One barrier per phase.
+  if (grp < ntiles) prefetch(gbeg + (int64_t)NT * grp);
+  lg_f16v acc = {};
+  for (int p = 0; p <= ntiles + 1; p++) {
+    if ((p & 1) == grp) {
+      if (p >= 2 && active && kh == 0) {
+        const int64_t m0 = gbeg + (int64_t)NT * (p - 2);
+        float* ub = a.U + ((size_t)J * L + 32 * mw) * a.ldu + (m0 - a.c0);
+        const bool full = full_rows && gend - m0 >= NT;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          float v = acc[r];
+#pragma unroll
+          for (int h2 = 1; h2 < KS; h2++) v += P[(((h2 - 1) * MB + mw) * 16 + r) * 64 + lane];
+          const int ro = 8 * (r >> 2) + (r & 3);
+          float* row = ub + (size_t)ro * a.ldu;
+          if (full || (32 * mw + ro + 4 * (lane >> 5) < L && m0 + (lane & 31) < gend)) row[uoff] = v;
+        }
+      }
+      if (p < ntiles) {
+        store(gbeg + (int64_t)NT * p, Tg);
+        if (p + 2 < ntiles) prefetch(gbeg + (int64_t)NT * (p + 2));
+      }
+    } else if (p >= 1 && p <= ntiles) {
+      acc = lg_f16v{};
+      if (active) {
+        const __bf16* T = Tg + (lane & 31) * S + 8 * (lane >> 5);
+#pragma unroll
+        for (int q = 0; q < KQ; q++) {
+          const lg_bf8 bh = *reinterpret_cast<const lg_bf8*>(T + 16 * (kh * KQ + q));
+          const lg_bf8 bl = *reinterpret_cast<const lg_bf8*>(T + plane + 16 * (kh * KQ + q));
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[q], bh, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[q], bl, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[q], bh, acc, 0, 0, 0);
+        }
+        if (KS > 1 && kh > 0) {
+#pragma unroll
+          for (int r = 0; r < 16; r++) P[(((kh - 1) * MB + mw) * 16 + r) * 64 + lane] = acc[r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int MB, int KS, int KQ>
+static void lg_b3_go(const LgArgs<float>& a, dim3 grid, size_t lds, hipStream_t st) {
+  auto kern = a.nbo > 0 ? k_lg_b3<MB, KS, KQ, true> : k_lg_b3<MB, KS, KQ, false>;
+  PQG_REQUIRE(a.nbo == 0 || a.mapT, PQG_ERR_ARG, "BF16x3 link GEMM: pair rows without their map");
+  PQG_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  kern<<<grid, 512, lds, st>>>(a);
+}
+void lg_b3_launch(const LgArgs<float>& a, const LgB3Shape& sh, dim3 grid, hipStream_t st) {
+  if (sh.mb == 2) {
+    switch (sh.kq) {
+      case 1: lg_b3_go<2, 2, 1>(a, grid, sh.lds, st); break;
+      case 2: lg_b3_go<2, 2, 2>(a, grid, sh.lds, st); break;
+      case 3: lg_b3_go<2, 2, 3>(a, grid, sh.lds, st); break;
+      case 4: lg_b3_go<2, 2, 4>(a, grid, sh.lds, st); break;
+      case 5: lg_b3_go<2, 2, 5>(a, grid, sh.lds, st); break;
+      case 6: lg_b3_go<2, 2, 6>(a, grid, sh.lds, st); break;
+      default: lg_b3_go<2, 2, 7>(a, grid, sh.lds, st); break;
+    }
+  } else {
+    switch (sh.kq) {
+      case 1: lg_b3_go<1, 4, 1>(a, grid, sh.lds, st); break;
+      case 2: lg_b3_go<1, 4, 2>(a, grid, sh.lds, st); break;
+      case 3: lg_b3_go<1, 4, 3>(a, grid, sh.lds, st); break;
+      default: lg_b3_go<1, 4, 4>(a, grid, sh.lds, st); break;
+    }
+  }
+}
+
+
+void lg_to_pairs(const float* x, uint32_t* y, int64_t n, hipStream_t st) {
+  k_lg_to_pairs<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(x, y, n);
+  PQG_HIP(hipGetLastError());
+}
+
+}  // namespace pqg
