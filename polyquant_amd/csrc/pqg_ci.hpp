@@ -59,7 +59,8 @@ struct LinkTab {
   int nin = 0;                // rows of the column-set map (electron sets)
   DevBuf<int2> links;         // [nstr][L]: {J | odd << 31, Q << 16 | rev}
   DevBuf<int32_t> tbm;        // [nin][nstr]: (column index << 1 | odd) or -1 (electron sets)
-  DevBuf<int32_t> tbmT;       // off-diagonal pair rows of tbm transposed, [nstr][nb] (BF16x3 kernel)
+  DevBuf<int32_t> tbmT;       // BF16x3 kernel: off-diagonal pair rows of tbm as [nstr + 32][nb4] words
+                              // (byte offset of the pair in the staged row | odd << 31)
 };
 struct LinkGemm {
   bool enabled = false;

@@ -443,10 +443,16 @@ void link_gemm_setup(pqg_ci* ci) {
       PQG_HIP(hipMemcpy(lt.tbm.get(), map.data(), map.size() * 4, hipMemcpyHostToDevice));
       if (nbo > 0) {
         // rows padded to a multiple of 4 pairs, and one tile of -1 rows past the last string
+        // words: byte offset of the target pair in the kernel's staged row | odd << 31, and for
+        // "no pair" the offset of the zero word after the row
         const int nbo4 = (nbo + 3) / 4 * 4;
-        std::vector<int32_t> mt((size_t)(s.nstr + LG_B3_NT) * nbo4, -1);
+        const uint32_t zero = (uint32_t)s.nstr * 4u;
+        std::vector<uint32_t> mt((size_t)(s.nstr + LG_B3_NT) * nbo4, zero);
         for (int m = 0; m < s.nstr; m++)
-          for (int kb = 0; kb < nbo; kb++) mt[(size_t)m * nbo4 + kb] = map[(size_t)kb * s.nstr + m];
+          for (int kb = 0; kb < nbo; kb++) {
+            const int32_t x = map[(size_t)kb * s.nstr + m];
+            if (x >= 0) mt[(size_t)m * nbo4 + kb] = ((uint32_t)(x >> 1) * 4u) | ((uint32_t)(x & 1) << 31);
+          }
         lt.tbmT.reset(ci->ctx, mt.size());
         PQG_HIP(hipMemcpy(lt.tbmT.get(), mt.data(), mt.size() * 4, hipMemcpyHostToDevice));
       }
@@ -501,13 +507,14 @@ static LgB3Shape lg_b3_shape(const LgPass& d, int n_c) {
   if (sh.kq > (sh.ks == 2 ? 7 : 4)) return LgB3Shape{};
   sh.stride = 16 * sh.ks * sh.kq + 8;  // bf16 elements per tile column (16-byte rows)
   auto r16 = [](size_t x) { return (x + 15) / 16 * 16; };
-  size_t off = r16((size_t)d.L * 8);
+  // the staged row (n_c pairs and a zero word) at offset 0, then links, pair numbers, tiles
+  size_t off = r16((size_t)(n_c + 1) * 4);
+  sh.lq_off = (int)off;
+  off = r16(off + (size_t)d.L * 8);
   sh.lab_off = (int)off;
   off = r16(off + (size_t)d.L * 4);
   sh.kbf_off = (int)off;
   off = r16(off + (size_t)d.nin * 4);
-  sh.crow_off = (int)off;
-  off = r16(off + (size_t)n_c * 4);
   sh.tile_off = (int)off;
   off = r16(off + (size_t)2 * 2 * (LG_B3_NT + 1) * sh.stride * 2);  // 2 groups x (hi, lo) planes
   sh.part_off = (int)off;
@@ -627,10 +634,12 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
     grow(lg.P, ci->ctx, (size_t)nv + LG_B3_NT, st);  // tile loads may run one tile past the end
     lg_to_pairs(reinterpret_cast<const float*>(C), lg.P.get(), nv, st);
     a.Cp = lg.P.get();
+    a.cp_bytes = (uint32_t)(((size_t)nv + LG_B3_NT) * 4);
     a.mapT = lg.tab[d.col].tbmT.get();
+    a.mapt_bytes = (uint32_t)(lg.tab[d.col].tbmT.n * 4);
+    a.lq_off = sh.lq_off;
     a.lab_off = sh.lab_off;
     a.kbf_off = sh.kbf_off;
-    a.crow_off = sh.crow_off;
     a.tile_off = sh.tile_off;
     a.part_off = sh.part_off;
     a.bstride = sh.stride;

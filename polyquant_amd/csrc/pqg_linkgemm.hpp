@@ -43,10 +43,13 @@ struct LgArgs {
   int ldu;             // U row stride: cw rounded up to 4 elements, so rows start 16-byte aligned
   int accumulate;      // gather: add to S (1) or overwrite it (0)
   int lab_off, kbf_off, crow_off, tile_off;  // LDS byte offsets (links at 0)
-  // BF16x3 kernel: this vector as bf16 pairs, the column-set map transposed to [m][kb]
+  // BF16x3 kernel: this vector as bf16 pairs; the column-set map transposed to [m][kb], each
+  // word the byte offset of the target pair in the staged row | odd << 31
   const uint32_t* Cp;
   const int32_t* mapT;
   int part_off;        // LDS byte offset of the partial sums of the K split
+  int lq_off;          // LDS byte offset of the links (the staged row is at offset 0)
+  uint32_t cp_bytes, mapt_bytes;  // sizes of Cp and mapT (buffer loads past them return 0)
   int bstride;         // LDS tile row stride in 32-bit words (one row per column)
 };
 
@@ -140,7 +143,7 @@ __device__ __forceinline__ T lg_a_entry(const LgArgs<T>& a, const int2* lq, cons
 struct LgB3Shape {
   int mb = 0, ks = 0, kq = 0, stride = 0;
   size_t lds = 0;
-  int lab_off = 0, kbf_off = 0, crow_off = 0, tile_off = 0, part_off = 0;
+  int lq_off = 0, lab_off = 0, kbf_off = 0, tile_off = 0, part_off = 0;
 };
 
 constexpr int LG_B3_NT = 32;  // columns per BF16x3 tile

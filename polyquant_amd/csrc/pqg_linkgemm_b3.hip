@@ -34,23 +34,63 @@ __global__ void k_lg_to_pairs(const float* __restrict__ x, uint32_t* __restrict_
   if (i < n) y[i] = lg_pair(x[i]);
 }
 
+// Source of A_J[ep][k] (k in k_lg_u's row order, -1 = padding) as a pointer into the integral
+// tables and a factor (0 for padding, the own row and vanishing entries; the pointer is then any
+// valid table entry), so that the loads of many entries can be issued together.
+__device__ __forceinline__ const float* lg_a_src(const LgArgs<float>& a, const int2* lq, const int* lab,
+                                                 const int* kbf, int Qp, int abp, int sp, int k, float& sc) {
+  const float* base = pick3(a.Vrc, sp);
+  sc = 0.f;
+  if (k < 0 || k >= a.K || k == a.nin - 1) return base;
+  if (k < a.nin - 1) {
+    sc = 1.f;
+    return base + (size_t)Qp * a.np_c + kbf[k];
+  }
+  const int e = k - a.nin;
+  const int abe = lab[e];
+  const int se = abe >> 16;
+  const int2 l = lq[e];
+  if (se == sp) {
+    // same-set doubles: the two links share neither hole nor particle
+    if (((abe ^ abp) & 0xff00) && ((abe ^ abp) & 0xff)) {
+      sc = l.x < 0 ? -0.5f : 0.5f;
+      return pick33(a.Vrr, sp, sp) + (size_t)Qp * pick3(a.np, sp) + (l.y >> 16);
+    }
+  } else if (sp < se) {
+    // one excitation in each of two row sets: the path whose first link lies in the lower set
+    sc = l.x < 0 ? -1.f : 1.f;
+    return pick33(a.Vrr, sp, se) + (size_t)Qp * pick3(a.np, se) + (l.y >> 16);
+  }
+  return base;
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t lg_rsrc(const void* p, uint32_t bytes) {
+  // raw buffer (stride 0) of `bytes` bytes: loads past the end return 0, stores there are dropped
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
 // Tile rows of the BF16x3 kernel, in an order that keeps every group 16-byte aligned: the
 // excited rows (links e < L, padded to L4 = 4 ceil(L/4)), the column set's off-diagonal pairs
 // (nbo, padded to a multiple of 4), then its diagonal pairs and the own row (nin - nbo, padded);
-// the A fragments map each tile row back to k_lg_u's order (lg_a_entry).
+// A maps each tile row back to k_lg_u's order.  Padding rows of A are zero, so the tile may hold
+// any finite value there, and a tile column only ever reaches the same output column, so the
+// columns past the chunk (never stored) need no masking either.
 // HASMAP: the column set has off-diagonal pair rows (pass A); without them (pass B, nbo = 0) the
-// map rows and their loads do not exist.
+// map rows and their loads do not exist.  LDS: the staged row at offset 0 (n_c pairs and a zero
+// word that the map's "no pair" entries point to), links, pair numbers, the tile buffers (first
+// used to stage A), the partial sums.
 template <int MB, int KS, int KQ, bool HASMAP>
 __global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
   constexpr int NT = LG_B3_NT;
   constexpr int NMAP = (LG_MAXBO / 4 * NT + 255) / 256;  // 4-pair groups of the map rows per thread
   constexpr int NEXC = (LG_MAXL / 4 * NT + 255) / 256;   // 4-link groups of the excited rows
+  constexpr int KP = 16 * KS * KQ;                       // tile rows
   static_assert(MB * KS == 4, "four waves per group");
   extern __shared__ __align__(16) char smem[];
-  int2* lq = reinterpret_cast<int2*>(smem);
+  uint32_t* crow = reinterpret_cast<uint32_t*>(smem);
+  int2* lq = reinterpret_cast<int2*>(smem + a.lq_off);
   int* lab = reinterpret_cast<int*>(smem + a.lab_off);
   int* kbf = reinterpret_cast<int*>(smem + a.kbf_off);
-  uint32_t* crow = reinterpret_cast<uint32_t*>(smem + a.crow_off);
   // tile planes: [2 groups][hi, lo][NT + 1 columns][S bf16]; column NT takes the stores of idle
   // threads, so no store is conditional
   __bf16* Bt = reinterpret_cast<__bf16*>(smem + a.tile_off);
@@ -72,28 +112,34 @@ __global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
     const int j = k - nbo;
     kbf[k] = k < nbo ? k + offpair_hi(k) : j * (j + 1) / 2 + j;
   }
-  for (int m = tid; m < n_c; m += 512) crow[m] = a.Cp[(size_t)J * n_c + m];
   {
-    uint32_t* z = reinterpret_cast<uint32_t*>(Bt);
-    for (int i = tid; i < 2 * plane; i += 512) z[i] = 0u;  // both groups, both planes
+    // the row C(J, :) as pairs, eight loads in flight per thread
+    const uint32_t* src = a.Cp + (size_t)J * n_c;
+    for (int m0 = 0; m0 < n_c; m0 += 512 * 8) {
+      uint32_t v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = src[min(m0 + 512 * u + tid, n_c - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (m0 + 512 * u + tid < n_c) crow[m0 + 512 * u + tid] = v[u];
+    }
+    if (tid == 0) crow[n_c] = 0u;
   }
   __syncthreads();
 
-  // A fragments: lane holds A[32 mw + lane % 32][k'] for the tile rows
-  // k' = 16 (kh KQ + q) + 8 (lane / 32) + e, mapped back to k_lg_u's row order
-  lg_bf8 ahi[KQ], alo[KQ];
+  // A staged in the tile region as bf16 halves [hi, lo][32 MB rows][S], entries from all threads
+  // with their loads issued eight at a time; the own row (several tables) after it
   {
-    const int ep = 32 * mw + (lane & 31);
-    const bool rowok = ep < L;
-    const int Qp = rowok ? (lq[ep].y >> 16) : 0;
-    const int abp = rowok ? lab[ep] : 0;
-    const int sp = abp >> 16;
-    const float own = rowok ? lg_own_entry(a, J, ep, Qp, sp) : 0.f;
+    __bf16* Ah = Bt;
+    __bf16* Al = Bt + 32 * MB * S;
+    constexpr int NA = 32 * MB * KP;
+    for (int i0 = 0; i0 < NA; i0 += 512 * 8) {
+      const float* src[8];
+      float sc[8];
 #pragma unroll
-    for (int q = 0; q < KQ; q++) {
-#pragma unroll
-      for (int e = 0; e < 8; e++) {
-        const int kk = 16 * (kh * KQ + q) + 8 * (lane >> 5) + e;
+      for (int u = 0; u < 8; u++) {
+        const int idx = min(i0 + 512 * u + tid, NA - 1);
+        const int ep = idx / KP, kk = idx - ep * KP;
         int k = -1;
         if (kk < L4) {
           if (kk < L) k = nin + kk;
@@ -102,13 +148,51 @@ __global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
         } else if (kk - kdg < ndg) {
           k = nbo + (kk - kdg);
         }
-        const float v = (rowok && k >= 0) ? lg_a_entry(a, lq, lab, kbf, Qp, abp, sp, own, k) : 0.f;
-        const __bf16 hi = (__bf16)v;
-        ahi[q][e] = hi;
-        alo[q][e] = (__bf16)(v - (float)hi);
+        if (ep >= L) k = -1;
+        const int epc = min(ep, L - 1);
+        const int abp = lab[epc];
+        src[u] = lg_a_src(a, lq, lab, kbf, lq[epc].y >> 16, abp, abp >> 16, k, sc[u]);
+      }
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = sc[u] * *src[u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int idx = i0 + 512 * u + tid;
+        if (idx < NA) {
+          const int ep = idx / KP, kk = idx - ep * KP;
+          const __bf16 hi = (__bf16)v[u];
+          Ah[ep * S + kk] = hi;
+          Al[ep * S + kk] = (__bf16)(v[u] - (float)hi);
+        }
       }
     }
+    __syncthreads();
+    if (tid < L) {
+      const int abp = lab[tid];
+      const float own = lg_own_entry(a, J, tid, lq[tid].y >> 16, abp >> 16);
+      const __bf16 hi = (__bf16)own;
+      Ah[tid * S + kdg + ndg - 1] = hi;
+      Al[tid * S + kdg + ndg - 1] = (__bf16)(own - (float)hi);
+    }
   }
+  __syncthreads();
+  // A fragments: lane holds A[32 mw + lane % 32][16 (kh KQ + q) + 8 (lane / 32) + e]
+  lg_bf8 ahi[KQ], alo[KQ];
+  {
+    const __bf16* Ah = Bt + (32 * mw + (lane & 31)) * S + 8 * (lane >> 5);
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+      ahi[q] = *reinterpret_cast<const lg_bf8*>(Ah + 16 * (kh * KQ + q));
+      alo[q] = *reinterpret_cast<const lg_bf8*>(Ah + 32 * MB * S + 16 * (kh * KQ + q));
+    }
+  }
+  __syncthreads();
+  {
+    uint32_t* z = reinterpret_cast<uint32_t*>(Bt);
+    for (int i = tid; i < 2 * plane; i += 512) z[i] = 0u;  // both groups, both planes
+  }
+  __syncthreads();
 
   // per-thread work of a tile, fixed for the block: byte offsets of its loads (relative to the
   // tile's first column) and element offsets of its LDS stores (idle threads: column NT)
@@ -119,56 +203,54 @@ __global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
   const int nxg = L4 / 4, nxgt = nxg * NT;
   const int ndg4 = (ndg + 3) / 4, ndgt = ndg4 * NT;
   uint32_t moff[NMAP];
-  int mst[NMAP], mcol[NMAP];
+  int mst[NMAP];
 #pragma unroll
   for (int i = 0; i < NMAP; i++) {
     const int g = min(gt + 256 * i, max(nmgt - 1, 0));
     const int col = g / max(nmg, 1), kb4 = g - col * max(nmg, 1);
     moff[i] = (uint32_t)((col * nbo4 + 4 * kb4) * 4);
-    const bool ok = gt + 256 * i < nmgt;
-    mst[i] = ok ? col * S + L4 + 4 * kb4 : NT * S;
-    mcol[i] = ok ? col : NT;
+    mst[i] = (gt + 256 * i < nmgt) ? col * S + L4 + 4 * kb4 : NT * S;
   }
   uint32_t xoff[NEXC][4];
-  int xst[NEXC], xcol[NEXC], xn[NEXC];
+  int xst[NEXC];
 #pragma unroll
   for (int i = 0; i < NEXC; i++) {
     const int g = min(gt + 256 * i, nxgt - 1);
     const int col = g & (NT - 1), e4 = g / NT;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const int e = min(4 * e4 + j, L - 1);
+      const int e = min(4 * e4 + j, L - 1);  // rows past L: any finite value (A is zero there)
       xoff[i][j] = (uint32_t)(((lq[e].x & 0x7fffffff) * (int64_t)n_c + col) * 4);
     }
-    const bool ok = gt + 256 * i < nxgt;
-    xst[i] = ok ? col * S + 4 * e4 : NT * S;
-    xcol[i] = ok ? col : NT;
-    xn[i] = min(4, L - 4 * e4);  // rows past L in the last group are padding
+    xst[i] = (gt + 256 * i < nxgt) ? col * S + 4 * e4 : NT * S;
   }
   const bool dg = gt < ndgt;
   const int dcol = dg ? gt / ndg4 : NT, dj4 = dg ? gt - dcol * ndg4 : 0;
   const int dst = dcol * S + kdg + 4 * dj4;
+  // occupancy bits of the diagonal rows of this thread's group, the own row and padding set
+  const uint64_t dpad = ~0ull << (ndg - 1);
 
   const int64_t gbeg = (int64_t)a.c0 + (int64_t)blockIdx.y * a.gcols;
   const int64_t gend = std::min<int64_t>((int64_t)a.c0 + a.cw, gbeg + a.gcols);
   const int ntiles = gend > gbeg ? (int)((gend - gbeg + NT - 1) / NT) : 0;
-  const int32_t* mapT = a.mapT;
-  const uint32_t* Cp = a.Cp;
+  const __amdgpu_buffer_rsrc_t rC = lg_rsrc(a.Cp, a.cp_bytes);
+  const __amdgpu_buffer_rsrc_t rM = lg_rsrc(a.mapT, a.mapt_bytes);
   const uint64_t* cbits = a.cbits;
   lg_u4 tm[NMAP];
   uint32_t tx[NEXC][4];
   uint64_t cbd = 0;
   auto prefetch = [&](int64_t m0) {
-    const char* mb = reinterpret_cast<const char*>(mapT + (size_t)m0 * nbo4);
-    const char* cb = reinterpret_cast<const char*>(Cp + m0);
     if constexpr (HASMAP) {
+      const int so = (int)(m0 * nbo4 * 4);
 #pragma unroll
-      for (int i = 0; i < NMAP; i++) tm[i] = *reinterpret_cast<const lg_u4*>(mb + moff[i]);
+      for (int i = 0; i < NMAP; i++)
+        tm[i] = __builtin_bit_cast(lg_u4, __builtin_amdgcn_raw_buffer_load_b128(rM, (int)moff[i], so, 0));
     }
+    const int so = (int)(m0 * 4);
 #pragma unroll
     for (int i = 0; i < NEXC; i++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) tx[i][j] = *reinterpret_cast<const uint32_t*>(cb + xoff[i][j]);
+      for (int j = 0; j < 4; j++) tx[i][j] = __builtin_amdgcn_raw_buffer_load_b32(rC, (int)xoff[i][j], so, 0);
     cbd = cbits[min<int64_t>(m0 + min(dcol, NT - 1), (int64_t)n_c - 1)];
   };
   // four pairs -> hi and lo planes (8 bytes each)
@@ -181,51 +263,46 @@ __global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
     *reinterpret_cast<uint2*>(T + off) = h;
     *reinterpret_cast<uint2*>(T + plane + off) = l;
   };
-  // branch-free: every load is in bounds, masks select what is stored
+  // branch-free tile build: map words are (byte offset into the staged row | odd << 31), the
+  // "no pair" word points at the zero word
   auto store = [&](int64_t m0, __bf16* T) {
-    const int ncol = (int)std::min<int64_t>(NT, gend - m0);  // columns of this tile in the chunk
     if constexpr (HASMAP) {
 #pragma unroll
       for (int i = 0; i < NMAP; i++) {
-        const uint32_t in = mcol[i] < ncol ? 0xffffffffu : 0u;
         lg_u4 v;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          const int x = (int)tm[i][j];  // -1 (no pair) or (column index << 1 | odd), index < n_c
-          const uint32_t keep = (x >= 0) ? in : 0u;
-          const uint32_t c = crow[max(x, 0) >> 1];
-          v[j] = (c ^ ((uint32_t)(x & 1) * 0x80008000u)) & keep;
+          const uint32_t x = tm[i][j];
+          const uint32_t c = *reinterpret_cast<const uint32_t*>(smem + (x & 0x7fffffffu));
+          v[j] = c ^ ((uint32_t)((int32_t)x >> 31) & 0x80008000u);
         }
         put4(T, mst[i], v);
       }
     }
 #pragma unroll
     for (int i = 0; i < NEXC; i++) {
-      const uint32_t in = xcol[i] < ncol ? 0xffffffffu : 0u;
       lg_u4 v;
 #pragma unroll
-      for (int j = 0; j < 4; j++) v[j] = tx[i][j] & (j < xn[i] ? in : 0u);
+      for (int j = 0; j < 4; j++) v[j] = tx[i][j];
       put4(T, xst[i], v);
     }
     {
-      const uint32_t c = crow[m0 + min(dcol, ncol - 1)];
-      const uint32_t in = dcol < ncol ? 0xffffffffu : 0u;
+      const uint32_t c = crow[(int)std::min<int64_t>(m0 + dcol, (int64_t)n_c)];
+      const uint64_t bits = (cbd | dpad) >> (4 * dj4);
       lg_u4 v;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int jj = 4 * dj4 + j;
-        const bool on = jj == ndg - 1 || (jj < ndg - 1 && ((cbd >> jj) & 1));
-        v[j] = c & (on ? in : 0u);
-      }
+      for (int j = 0; j < 4; j++) v[j] = c & (0u - (uint32_t)((bits >> j) & 1));
       put4(T, dst, v);
     }
   };
 
   const bool active = 32 * mw < L;
   const bool full_rows = 32 * mw + 32 <= L;
-  // lane's output offset in U relative to the row block (32-bit), rows 8 (r / 4) + r % 4 added
-  // per register as a uniform offset
-  const uint32_t uoff = (uint32_t)((4 * (lane >> 5)) * a.ldu + (lane & 31));
+  // U rows of this wave as a buffer; the lane's byte offset, rows 8 (r / 4) + r % 4 and the tile
+  // column as a uniform offset
+  const __amdgpu_buffer_rsrc_t rU =
+      lg_rsrc(a.U + ((size_t)J * L + 32 * mw) * a.ldu, (uint32_t)(32 * a.ldu * 4));
+  const int uoffb = ((4 * (lane >> 5)) * a.ldu + (lane & 31)) * 4;
   __bf16* Tg = Bt + 2 * grp * plane;  // this group's buffer
   float* P = Pt + grp * (KS - 1) * MB * 1024;
   // Tile i belongs to group i % 2.  Phase p: the owner of tile p builds it (and stores the output
@@ -236,16 +313,30 @@ __global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
     if ((p & 1) == grp) {
       if (p >= 2 && active && kh == 0) {
         const int64_t m0 = gbeg + (int64_t)NT * (p - 2);
-        float* ub = a.U + ((size_t)J * L + 32 * mw) * a.ldu + (m0 - a.c0);
-        const bool full = full_rows && gend - m0 >= NT;
+        const int cofs = (int)(m0 - a.c0) * 4;
+        float v[16];
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-          float v = acc[r];
+          v[r] = acc[r];
 #pragma unroll
-          for (int h2 = 1; h2 < KS; h2++) v += P[(((h2 - 1) * MB + mw) * 16 + r) * 64 + lane];
-          const int ro = 8 * (r >> 2) + (r & 3);
-          float* row = ub + (size_t)ro * a.ldu;
-          if (full || (32 * mw + ro + 4 * (lane >> 5) < L && m0 + (lane & 31) < gend)) row[uoff] = v;
+          for (int h2 = 1; h2 < KS; h2++) v[r] += P[(((h2 - 1) * MB + mw) * 16 + r) * 64 + lane];
+        }
+        if (full_rows && gend - m0 >= NT) {
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const int ro = 8 * (r >> 2) + (r & 3);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v[r]), rU, uoffb,
+                                                  ro * a.ldu * 4 + cofs, 0);
+          }
+        } else {
+          const bool colok = m0 + (lane & 31) < gend;
+#pragma unroll
+          for (int r = 0; r < 16; r++) {
+            const int ro = 8 * (r >> 2) + (r & 3);
+            if (colok && 32 * mw + ro + 4 * (lane >> 5) < L)
+              __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v[r]), rU, uoffb,
+                                                    ro * a.ldu * 4 + cofs, 0);
+          }
         }
       }
       if (p < ntiles) {
