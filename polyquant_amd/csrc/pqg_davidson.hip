@@ -31,11 +31,6 @@ namespace pqg {
 
 constexpr int kTB = 256;
 
-// Partials of A^T B for tall-skinny A (nrows x na) and B (nrows x nb), column-major: block
-// (x, y, z) sums A[r][a] * B[r][b] for column a = y, b in [4z, 4z+4), rows r of chunk x into
-// part[x][a][b].  One column of A per block keeps the per-block reduction to four values and
-// lets many blocks stream A concurrently.
-
 // out[a][b] = sum over the chunks of the partials in a fixed order (one block per output)
 __global__ __launch_bounds__(256) void k_tn_reduce(const double* __restrict__ part, int nchunk, int na, int nb,
                                                    double* __restrict__ out, int ldo) {
@@ -49,48 +44,49 @@ __global__ __launch_bounds__(256) void k_tn_reduce(const double* __restrict__ pa
   if (threadIdx.x == 0) out[(o / nb) + (o % nb) * ldo] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-__global__ __launch_bounds__(kTB) void k_tn_partial(const double* __restrict__ A, int64_t lda, int na,
-                                                    const double* __restrict__ B, int64_t ldb, int nb, int64_t nrows,
-                                                    int64_t chunk, double* __restrict__ part) {
-  const int a = blockIdx.y, b0 = blockIdx.z * 4;
-  const int64_t r0 = (int64_t)blockIdx.x * chunk;
-  const int64_t r1 = min(nrows, r0 + chunk);
-  const double* __restrict__ Aa = A + (int64_t)a * lda;
-  const int nbl = min(4, nb - b0);
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  int64_t r = r0 + threadIdx.x;
-  for (; r + 3 * kTB < r1; r += 4 * kTB) {
-    double av[4], bv[4][4];
+// Partials of A^T B for tall-skinny A (nrows x na) and B (nrows x nb), column-major: wave g sums
+// rows [g rpw, (g + 1) rpw) into part[g][a][b].  Lane = row; 16 columns of A and 4 of B are held
+// in registers at a time, so a row of A is read once per 4 columns of B and a row of B once per
+// 16 columns of A (each operand once when na <= 16 or nb <= 4).  Every wave sums its rows in the
+// same order and the partials are summed in a fixed order: deterministic.
+__global__ __launch_bounds__(256) void k_tn_waves(const double* __restrict__ A, int64_t lda, int na,
+                                                  const double* __restrict__ B, int64_t ldb, int nb, int64_t nrows,
+                                                  int64_t rpw, double* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t r0 = g * rpw, r1 = min(nrows, r0 + rpw);
+  if (r0 >= nrows) return;  // waves past the last slice (the launch rounds up to whole blocks)
+  for (int b0 = 0; b0 < nb; b0 += 4) {
+    const int nbl = min(4, nb - b0);
+    for (int a0 = 0; a0 < na; a0 += 16) {
+      const int nal = min(16, na - a0);
+      double acc[16][4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) av[u] = Aa[r + u * kTB];
+      for (int i = 0; i < 16; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+      for (int64_t r = r0 + lane; r < r1; r += 64) {
+        double y[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) bv[j][u] = (j < nbl) ? B[r + u * kTB + (b0 + j) * ldb] : 0.0;
+        for (int j = 0; j < 4; j++) y[j] = j < nbl ? B[r + (int64_t)(b0 + j) * ldb] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+        for (int i = 0; i < 16; i++) {
+          if (i < nal) {
+            const double x = A[r + (int64_t)(a0 + i) * lda];
 #pragma unroll
-      for (int u = 0; u < 4; u++) acc[j] += av[u] * bv[j][u];
-  }
-  for (; r < r1; r += kTB) {
-    const double av = Aa[r];
+            for (int j = 0; j < 4; j++) acc[i][j] += x * y[j];
+          }
+        }
+      }
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[j] += (j < nbl) ? av * B[r + (b0 + j) * ldb] : 0.0;
-  }
-  __shared__ double red[kTB];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+      for (int i = 0; i < 16; i++)
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    double v = acc[j];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) red[wv * 4 + j] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < nbl) {
-    const int j = threadIdx.x;
-    double v = 0.0;
-    for (int w = 0; w < kTB / 64; w++) v += red[w * 4 + j];
-    part[((size_t)blockIdx.x * na + a) * nb + (b0 + j)] = v;
+        for (int j = 0; j < 4; j++) {
+          double v = acc[i][j];
+          for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+          if (lane == 0 && i < nal && j < nbl) part[(g * na + a0 + i) * nb + b0 + j] = v;
+        }
+    }
   }
 }
 
@@ -514,15 +510,17 @@ uint64_t Davidson::device_bytes() const {
 
 void Davidson::ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, const double* B, int64_t ldb, int nb,
                           double* out, int ldo) {
-  // rows per block: about 1024 blocks in flight, at least 512 rows each
-  const int64_t blocks_per_col = std::max<int64_t>(1, 1024 / std::max(1, na * ((nb + 3) / 4)));
-  const int64_t chunk = std::max<int64_t>(512, (N_ + blocks_per_col - 1) / blocks_per_col);
-  const int nchunk = (int)((N_ + chunk - 1) / chunk);
-  PQG_REQUIRE((size_t)nchunk * na * nb <= part_.n, PQG_ERR_ARG, "tall-skinny partial buffer too small");
-  dim3 g(nchunk, na, (nb + 3) / 4);
-  k_tn_partial<<<g, kTB, 0, st>>>(A, lda, na, B, ldb, nb, N_, chunk, part_.get());
+  // one wave per row slice of at least 4096 rows, at most 3072 waves (the 3 per SIMD its
+  // registers allow on 256 CUs) and what the partial buffer holds
+  int64_t waves = std::min<int64_t>(3072, (N_ + 4095) / 4096);
+  waves = std::min<int64_t>(waves, (int64_t)(part_.n / std::max(1, na * nb)));
+  waves = std::max<int64_t>(1, waves / 4 * 4);
+  const int64_t rpw = (N_ + waves - 1) / waves;
+  const int nwave = (int)((N_ + rpw - 1) / rpw);
+  PQG_REQUIRE((size_t)nwave * na * nb <= part_.n, PQG_ERR_ARG, "tall-skinny partial buffer too small");
+  k_tn_waves<<<(nwave + 3) / 4, 256, 0, st>>>(A, lda, na, B, ldb, nb, N_, rpw, part_.get());
   PQG_HIP(hipGetLastError());
-  k_tn_reduce<<<na * nb, 256, 0, st>>>(part_.get(), nchunk, na, nb, out, ldo);
+  k_tn_reduce<<<na * nb, 256, 0, st>>>(part_.get(), nwave, na, nb, out, ldo);
   PQG_HIP(hipGetLastError());
 }
 
