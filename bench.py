@@ -173,9 +173,11 @@ def bench_ao2mo(args):
 
 
 def cpu_baseline_fci16(prob, sigma_vectors, threads):
-    """Oracle sigma of 4 vectors on determinants i = 0 mod s (two strides fit the fixed
-    per-call overhead) of the 16-orbital 8a8b space; the per-row cost, scaled to all rows, is one
-    full sigma; times the sigma vectors of the GPU FP64 solve / 4 gives one CPU solve."""
+    """Oracle sigma of 4 vectors on the determinants i = 0 mod 256 of the 16-orbital 8a8b space
+    (647,020 rows), less the fixed cost of a call (its per-thread private sigma of N x 4 values,
+    zeroed and then summed, as the reference's singleshot sigma does), measured by a call on one
+    row; the per-row cost, scaled to all rows, is one full sigma; times the sigma vectors of the
+    GPU FP64 solve / 4 gives one CPU solve."""
     import numpy as np
 
     from oracle import refcpu
@@ -185,15 +187,17 @@ def cpu_baseline_fci16(prob, sigma_vectors, threads):
     oc = refcpu.OracleCI(prob)
     t_set = time.perf_counter() - t0
     C = np.random.default_rng(0).standard_normal((oc.N, k))
-    s_big, s_small = 1 << 14, 1 << 12
+    stride = 256
     t0 = time.perf_counter()
-    oc.sigma_sample(C, s_big)
-    t_big = time.perf_counter() - t0
+    oc.sigma_sample(C, oc.N)  # row 0 only: the fixed cost
+    t_fixed = time.perf_counter() - t0
     t0 = time.perf_counter()
-    oc.sigma_sample(C, s_small)
-    t_small = time.perf_counter() - t0
-    rows_big, rows_small = -(-oc.N // s_big), -(-oc.N // s_small)
-    per_row = max((t_small - t_big) / (rows_small - rows_big), 1e-12)
+    oc.sigma_sample(C, stride)
+    t_sample = time.perf_counter() - t0
+    rows = -(-oc.N // stride)
+    if t_sample <= 1.5 * t_fixed:
+        raise RuntimeError(f"sample ({t_sample:.1f} s) not above the fixed cost ({t_fixed:.1f} s)")
+    per_row = (t_sample - t_fixed) / (rows - 1)
     t_sigma = per_row * oc.N
     t_solve = t_sigma * sigma_vectors / k
     conn = 5792
@@ -203,10 +207,11 @@ def cpu_baseline_fci16(prob, sigma_vectors, threads):
         "cores": threads,
         "kind": "port",
         "sample": (f"extrapolated: oracle symmetric-scatter sigma (k={k} vectors) on determinants i = 0 mod "
-                   f"{s_small} ({rows_small} of {oc.N}) and mod {s_big}: {per_row * 1e6:.1f} us per row on "
-                   f"{threads} threads = {conn * k / per_row / 1e9:.3f} G nonzero-vector products/s; "
-                   f"full sigma {t_sigma:.0f} s x {sigma_vectors / k:.1f} sigma applications of the GPU FP64 "
-                   f"solve ({sigma_vectors} vectors); oracle determinant set {t_set:.1f} s, not counted"),
+                   f"{stride} ({rows} of {oc.N}): {t_sample:.1f} s, less {t_fixed:.1f} s of fixed per-call cost "
+                   f"(one row) = {per_row * 1e6:.2f} us per row on {threads} threads = "
+                   f"{conn * k / per_row / 1e9:.3f} G nonzero-vector products/s; full sigma {t_sigma:.0f} s x "
+                   f"{sigma_vectors / k:.1f} sigma applications of the GPU FP64 solve ({sigma_vectors} vectors); "
+                   f"oracle determinant set {t_set:.1f} s, not counted"),
         "sigma_s": t_sigma,
         "per_nonzero_ns": per_row / (conn * k) * 1e9,
     }
@@ -292,7 +297,9 @@ def bench_fci16(args):
     C = torch.randn(N * k, dtype=torch.float64, device="cuda") / math.sqrt(N)
     S = torch.empty_like(C)
     kern = {}
-    for tag, prec, suf, peak in (("fp64", PQG_FP64, "_f64", 78.6), ("fp32", PQG_MIXED, "_f32", 157.3)):
+    # the mixed-precision sigma runs its link GEMM as BF16x3 (three bf16 products per FP32 product on
+    # the bf16 matrix cores: 2.5 PFLOP/s dense / 3 of FP32-equivalent work)
+    for tag, prec, suf, peak in (("fp64", PQG_FP64, "_f64", 78.6), ("bf16x3", PQG_MIXED, "_f32", 2500.0 / 3)):
         ds.sigma_device(C.data_ptr(), N, k, S.data_ptr(), N, prec)  # warm-up
         torch.cuda.synchronize()
         ctx.profile(True)

@@ -213,8 +213,10 @@ pqg_status pqg_ci_dump_excitations(pqg_ci* ci, int which, int part, int spin, in
 /* --- AO -> MO transform (replaces transform_mo_2_body_integrals, integral.cpp:207-477) -- */
 /* eri8: AO integrals with 8-fold symmetry, packed as idx2(idx2(p,q), idx2(r,s)) (idx8,
  * integral.hpp:76-88), length npair*(npair+1)/2 with npair = nao*(nao+1)/2.
- * C: nao x nmo column-major.  out4: npair_mo x npair_mo column-major, element
- * (idx2(i,j), idx2(k,l)).  on_device != 0: eri8, C and out4 are device pointers.
+ * C: nao x nmo column-major, 1 <= nmo <= nao (the reference's num_mo = C.cols() - frozen_core -
+ * deleted_virtual, integral.cpp:214-217: pass the active columns, C + frozen_core * nao).
+ * out4: npair_mo x npair_mo column-major, element (idx2(i,j), idx2(k,l)).
+ * on_device != 0: eri8, C and out4 are device pointers.
  * mode: PQG_AO2MO_FP64 or PQG_AO2MO_BF16X3.  seconds (may be NULL): GPU time.              */
 pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* C, double* out4,
                      int on_device, int mode, double* seconds);
@@ -225,6 +227,17 @@ pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const d
  * Cb, npair x npair column-major.  Arguments otherwise as pqg_ao2mo.                        */
 pqg_status pqg_ao2mo_pair(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* Ca, const double* Cb,
                           double* out4, int on_device, int mode, double* seconds);
+/* transform_mo_2_body_integrals for two species with their own bases (integral.cpp:207-477 with
+ * quantum_part_a_idx != quantum_part_b_idx: num_ao_a / num_ao_b, :214-217; the AO integrals
+ * (pq|rs) with p, q in basis a and r, s in basis b that libint computes at :259-279), as
+ * calculate_mo_2_body_integrals calls it for the electron-positron block (:479-516).
+ * eri_ab: npair(nao_a) x npair(nao_b) column-major, element (idx2(p,q), idx2(r,s)) (no charge
+ * factor: the CI applies the charge product).  Ca: nao_a x nmo_a, Cb: nao_b x nmo_b, column-major
+ * (1 <= nmo <= nao).  out4: npair(nmo_a) x npair(nmo_b) column-major, element (idx2(i,j), idx2(k,l)),
+ * the reference's mo_two_body_ints[a][sa][b][sb].  on_device, mode, seconds as pqg_ao2mo.        */
+pqg_status pqg_ao2mo_mixed(pqg_ctx* ctx, int nao_a, int nmo_a, int nao_b, int nmo_b, const double* eri_ab,
+                           const double* Ca, const double* Cb, double* out4, int on_device, int mode,
+                           double* seconds);
 /* compute_frozen_core_ints (integral.cpp:565-698) for one (species a, spin a) operator and one
  * (species b, spin b) core density, as called by calculate_frozen_core_ints
  * (integral.cpp:166-205): out (nao_a x nao_a, column-major, host) +=

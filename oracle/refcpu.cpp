@@ -1251,40 +1251,48 @@ int rc_num_excitation(const uint64_t* Di, const uint64_t* Dj, int W) { return si
 
 // AO->MO transform restated from transform_mo_2_body_integrals (integral.cpp:207-477):
 // four quarter transforms over a full AO tensor (the reference recomputes AO integrals with
-// libint in step 1; here they are an input).  ao: full n_ao^4 tensor (pq|rs) row-major over
-// p,q,r,s.  Output: npair_a x npair_b column-major, eri(idx2(i,j), idx2(k,l)).
-void rc_ao2mo(int nao, int nmo_a, int nmo_b, const double* ao, const double* Ca, const double* Cb, double* out) {
-  const int64_t n = nao;
-  std::vector<double> t1((size_t)nmo_a * n * n * n, 0.0);
+// libint in step 1; here they are an input).  ao: full tensor (pq|rs) row-major over p,q,r,s
+// with p, q in basis a (nao_a functions, coefficients Ca: nao_a x nmo_a column-major) and r, s in
+// basis b (nao_b, Cb: nao_b x nmo_b) -- one basis for a species with itself, two for the
+// electron-positron block (num_ao_a / num_ao_b, integral.cpp:214-217).  Output:
+// npair(nmo_a) x npair(nmo_b) column-major, eri(idx2(i,j), idx2(k,l)).
+void rc_ao2mo2(int nao_a, int nao_b, int nmo_a, int nmo_b, const double* ao, const double* Ca, const double* Cb,
+               double* out) {
+  const int64_t na = nao_a, nb = nao_b;
+  // step 1 (integral.cpp:253-320): t1(i, q, r, s) = sum_p Ca(p, i) (pq|rs)
+  std::vector<double> t1((size_t)nmo_a * na * nb * nb, 0.0);
 #pragma omp parallel for collapse(2)
   for (int i = 0; i < nmo_a; i++)
-    for (int64_t q = 0; q < n; q++)
-      for (int64_t r = 0; r < n; r++)
-        for (int64_t s = 0; s < n; s++) {
+    for (int64_t q = 0; q < na; q++)
+      for (int64_t r = 0; r < nb; r++)
+        for (int64_t s = 0; s < nb; s++) {
           double v = 0.0;
-          for (int64_t p = 0; p < n; p++) v += Ca[p + i * n] * ao[((p * n + q) * n + r) * n + s];
-          t1[((i * n + q) * n + r) * n + s] = v;
+          for (int64_t p = 0; p < na; p++) v += Ca[p + i * na] * ao[((p * na + q) * nb + r) * nb + s];
+          t1[((i * na + q) * nb + r) * nb + s] = v;
         }
-  std::vector<double> t2((size_t)nmo_a * nmo_a * n * n, 0.0);
+  // step 2 (:330-372): t2(i, j, r, s) = sum_q Ca(q, j) t1(i, q, r, s)
+  std::vector<double> t2((size_t)nmo_a * nmo_a * nb * nb, 0.0);
 #pragma omp parallel for collapse(2)
   for (int i = 0; i < nmo_a; i++)
     for (int j = 0; j < nmo_a; j++)
-      for (int64_t r = 0; r < n; r++)
-        for (int64_t s = 0; s < n; s++) {
+      for (int64_t r = 0; r < nb; r++)
+        for (int64_t s = 0; s < nb; s++) {
           double v = 0.0;
-          for (int64_t q = 0; q < n; q++) v += Ca[q + j * n] * t1[((i * n + q) * n + r) * n + s];
-          t2[(((int64_t)i * nmo_a + j) * n + r) * n + s] = v;
+          for (int64_t q = 0; q < na; q++) v += Ca[q + j * na] * t1[((i * na + q) * nb + r) * nb + s];
+          t2[(((int64_t)i * nmo_a + j) * nb + r) * nb + s] = v;
         }
-  std::vector<double> t3((size_t)nmo_a * nmo_a * nmo_b * n, 0.0);
+  // step 3 (:386-427): t3(i, j, k, s) = sum_r Cb(r, k) t2(i, j, r, s)
+  std::vector<double> t3((size_t)nmo_a * nmo_a * nmo_b * nb, 0.0);
 #pragma omp parallel for collapse(2)
   for (int i = 0; i < nmo_a; i++)
     for (int j = 0; j < nmo_a; j++)
       for (int k = 0; k < nmo_b; k++)
-        for (int64_t s = 0; s < n; s++) {
+        for (int64_t s = 0; s < nb; s++) {
           double v = 0.0;
-          for (int64_t r = 0; r < n; r++) v += Cb[r + k * n] * t2[(((int64_t)i * nmo_a + j) * n + r) * n + s];
-          t3[(((int64_t)i * nmo_a + j) * nmo_b + k) * n + s] = v;
+          for (int64_t r = 0; r < nb; r++) v += Cb[r + k * nb] * t2[(((int64_t)i * nmo_a + j) * nb + r) * nb + s];
+          t3[(((int64_t)i * nmo_a + j) * nmo_b + k) * nb + s] = v;
         }
+  // step 4 (:441-469): eri(idx2(i,j), idx2(k,l)) = sum_s Cb(s, l) t3(i, j, k, s), j >= i, l >= k
   const int64_t npa = (int64_t)nmo_a * (nmo_a + 1) / 2;
 #pragma omp parallel for collapse(2)
   for (int i = 0; i < nmo_a; i++)
@@ -1293,10 +1301,15 @@ void rc_ao2mo(int nao, int nmo_a, int nmo_b, const double* ao, const double* Ca,
       for (int k = 0; k < nmo_b; k++)
         for (int l = k; l < nmo_b; l++) {
           double v = 0.0;
-          for (int64_t s = 0; s < n; s++) v += Cb[s + l * n] * t3[(((int64_t)i * nmo_a + j) * nmo_b + k) * n + s];
+          for (int64_t s = 0; s < nb; s++) v += Cb[s + l * nb] * t3[(((int64_t)i * nmo_a + j) * nmo_b + k) * nb + s];
           out[idx2(i, j) + (int64_t)idx2(k, l) * npa] = v;
         }
     }
+}
+
+// one basis (nao_a == nao_b)
+void rc_ao2mo(int nao, int nmo_a, int nmo_b, const double* ao, const double* Ca, const double* Cb, double* out) {
+  rc_ao2mo2(nao, nao, nmo_a, nmo_b, ao, Ca, Cb, out);
 }
 
 }  // extern "C"

@@ -76,6 +76,8 @@ def lib():
                                       ct.c_void_p]
         L.rc_num_excitation.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int]
         L.rc_ao2mo.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        L.rc_ao2mo2.argtypes = [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                ct.c_void_p]
         _lib = L
     return _lib
 
@@ -344,14 +346,17 @@ def frozen_core_ints_block(out, eri, fc_dm_b, same_particle, a_spin, b_spin, res
 
 
 def ao2mo(ao_full, Ca, Cb=None):
-    """Quarter-transform restatement; ao_full is the (n,n,n,n) AO tensor."""
+    """Quarter-transform restatement; ao_full is the AO tensor (pq|rs) of shape
+    (nao_a, nao_a, nao_b, nao_b): one basis (Cb defaults to Ca), or two bases for the
+    electron-positron block, Ca (nao_a x nmo_a) transforming p, q and Cb (nao_b x nmo_b) r, s."""
     if Cb is None:
         Cb = Ca
-    n = ao_full.shape[0]
+    na, nb = ao_full.shape[0], ao_full.shape[2]
+    assert ao_full.shape == (na, na, nb, nb) and Ca.shape[0] == na and Cb.shape[0] == nb
     nma, nmb = Ca.shape[1], Cb.shape[1]
     ao = np.ascontiguousarray(ao_full, dtype=np.float64)
     Ca = np.asfortranarray(Ca, dtype=np.float64)
     Cb = np.asfortranarray(Cb, dtype=np.float64)
     out = np.zeros((nma * (nma + 1) // 2, nmb * (nmb + 1) // 2), order="F")
-    lib().rc_ao2mo(n, nma, nmb, _ptr(ao), _ptr(Ca), _ptr(Cb), _ptr(out))
+    lib().rc_ao2mo2(na, nb, nma, nmb, _ptr(ao), _ptr(Ca), _ptr(Cb), _ptr(out))
     return out

@@ -92,6 +92,8 @@ SIGNATURES = [
     ("pqg_ci_dense_s2", ct.c_int, [_vp, ct.c_int, ct.c_double, ct.c_int, _vp, ct.c_int64, ct.c_int]),
     ("pqg_ao2mo_pair", ct.c_int, [_vp, ct.c_int, ct.c_int, _vp, _vp, _vp, _vp, ct.c_int, ct.c_int,
                                   ct.POINTER(ct.c_double)]),
+    ("pqg_ao2mo_mixed", ct.c_int, [_vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _vp, _vp, _vp, _vp, ct.c_int,
+                                   ct.c_int, ct.POINTER(ct.c_double)]),
     ("pqg_fc_ints", ct.c_int, [_vp, ct.c_int, ct.c_int, _vp, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_double,
                                _vp, _vp]),
     ("pqg_ci_csr", ct.c_int, [_vp, ct.c_double, _vp, _vp, _vp, ct.c_int64, ct.POINTER(ct.c_int64)]),

@@ -1,21 +1,22 @@
 // AO -> MO four-index transform on MFMA (gfx950).
 //
-// (ij|kl) = sum_{mu nu la si} C[mu,i] C[nu,j] C[la,k] C[si,l] (mu nu|la si), the contraction
-// of transform_mo_2_body_integrals (src/integral/integral.cpp:207-477), organised as two
-// half-transforms over pair indices:
-//   half 1, for every AO pair Q = (la si):  M_Q = unpack((.. | Q)),  T_Q = C^T M_Q C,
-//            B[Q][(ij)] = T_Q[i][j] (i <= j);
-//   half 2, for every MO pair (ij):         M = unpack(B[.][(ij)]),  T = C^T M C,
-//            out[(kl) + (ij) * npair] = T[k][l].
-// Each C^T M C is two dense n x n x n products on the matrix cores:
-//   U = M C (M symmetric), then T = U^T C.
+// (ij|kl) = sum_{mu nu la si} Ca[mu,i] Ca[nu,j] Cb[la,k] Cb[si,l] (mu nu|la si), the contraction
+// of transform_mo_2_body_integrals (src/integral/integral.cpp:207-477), with mu nu, i j in the
+// basis and orbitals of species a and la si, k l in those of species b (the same basis for one
+// species; two bases for the electron-positron block), organised as two half-transforms over
+// pair indices:
+//   half 1, for every AO pair P = (mu nu) of basis a:  M_P = unpack((P | ..)),  T_P = Cb^T M_P Cb,
+//            B[P][(kl)] = T_P[k][l] (k <= l);
+//   half 2, for every MO pair (kl) of basis b:         M = unpack(B[.][(kl)]),  T = Ca^T M Ca,
+//            out[(ij) + (kl) * npair_a] = T[i][j]  (output column (kl), written contiguously).
+// Each C^T M C is two dense products on the matrix cores: U = M C (nao x nao x nmo, M
+// symmetric), then T = U^T C (nmo x nao x nmo).  The coefficient matrices may have fewer
+// columns than basis functions (frozen core and deleted virtuals removed by the caller).
 // Batches of pairs run as one launch (batched GEMM, 64 x 64 output tile per workgroup, K
 // staged through LDS).
 //   PQG_AO2MO_FP64:   v_mfma_f64_16x16x4f64, FP64 accumulation.
 //   PQG_AO2MO_BF16X3: each FP32 operand split into bf16 hi + lo, three products
 //                     hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_bf16, FP32 accumulation.
-// The final (ij|kl) block is symmetric for a single coefficient matrix, so the output row
-// (ij) is written as column (ij) of the column-major result (coalesced).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -47,9 +48,10 @@ __global__ void k_unpack8(const double* __restrict__ eri8, int n, int64_t q0, in
   X[idx] = (T)eri8[pidx(P, Q)];
 }
 
-// X[b][la][si] = B[idx2(la, si)][c0 + b]   (B row-major npair x ldb)
-template <typename T>
-__global__ void k_unpack_half(const T* __restrict__ B, int64_t ldb, int n, int64_t c0, int nb, T* __restrict__ X) {
+// X[b][la][si] = B[idx2(la, si)][c0 + b]   (B row-major npair x ldb; also row c0 + b of a
+// column-major matrix with leading dimension ldb whose columns are pairs (la si))
+template <typename S, typename T>
+__global__ void k_unpack_half(const S* __restrict__ B, int64_t ldb, int n, int64_t c0, int nb, T* __restrict__ X) {
   const int64_t n2 = (int64_t)n * n;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n2 * nb) return;
@@ -57,7 +59,7 @@ __global__ void k_unpack_half(const T* __restrict__ B, int64_t ldb, int n, int64
   const int b = (int)(idx % nb);
   const int64_t r = idx / nb;
   const int la = (int)(r / n), si = (int)(r % n);
-  X[b * n2 + r] = B[pidx(la, si) * ldb + c0 + b];
+  X[b * n2 + r] = (T)B[pidx(la, si) * ldb + c0 + b];
 }
 
 // out[b][pidx(i, j)] = T[b][i][j] for i <= j, with row stride ldo (batch b -> row b0 + b)
@@ -77,8 +79,8 @@ __global__ void k_pack(const T* __restrict__ Tm, int n, int nb, O* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------------------
-// batched GEMM: out_b (n x n) = op(A_b) * Cm,  op(A) = A (TA = 0) or A^T (TA = 1),
-// A_b row-major n x n, Cm row-major n x n (shared), out row-major.
+// batched GEMM: out_b (m x nn) = op(A_b) * Cm,  op(A) = A (TA = 0) or A^T (TA = 1),
+// A_b row-major m x kd (TA = 0) or kd x m (TA = 1), Cm row-major kd x nn (shared), out row-major.
 // ---------------------------------------------------------------------------------------
 constexpr int BM = 64, BN = 64, BK = 16;
 
@@ -87,12 +89,11 @@ constexpr int BM = 64, BN = 64, BK = 16;
 // D 16x16 -> lane l, register r holds D[4*r + l/16][l%16], r = 0..3 (rows interleaved).
 template <int TA>
 __global__ __launch_bounds__(256) void k_bgemm_f64(const double* __restrict__ A, const double* __restrict__ Cm,
-                                                   double* __restrict__ out, int n) {
+                                                   double* __restrict__ out, int m, int nn, int kd) {
   __shared__ double As[BK][BM + 2];  // As[k][m]
   __shared__ double Bs[BK][BN + 2];  // Bs[k][nn]
-  const int64_t n2 = (int64_t)n * n;
-  const double* Ab = A + (int64_t)blockIdx.z * n2;
-  double* Ob = out + (int64_t)blockIdx.z * n2;
+  const double* Ab = A + (int64_t)blockIdx.z * m * kd;
+  double* Ob = out + (int64_t)blockIdx.z * m * nn;
   const int m0 = blockIdx.y * BM, c0 = blockIdx.x * BN;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
@@ -101,18 +102,18 @@ __global__ __launch_bounds__(256) void k_bgemm_f64(const double* __restrict__ A,
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) acc[a][b] = d4{0, 0, 0, 0};
-  for (int k0 = 0; k0 < n; k0 += BK) {
+  for (int k0 = 0; k0 < kd; k0 += BK) {
     for (int i = tid; i < BK * BM; i += 256) {
       const int kk = i / BM, mm = i % BM;
       const int gm = m0 + mm, gk = k0 + kk;
       double v = 0.0;
-      if (gm < n && gk < n) v = TA ? Ab[(int64_t)gk * n + gm] : Ab[(int64_t)gm * n + gk];
+      if (gm < m && gk < kd) v = TA ? Ab[(int64_t)gk * m + gm] : Ab[(int64_t)gm * kd + gk];
       As[kk][mm] = v;
     }
     for (int i = tid; i < BK * BN; i += 256) {
-      const int kk = i / BN, nn = i % BN;
-      const int gn = c0 + nn, gk = k0 + kk;
-      Bs[kk][nn] = (gn < n && gk < n) ? Cm[(int64_t)gk * n + gn] : 0.0;
+      const int kk = i / BN, c = i % BN;
+      const int gn = c0 + c, gk = k0 + kk;
+      Bs[kk][c] = (gn < nn && gk < kd) ? Cm[(int64_t)gk * nn + gn] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(256) void k_bgemm_f64(const double* __restrict__ A,
       for (int r = 0; r < 4; r++) {
         const int gm = m0 + wm + a * 16 + 4 * r + (lane >> 4);
         const int gn = c0 + wn + b * 16 + (lane & 15);
-        if (gm < n && gn < n) Ob[(int64_t)gm * n + gn] = acc[a][b][r];
+        if (gm < m && gn < nn) Ob[(int64_t)gm * nn + gn] = acc[a][b][r];
       }
 }
 
@@ -153,29 +154,28 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 
 template <int TA>
 __global__ __launch_bounds__(256) void k_bgemm_bf16x3(const float* __restrict__ A, const float* __restrict__ Cm,
-                                                      float* __restrict__ out, int n) {
+                                                      float* __restrict__ out, int m, int nn, int kd) {
   __shared__ __bf16 Ah[BM][BK + 8], Al[BM][BK + 8];  // [m][k]
   __shared__ __bf16 Bh[BN][BK + 8], Bl[BN][BK + 8];  // [n][k]
-  const int64_t n2 = (int64_t)n * n;
-  const float* Ab = A + (int64_t)blockIdx.z * n2;
-  float* Ob = out + (int64_t)blockIdx.z * n2;
+  const float* Ab = A + (int64_t)blockIdx.z * m * kd;
+  float* Ob = out + (int64_t)blockIdx.z * m * nn;
   const int m0 = blockIdx.y * BM, c0 = blockIdx.x * BN;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
   f16v acc = {};
-  for (int k0 = 0; k0 < n; k0 += BK) {
+  for (int k0 = 0; k0 < kd; k0 += BK) {
     for (int i = tid; i < BK * BM; i += 256) {
       const int mm = i / BK, kk = i % BK;
       const int gm = m0 + mm, gk = k0 + kk;
       float v = 0.f;
-      if (gm < n && gk < n) v = TA ? Ab[(int64_t)gk * n + gm] : Ab[(int64_t)gm * n + gk];
+      if (gm < m && gk < kd) v = TA ? Ab[(int64_t)gk * m + gm] : Ab[(int64_t)gm * kd + gk];
       split_bf16(v, Ah[mm][kk], Al[mm][kk]);
     }
     for (int i = tid; i < BK * BN; i += 256) {
-      const int kk = i / BN, nn = i % BN;
-      const int gn = c0 + nn, gk = k0 + kk;
-      const float v = (gn < n && gk < n) ? Cm[(int64_t)gk * n + gn] : 0.f;
-      split_bf16(v, Bh[nn][kk], Bl[nn][kk]);
+      const int kk = i / BN, c = i % BN;
+      const int gn = c0 + c, gk = k0 + kk;
+      const float v = (gn < nn && gk < kd) ? Cm[(int64_t)gk * nn + gn] : 0.f;
+      split_bf16(v, Bh[c][kk], Bl[c][kk]);
     }
     __syncthreads();
     bf8 ah, al, bh, bl;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void k_bgemm_bf16x3(const float* __restrict__ 
   for (int r = 0; r < 16; r++) {
     const int gm = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     const int gn = c0 + wn + (lane & 31);
-    if (gm < n && gn < n) Ob[(int64_t)gm * n + gn] = acc[r];
+    if (gm < m && gn < nn) Ob[(int64_t)gm * nn + gn] = acc[r];
   }
 }
 
@@ -210,74 +210,93 @@ static int gr(int64_t n, int b) { return (int)((n + b - 1) / b); }
 static thread_local pqg_ctx* g_prof_ctx = nullptr;  // context of the running transform
 
 template <typename T>
-static void bgemm(const T* A, const T* Cm, T* out, int n, int nb, bool transA, hipStream_t st) {
-  dim3 g(gr(n, BN), gr(n, BM), nb);
+static void bgemm(const T* A, const T* Cm, T* out, int m, int nn, int kd, int nb, bool transA, hipStream_t st) {
+  dim3 g(gr(nn, BN), gr(m, BM), nb);
   ProfScope prof(g_prof_ctx, st, std::is_same<T, double>::value ? "ao2mo_gemm_f64" : "ao2mo_gemm_bf16x3",
-                 2ll * n * n * n * nb);  // units: flops of the batch
+                 2ll * m * nn * kd * nb);  // units: flops of the batch
   if constexpr (std::is_same<T, double>::value) {
     if (transA)
-      k_bgemm_f64<1><<<g, 256, 0, st>>>(A, Cm, out, n);
+      k_bgemm_f64<1><<<g, 256, 0, st>>>(A, Cm, out, m, nn, kd);
     else
-      k_bgemm_f64<0><<<g, 256, 0, st>>>(A, Cm, out, n);
+      k_bgemm_f64<0><<<g, 256, 0, st>>>(A, Cm, out, m, nn, kd);
   } else {
     if (transA)
-      k_bgemm_bf16x3<1><<<g, 256, 0, st>>>(A, Cm, out, n);
+      k_bgemm_bf16x3<1><<<g, 256, 0, st>>>(A, Cm, out, m, nn, kd);
     else
-      k_bgemm_bf16x3<0><<<g, 256, 0, st>>>(A, Cm, out, n);
+      k_bgemm_bf16x3<0><<<g, 256, 0, st>>>(A, Cm, out, m, nn, kd);
   }
   PQG_HIP(hipGetLastError());
 }
 
-// C^T M_b C for nb symmetric matrices M_b (row-major), through U = M C, T = U^T C
+// C^T M_b C for nb symmetric nao x nao matrices M_b (row-major) and C nao x nmo (row-major),
+// through U = M C (nao x nmo), T = U^T C (nmo x nmo)
 template <typename T>
-static void ctmc(const T* M, const T* Cm, T* U, T* Tout, int n, int nb, hipStream_t st) {
-  bgemm<T>(M, Cm, U, n, nb, false, st);
-  bgemm<T>(U, Cm, Tout, n, nb, true, st);
+static void ctmc(const T* M, const T* Cm, T* U, T* Tout, int nao, int nmo, int nb, hipStream_t st) {
+  bgemm<T>(M, Cm, U, nao, nmo, nao, nb, false, st);
+  bgemm<T>(U, Cm, Tout, nmo, nmo, nao, nb, true, st);
 }
 
-// out4(idx2(i,j), idx2(k,l)) = (ij|kl) with i, j over the columns of Cd_row and k, l over those of
-// Cd_col (the same matrix for one spin block): half 1 transforms the column pair with Cd_col, half 2
-// the row pair with Cd_row.
+// The AO integrals a transform reads: the 8-fold packed tensor of one basis (mixed == 0), or
+// the npair(nao_a) x npair(nao_b) column-major block (mu nu|la si) of two bases, mu nu in basis
+// a and la si in basis b (mixed == 1).
+struct AoSource {
+  const double* eri;
+  int mixed;
+};
+
+// out4(idx2(i,j), idx2(k,l)) = (ij|kl), npair(nmo_a) x npair(nmo_b) column-major, with i, j over
+// the columns of Ca (basis a) and k, l over those of Cb (basis b).  Half 1 transforms the pair of
+// basis b of every AO pair P of basis a (one AO row of the source), B[P][(kl)]; half 2 transforms
+// the pair of basis a of every column (kl) of B and writes output column (kl) contiguously.
 template <typename T>
-static void ao2mo_impl(pqg_ctx* ctx, int n, const double* eri8, const double* Cd_row, const double* Cd_col,
-                       double* out4, hipStream_t st) {
-  const int64_t np = (int64_t)n * (n + 1) / 2;
-  const int64_t n2 = (int64_t)n * n;
-  // batch size: keep the three n x n x nb work arrays within ~1/8 of the budget
+static void ao2mo_impl(pqg_ctx* ctx, AoSource src, int nao_a, int nmo_a, int nao_b, int nmo_b, const double* Ca,
+                       const double* Cb, double* out4, hipStream_t st) {
+  const int64_t pa_ao = (int64_t)nao_a * (nao_a + 1) / 2, pa_mo = (int64_t)nmo_a * (nmo_a + 1) / 2;
+  const int64_t pb_mo = (int64_t)nmo_b * (nmo_b + 1) / 2;
+  const int64_t n2 = (int64_t)std::max(nao_a, nao_b) * std::max(nao_a, nao_b);
+  // batch size: keep the three work arrays (at most n2 elements per batch entry each) within
+  // about 1/4 of what the budget leaves after the half-transformed B
   const uint64_t avail = ctx->arena->budget() > ctx->arena->used() ? ctx->arena->budget() - ctx->arena->used() : 0;
-  const uint64_t interm = (uint64_t)np * np * sizeof(T);  // half-transformed B
+  const uint64_t interm = (uint64_t)pa_ao * pb_mo * sizeof(T);  // half-transformed B
   PQG_REQUIRE(avail > interm + (64ull << 20), PQG_ERR_OOM, "AO->MO intermediate does not fit the HBM budget");
-  int nb = (int)std::min<int64_t>(np, std::max<int64_t>(1, (int64_t)((avail - interm) / 4 / (3 * n2 * sizeof(T)))));
+  const int64_t most = std::max(pa_ao, pb_mo);
+  int nb = (int)std::min<int64_t>(most, std::max<int64_t>(1, (int64_t)((avail - interm) / 4 / (3 * n2 * sizeof(T)))));
   nb = std::min(nb, 4096);
-  DevBuf<T> Cr(ctx, n2), Cc(ctx, n2), X(ctx, n2 * nb), U(ctx, n2 * nb), Tm(ctx, n2 * nb), B(ctx, np * np);
+  DevBuf<T> Car(ctx, (size_t)nao_a * nmo_a), Cbr(ctx, (size_t)nao_b * nmo_b), X(ctx, n2 * nb), U(ctx, n2 * nb),
+      Tm(ctx, n2 * nb), B(ctx, pa_ao * pb_mo);
   // C in row-major [mu][i] (input is column-major nao x nmo)
   for (int w = 0; w < 2; w++) {
-    std::vector<double> h(n2);
-    PQG_HIP(hipMemcpyAsync(h.data(), w ? Cd_col : Cd_row, n2 * 8, hipMemcpyDeviceToHost, st));
+    const int nao = w ? nao_b : nao_a, nmo = w ? nmo_b : nmo_a;
+    std::vector<double> h((size_t)nao * nmo);
+    PQG_HIP(hipMemcpyAsync(h.data(), w ? Cb : Ca, h.size() * 8, hipMemcpyDeviceToHost, st));
     PQG_HIP(hipStreamSynchronize(st));
-    std::vector<T> r(n2);
-    for (int mu = 0; mu < n; mu++)
-      for (int i = 0; i < n; i++) r[(size_t)mu * n + i] = (T)h[mu + (size_t)i * n];
-    PQG_HIP(hipMemcpyAsync((w ? Cc : Cr).get(), r.data(), n2 * sizeof(T), hipMemcpyHostToDevice, st));
+    std::vector<T> r(h.size());
+    for (int mu = 0; mu < nao; mu++)
+      for (int i = 0; i < nmo; i++) r[(size_t)mu * nmo + i] = (T)h[mu + (size_t)i * nao];
+    PQG_HIP(hipMemcpyAsync((w ? Cbr : Car).get(), r.data(), r.size() * sizeof(T), hipMemcpyHostToDevice, st));
     PQG_HIP(hipStreamSynchronize(st));
   }
-  // half 1: rows Q of B
-  for (int64_t q0 = 0; q0 < np; q0 += nb) {
-    const int b = (int)std::min<int64_t>(nb, np - q0);
-    k_unpack8<T><<<gr(n2 * b, 256), 256, 0, st>>>(eri8, n, q0, b, X.get());
+  const int64_t nb2 = (int64_t)nao_b * nao_b, na2 = (int64_t)nao_a * nao_a;
+  // half 1: row P of B from AO row P of the source
+  for (int64_t p0 = 0; p0 < pa_ao; p0 += nb) {
+    const int b = (int)std::min<int64_t>(nb, pa_ao - p0);
+    if (src.mixed)
+      k_unpack_half<double, T><<<gr(nb2 * b, 256), 256, 0, st>>>(src.eri, pa_ao, nao_b, p0, b, X.get());
+    else
+      k_unpack8<T><<<gr(nb2 * b, 256), 256, 0, st>>>(src.eri, nao_b, p0, b, X.get());
     PQG_HIP(hipGetLastError());
-    ctmc<T>(X.get(), Cc.get(), U.get(), Tm.get(), n, b, st);
-    k_pack<T, T><<<gr(np * b, 256), 256, 0, st>>>(Tm.get(), n, b, B.get() + q0 * np, np);
+    ctmc<T>(X.get(), Cbr.get(), U.get(), Tm.get(), nao_b, nmo_b, b, st);
+    k_pack<T, T><<<gr(pb_mo * b, 256), 256, 0, st>>>(Tm.get(), nmo_b, b, B.get() + p0 * pb_mo, pb_mo);
     PQG_HIP(hipGetLastError());
   }
-  // half 2: column (kl) of B holds (rho sigma|kl); its transform over the row pair is output
+  // half 2: column (kl) of B holds (mu nu|kl); its transform over the pair of basis a is output
   // column (kl)
-  for (int64_t c0 = 0; c0 < np; c0 += nb) {
-    const int b = (int)std::min<int64_t>(nb, np - c0);
-    k_unpack_half<T><<<gr(n2 * b, 256), 256, 0, st>>>(B.get(), np, n, c0, b, X.get());
+  for (int64_t c0 = 0; c0 < pb_mo; c0 += nb) {
+    const int b = (int)std::min<int64_t>(nb, pb_mo - c0);
+    k_unpack_half<T, T><<<gr(na2 * b, 256), 256, 0, st>>>(B.get(), pb_mo, nao_a, c0, b, X.get());
     PQG_HIP(hipGetLastError());
-    ctmc<T>(X.get(), Cr.get(), U.get(), Tm.get(), n, b, st);
-    k_pack<T, double><<<gr(np * b, 256), 256, 0, st>>>(Tm.get(), n, b, out4 + c0 * np, np);
+    ctmc<T>(X.get(), Car.get(), U.get(), Tm.get(), nao_a, nmo_a, b, st);
+    k_pack<T, double><<<gr(pa_mo * b, 256), 256, 0, st>>>(Tm.get(), nmo_a, b, out4 + c0 * pa_mo, pa_mo);
     PQG_HIP(hipGetLastError());
   }
 }
@@ -360,28 +379,35 @@ extern "C" pqg_status pqg_fc_ints(pqg_ctx* ctx, int nao_a, int nao_b, const doub
   });
 }
 
-static void ao2mo_call(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* Ca, const double* Cb,
-                       double* out4, int on_device, int mode, double* seconds) {
-  PQG_REQUIRE(ctx && eri8 && Ca && Cb && out4, PQG_ERR_ARG, "NULL argument");
-  PQG_REQUIRE(nao >= 1 && nmo == nao, PQG_ERR_UNSUPPORTED, "pqg_ao2mo needs nmo == nao");
+// nao_a, nmo_a / nao_b, nmo_b: the bases and MO counts of the row pair (Ca) and the column pair
+// (Cb); mixed == 0: eri is the 8-fold packed tensor of one basis (nao_a == nao_b), mixed == 1: the
+// npair(nao_a) x npair(nao_b) column-major block of two bases.
+static void ao2mo_call(pqg_ctx* ctx, int nao_a, int nmo_a, int nao_b, int nmo_b, int mixed, const double* eri,
+                       const double* Ca, const double* Cb, double* out4, int on_device, int mode, double* seconds) {
+  PQG_REQUIRE(ctx && eri && Ca && Cb && out4, PQG_ERR_ARG, "NULL argument");
+  PQG_REQUIRE(nao_a >= 1 && nao_b >= 1, PQG_ERR_ARG, "empty basis");
+  PQG_REQUIRE(nmo_a >= 1 && nmo_a <= nao_a && nmo_b >= 1 && nmo_b <= nao_b, PQG_ERR_ARG,
+              "need 1 <= nmo <= nao (the coefficient matrix is nao x nmo)");
+  PQG_REQUIRE(mixed || nao_a == nao_b, PQG_ERR_ARG, "the 8-fold packed tensor holds one basis");
   PQG_REQUIRE(mode == PQG_AO2MO_FP64 || mode == PQG_AO2MO_BF16X3, PQG_ERR_ARG, "unknown mode");
   PQG_HIP(hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
-  const int n = nao;
-  const int64_t np = (int64_t)n * (n + 1) / 2;
-  const int64_t n8 = np * (np + 1) / 2;
+  const int64_t pa = (int64_t)nao_a * (nao_a + 1) / 2, pb = (int64_t)nao_b * (nao_b + 1) / 2;
+  const int64_t neri = mixed ? pa * pb : pa * (pa + 1) / 2;
+  const int64_t nout = (int64_t)nmo_a * (nmo_a + 1) / 2 * ((int64_t)nmo_b * (nmo_b + 1) / 2);
+  const size_t nca = (size_t)nao_a * nmo_a, ncb = (size_t)nao_b * nmo_b;
   DevBuf<double> e8, ca, cb, o4;
-  const double *de8 = eri8, *dA = Ca, *dB = Cb;
+  const double *deri = eri, *dA = Ca, *dB = Cb;
   double* dout = out4;
   if (!on_device) {
-    e8.reset(ctx, n8);
-    ca.reset(ctx, (size_t)n * n);
-    cb.reset(ctx, (size_t)n * n);
-    o4.reset(ctx, (size_t)np * np);
-    PQG_HIP(hipMemcpyAsync(e8.get(), eri8, n8 * 8, hipMemcpyHostToDevice, st));
-    PQG_HIP(hipMemcpyAsync(ca.get(), Ca, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
-    PQG_HIP(hipMemcpyAsync(cb.get(), Cb, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
-    de8 = e8.get();
+    e8.reset(ctx, neri);
+    ca.reset(ctx, nca);
+    cb.reset(ctx, ncb);
+    o4.reset(ctx, nout);
+    PQG_HIP(hipMemcpyAsync(e8.get(), eri, neri * 8, hipMemcpyHostToDevice, st));
+    PQG_HIP(hipMemcpyAsync(ca.get(), Ca, nca * 8, hipMemcpyHostToDevice, st));
+    PQG_HIP(hipMemcpyAsync(cb.get(), Cb, ncb * 8, hipMemcpyHostToDevice, st));
+    deri = e8.get();
     dA = ca.get();
     dB = cb.get();
     dout = o4.get();
@@ -391,10 +417,11 @@ static void ao2mo_call(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const
   PQG_HIP(hipEventCreate(&t1));
   PQG_HIP(hipEventRecord(t0, st));
   g_prof_ctx = ctx;
+  const AoSource src{deri, mixed};
   if (mode == PQG_AO2MO_FP64)
-    ao2mo_impl<double>(ctx, n, de8, dA, dB, dout, st);
+    ao2mo_impl<double>(ctx, src, nao_a, nmo_a, nao_b, nmo_b, dA, dB, dout, st);
   else
-    ao2mo_impl<float>(ctx, n, de8, dA, dB, dout, st);
+    ao2mo_impl<float>(ctx, src, nao_a, nmo_a, nao_b, nmo_b, dA, dB, dout, st);
   PQG_HIP(hipEventRecord(t1, st));
   PQG_HIP(hipEventSynchronize(t1));
   float ms = 0.f;
@@ -403,17 +430,25 @@ static void ao2mo_call(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const
   (void)hipEventDestroy(t1);
   if (seconds) *seconds = ms * 1e-3;
   if (!on_device) {
-    PQG_HIP(hipMemcpyAsync(out4, dout, (size_t)np * np * 8, hipMemcpyDeviceToHost, st));
+    PQG_HIP(hipMemcpyAsync(out4, dout, nout * 8, hipMemcpyDeviceToHost, st));
     PQG_HIP(hipStreamSynchronize(st));
   }
 }
 
 extern "C" pqg_status pqg_ao2mo(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* C, double* out4,
                                 int on_device, int mode, double* seconds) {
-  return guard([&] { ao2mo_call(ctx, nao, nmo, eri8, C, C, out4, on_device, mode, seconds); });
+  return guard([&] { ao2mo_call(ctx, nao, nmo, nao, nmo, 0, eri8, C, C, out4, on_device, mode, seconds); });
 }
 
 extern "C" pqg_status pqg_ao2mo_pair(pqg_ctx* ctx, int nao, int nmo, const double* eri8, const double* Ca,
                                      const double* Cb, double* out4, int on_device, int mode, double* seconds) {
-  return guard([&] { ao2mo_call(ctx, nao, nmo, eri8, Ca, Cb, out4, on_device, mode, seconds); });
+  return guard([&] { ao2mo_call(ctx, nao, nmo, nao, nmo, 0, eri8, Ca, Cb, out4, on_device, mode, seconds); });
+}
+
+extern "C" pqg_status pqg_ao2mo_mixed(pqg_ctx* ctx, int nao_a, int nmo_a, int nao_b, int nmo_b, const double* eri_ab,
+                                      const double* Ca, const double* Cb, double* out4, int on_device, int mode,
+                                      double* seconds) {
+  return guard([&] {
+    ao2mo_call(ctx, nao_a, nmo_a, nao_b, nmo_b, 1, eri_ab, Ca, Cb, out4, on_device, mode, seconds);
+  });
 }

@@ -32,17 +32,18 @@ def pack8(ao_full: np.ndarray) -> np.ndarray:
 
 def transform_mo_2_body_integrals(eri8: np.ndarray, C: np.ndarray, mode: int = FP64,
                                   ctx: Optional[Context] = None):
-    """(ij|kl) on the GPU from 8-fold packed AO integrals and C (nao x nmo, nmo == nao).
-    Returns (out4, gpu_seconds)."""
+    """(ij|kl) on the GPU from 8-fold packed AO integrals and C (nao x nmo, nmo <= nao: the
+    active columns when orbitals are frozen or deleted).  Returns (out4, gpu_seconds), out4 the
+    npair(nmo) x npair(nmo) matrix."""
     ctx = ctx or default_context()
-    n = C.shape[0]
-    npair = n * (n + 1) // 2
+    nao, nmo = C.shape
+    npa = nao * (nao + 1) // 2
     eri8 = np.ascontiguousarray(eri8, dtype=np.float64)
-    assert eri8.size == npair * (npair + 1) // 2
+    assert eri8.size == npa * (npa + 1) // 2
     Cf = np.asfortranarray(C, dtype=np.float64)
-    out = np.zeros((npair, npair), order="F")
+    out = np.zeros((nmo * (nmo + 1) // 2,) * 2, order="F")
     sec = ct.c_double()
-    check(load().pqg_ao2mo(ctx.h, n, C.shape[1], eri8.ctypes.data_as(ct.c_void_p), Cf.ctypes.data_as(ct.c_void_p),
+    check(load().pqg_ao2mo(ctx.h, nao, nmo, eri8.ctypes.data_as(ct.c_void_p), Cf.ctypes.data_as(ct.c_void_p),
                            out.ctypes.data_as(ct.c_void_p), 0, mode, ct.byref(sec)))
     return out, sec.value
 
@@ -52,17 +53,39 @@ def transform_mo_2_body_integrals_pair(eri8: np.ndarray, Ca: np.ndarray, Cb: np.
     """transform_mo_2_body_integrals with mo_coeffs_a != mo_coeffs_b (integral.cpp:207-477) in
     one basis: (ij|kl) with the pair ij over Ca (rows) and kl over Cb (columns)."""
     ctx = ctx or default_context()
-    n = Ca.shape[0]
-    npair = n * (n + 1) // 2
+    nao, nmo = Ca.shape
+    npa = nao * (nao + 1) // 2
     eri8 = np.ascontiguousarray(eri8, dtype=np.float64)
-    assert eri8.size == npair * (npair + 1) // 2 and Cb.shape == Ca.shape
+    assert eri8.size == npa * (npa + 1) // 2 and Cb.shape == Ca.shape
     Af = np.asfortranarray(Ca, dtype=np.float64)
     Bf = np.asfortranarray(Cb, dtype=np.float64)
-    out = np.zeros((npair, npair), order="F")
+    out = np.zeros((nmo * (nmo + 1) // 2,) * 2, order="F")
     sec = ct.c_double()
-    check(load().pqg_ao2mo_pair(ctx.h, n, Ca.shape[1], eri8.ctypes.data_as(ct.c_void_p), Af.ctypes.data_as(ct.c_void_p),
+    check(load().pqg_ao2mo_pair(ctx.h, nao, nmo, eri8.ctypes.data_as(ct.c_void_p), Af.ctypes.data_as(ct.c_void_p),
                                 Bf.ctypes.data_as(ct.c_void_p), out.ctypes.data_as(ct.c_void_p), 0, mode,
                                 ct.byref(sec)))
+    return out, sec.value
+
+
+def transform_mo_2_body_integrals_mixed(eri_ab: np.ndarray, Ca: np.ndarray, Cb: np.ndarray, mode: int = FP64,
+                                        ctx: Optional[Context] = None):
+    """transform_mo_2_body_integrals for two species with their own bases (integral.cpp:207-477,
+    num_ao_a != num_ao_b): eri_ab is the npair(nao_a) x npair(nao_b) AO block (pq|rs), p, q in
+    basis a and r, s in basis b; Ca (nao_a x nmo_a) transforms the row pair, Cb (nao_b x nmo_b)
+    the column pair.  Returns (out4, gpu_seconds), out4 npair(nmo_a) x npair(nmo_b) --
+    mo_two_body_ints[a][sa][b][sb] of calculate_mo_2_body_integrals (:479-516)."""
+    ctx = ctx or default_context()
+    nao_a, nmo_a = Ca.shape
+    nao_b, nmo_b = Cb.shape
+    eri_ab = np.asfortranarray(eri_ab, dtype=np.float64)
+    assert eri_ab.shape == (nao_a * (nao_a + 1) // 2, nao_b * (nao_b + 1) // 2)
+    Af = np.asfortranarray(Ca, dtype=np.float64)
+    Bf = np.asfortranarray(Cb, dtype=np.float64)
+    out = np.zeros((nmo_a * (nmo_a + 1) // 2, nmo_b * (nmo_b + 1) // 2), order="F")
+    sec = ct.c_double()
+    check(load().pqg_ao2mo_mixed(ctx.h, nao_a, nmo_a, nao_b, nmo_b, eri_ab.ctypes.data_as(ct.c_void_p),
+                                 Af.ctypes.data_as(ct.c_void_p), Bf.ctypes.data_as(ct.c_void_p),
+                                 out.ctypes.data_as(ct.c_void_p), 0, mode, ct.byref(sec)))
     return out, sec.value
 
 
@@ -216,41 +239,62 @@ def active_two_body(out4: np.ndarray, nmo: int, first: int, nact: int) -> np.nda
 
 
 def frozen_core_problem(species, C, kinetic, nuclear, eri_ao, e_nuc: float, frozen_core, deleted_virtual,
-                        nroots: int = 1, ctx: Optional[Context] = None):
-    """calculate_fc_energy, calculate_mo_1_body_integrals and the MO two-body transform with
-    frozen core and deleted virtuals (epci.cpp:47-129, 575-594; integral.cpp:166-205, 207-477,
-    565-610) for same-species AO integrals: returns the active-space CIProblem whose e_shift is
-    e_nuc plus the frozen-core energies, and those energies.  species: objects with .charge,
-    .restricted, .n_alpha, .n_beta (all particles, core included) and .ex_level; C[a][s]: the
-    MO coefficients (one block, or alpha and beta when unrestricted); eri_ao[(0, 0)]: 8-fold
-    packed AO integrals."""
-    from .workloads import CIProblem, Species
-
-    if len(species) != 1:
-        raise ValueError("frozen_core_problem takes one species (two-species AO blocks: fc_ints_block)")
+                        nroots: int = 1, max_collective: Optional[int] = None, ctx: Optional[Context] = None):
+    """calculate_fc_energy, calculate_mo_1_body_integrals and calculate_mo_2_body_integrals with
+    frozen core and deleted virtuals (epci.cpp:27-129, 575-594; integral.cpp:166-205, 207-516,
+    565-610) for one species or two (electron, positron): returns the active-space CIProblem whose
+    e_shift is e_nuc plus the frozen-core energies, and those energies.  species: objects with
+    .charge, .restricted, .n_alpha, .n_beta (all particles, core included) and .ex_level;
+    C[a][s]: the MO coefficients of species a (one block, or alpha and beta when unrestricted);
+    eri_ao[(a, a)]: the 8-fold packed AO integrals of species a, eri_ao[(a, b)] (a < b): the
+    npair_a x npair_b AO block (pq|rs) between the two bases.  Every two-body block is
+    transformed over the active orbitals only (the columns frozen_core[a] ..
+    nmo - deleted_virtual[a] - 1), as the reference's transform does (integral.cpp:214-217,
+    :298-300)."""
     from types import SimpleNamespace
 
-    sp = species[0]
-    # the caller's object is left as it is; calculate_fc_energy reads num_parts
-    spc = SimpleNamespace(charge=sp.charge, restricted=sp.restricted, num_parts=sp.n_alpha + sp.n_beta)
-    fc_dm, fci, e_fc = calculate_fc_energy([spc], C, frozen_core, eri_ao, kinetic, nuclear, ctx)
+    from .workloads import CIProblem, Species
+
+    ns = len(species)
+    if not 1 <= ns <= 2:
+        raise ValueError("frozen_core_problem takes one or two species")
+    # the caller's objects are left as they are; calculate_fc_energy reads num_parts
+    spc = [SimpleNamespace(charge=sp.charge, restricted=sp.restricted, num_parts=sp.n_alpha + sp.n_beta)
+           for sp in species]
+    fc_dm, fci, e_fc = calculate_fc_energy(spc, C, frozen_core, eri_ao, kinetic, nuclear, ctx)
     h = calculate_mo_1_body_integrals(C, frozen_core, deleted_virtual, fci)
-    nmo = np.asarray(C[0][0]).shape[1]
-    nact = nmo - frozen_core[0] - deleted_virtual[0]
-    nspin = len(C[0])
+    act, nact = [], []
+    for a in range(ns):
+        nmo = np.asarray(C[a][0]).shape[1]
+        na = nmo - frozen_core[a] - deleted_virtual[a]
+        if na < 1:
+            raise ValueError(f"species {a}: no active orbitals")
+        act.append([np.asfortranarray(np.asarray(Cs, dtype=np.float64)[:, frozen_core[a]:frozen_core[a] + na])
+                    for Cs in C[a]])
+        nact.append(na)
+    # calculate_mo_2_body_integrals (integral.cpp:479-516): every (a, sa, b, sb) with b >= a
     eri = {}
-    for s1 in range(nspin):
-        for s2 in range(s1, nspin):
-            Ca, Cb = np.asarray(C[0][s1]), np.asarray(C[0][s2])
-            if s1 == s2:
-                out4, _ = transform_mo_2_body_integrals(eri_ao[(0, 0)], Ca, ctx=ctx)
-            else:
-                out4, _ = transform_mo_2_body_integrals_pair(eri_ao[(0, 0)], Ca, Cb, ctx=ctx)
-            eri[(0, s1, 0, s2)] = active_two_body(out4, nmo, frozen_core[0], nact)
-    if nspin == 2:
-        eri[(0, 1, 0, 0)] = np.ascontiguousarray(eri[(0, 0, 0, 1)].T)
-    act = Species(nact, sp.n_alpha - frozen_core[0], sp.n_beta - frozen_core[0], sp.charge, nspin,
-                  getattr(sp, "ex_level", None))
-    prob = CIProblem(species=[act], h=[h[0]], eri=eri, nroots=nroots, e_shift=e_nuc + sum(e_fc),
-                     name=f"frozen_core_{frozen_core[0]}_deleted_{deleted_virtual[0]}")
+    for a in range(ns):
+        for s1 in range(len(C[a])):
+            for b in range(a, ns):
+                for s2 in range(len(C[b])):
+                    if a == b and s2 < s1:
+                        continue
+                    Ca, Cb = act[a][s1], act[b][s2]
+                    if a != b:
+                        out4, _ = transform_mo_2_body_integrals_mixed(eri_ao[(a, b)], Ca, Cb, ctx=ctx)
+                    elif s1 == s2:
+                        out4, _ = transform_mo_2_body_integrals(eri_ao[(a, a)], Ca, ctx=ctx)
+                    else:
+                        out4, _ = transform_mo_2_body_integrals_pair(eri_ao[(a, a)], Ca, Cb, ctx=ctx)
+                    eri[(a, s1, b, s2)] = out4
+        if len(C[a]) == 2:
+            eri[(a, 1, a, 0)] = np.ascontiguousarray(eri[(a, 0, a, 1)].T)
+    acts = [Species(nact[a], sp.n_alpha - frozen_core[a], sp.n_beta - frozen_core[a], sp.charge, len(C[a]),
+                    getattr(sp, "ex_level", None)) for a, sp in enumerate(species)]
+    prob = CIProblem(species=acts, h=[h[a] for a in range(ns)], eri=eri, nroots=nroots,
+                     e_shift=e_nuc + sum(e_fc),
+                     name="frozen_core_" + "_".join(f"{frozen_core[a]}_{deleted_virtual[a]}" for a in range(ns)))
+    if max_collective is not None:
+        prob.max_collective = int(max_collective)
     return prob, e_fc

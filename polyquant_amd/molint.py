@@ -50,7 +50,21 @@ BASIS: Dict[str, Dict[str, List[Tuple[str, List[Tuple[float, ...]]]]]] = {
     },
 }
 
-ZNUC = {"H": 1, "O": 8}
+# Custom basis sets of the reference's electron + positron test systems, same layout:
+#   "psh_electron" / "psh_positron": tests/data/PsH_wpos/electron_basis.g94, positron_basis.g94
+#   (the H atom of the PsH_wpos inputs);
+#   "li_s": the s shells of tests/data/li-_custombasis_wpos/Li_basis.g94 (its p shells left out).
+BASIS["psh_electron"] = {"H": [("S", [(17.37, 0.032828), (2.6173, 0.20121), (0.58994, 0.81724)]),
+                               ("S", [(0.169029, 1.0)]), ("S", [(0.06, 1.0)]), ("S", [(0.02, 1.0)]),
+                               ("S", [(0.009, 1.0)])]}
+BASIS["psh_positron"] = {"H": [("S", [(0.02, 1.0)]), ("S", [(0.009, 1.0)])]}
+BASIS["li_s"] = {"Li": [("S", [(642.4189, 0.00214), (96.79849, 0.01621), (22.09109, 0.07732), (6.20107, 0.24579),
+                               (1.93512, 0.47019)]),
+                        ("S", [(2.19146, 0.03509), (0.59613, 0.19123)]), ("S", [(0.07455, 1.0)]),
+                        ("S", [(0.02867, 1.0)]), ("S", [(0.0092, 1.0)]), ("S", [(0.0031, 1.0)]),
+                        ("S", [(0.0008, 1.0)])]}
+
+ZNUC = {"H": 1, "Li": 3, "O": 8}
 
 
 @dataclasses.dataclass
@@ -239,6 +253,49 @@ def ao_integrals(basis: List[BasisFunction], atoms: List[Tuple[float, np.ndarray
                     eri[x, y, z, w] = v
                     eri[z, w, x, y] = v
     return S, T, V, eri
+
+
+def one_body_integrals(basis: List[BasisFunction], atoms: List[Tuple[float, np.ndarray]]):
+    """(S, T, V) in the AO basis; V is the attraction of a charge -1 to the nuclei (negative)."""
+    n = len(basis)
+    S = np.zeros((n, n))
+    T = np.zeros((n, n))
+    V = np.zeros((n, n))
+    for i in range(n):
+        for j in range(i + 1):
+            S[i, j] = S[j, i] = _overlap_bf(basis[i], basis[j])
+            T[i, j] = T[j, i] = _kinetic_bf(basis[i], basis[j])
+            V[i, j] = V[j, i] = _nuclear_bf(basis[i], basis[j], atoms)
+    return S, T, V
+
+
+def eri_two_bases(basis_a: List[BasisFunction], basis_b: List[BasisFunction]) -> np.ndarray:
+    """(pq|rs) with p, q in basis a and r, s in basis b: the (na, na, nb, nb) Coulomb tensor between
+    two particle species (no charge factor)."""
+    na, nb = len(basis_a), len(basis_b)
+    out = np.zeros((na, na, nb, nb))
+    for i in range(na):
+        for j in range(i + 1):
+            for k in range(nb):
+                for l in range(k + 1):
+                    v = _eri_bf(basis_a[i], basis_a[j], basis_b[k], basis_b[l])
+                    out[i, j, k, l] = out[j, i, k, l] = out[i, j, l, k] = out[j, i, l, k] = v
+    return out
+
+
+def electron_positron_atom(symbol: str, electron_basis: str, positron_basis: str):
+    """AO integrals of one atom at the origin with an electron and a positron basis (the reference's
+    PsH_wpos and Li_wpos test systems): per species S, T, V (V: attraction of a charge -1), the
+    electron-electron and positron-positron tensors and the electron-positron tensor
+    (ne, ne, np, np)."""
+    R = np.zeros((1, 3))
+    atoms = [(float(ZNUC[symbol]), R[0])]
+    be = build_basis([symbol], R, electron_basis)
+    bp = build_basis([symbol], R, positron_basis)
+    Se, Te, Ve = one_body_integrals(be, atoms)
+    Sp, Tp, Vp = one_body_integrals(bp, atoms)
+    return {"S": [Se, Sp], "T": [Te, Tp], "V": [Ve, Vp], "eri_ee": eri_two_bases(be, be),
+            "eri_pp": eri_two_bases(bp, bp), "eri_ep": eri_two_bases(be, bp), "Z": ZNUC[symbol]}
 
 
 def nuclear_repulsion(atoms: List[Tuple[float, np.ndarray]]) -> float:

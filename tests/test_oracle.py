@@ -470,6 +470,25 @@ def test_frozen_core_oracle_water_equals_restricted_full_space():
     assert np.max(np.abs(E - ref)) < 1e-9
 
 
+def test_frozen_core_energy_matches_reference_known_answer():
+    """The reference's frozen-core energy test (ci_test.cpp:131-146): water STO-3G
+    (tests/data/h2o_sto3gfile: the water geometry of the other fixtures, atoms listed H, H, O),
+    restricted, frozen_core = 2: frozen_core_energy[0] = -71.3745646924 Eh (checked there to
+    1e-6).  Here: RHF orbitals from this repository's integrals, the restated operator loop of
+    compute_frozen_core_ints and the restricted energy sum D (H_core + F) of calculate_fc_energy
+    (epci.cpp:118-125).  A dropped exchange term, a wrong Coulomb weight or a core of one orbital
+    misses it by more than 0.1 Eh."""
+    from polyquant_amd import molint
+
+    w = molint.water("sto-3g")
+    n, nfc = w["n_orb"], 2
+    C, T, V, X = w["C"], w["T"], w["V"], w["eri_ao"]
+    D = C[:, :nfc] @ C[:, :nfc].T
+    F = refcpu.frozen_core_ints_block(np.zeros((n, n)), X, [D], True, 0, 0, True, 1.0) + T + V
+    efc = float(np.sum(D * (T + V + F)))
+    assert abs(efc - (-71.3745646924)) < 1e-6, efc
+
+
 def test_ao2mo_two_coefficient_matrices_closed_form():
     """The transform with mo_coeffs_a != mo_coeffs_b (integral.cpp:207-477): row pair ij over Ca,
     column pair kl over Cb, against the einsum of the quarter transforms."""
@@ -484,3 +503,103 @@ def test_ao2mo_two_coefficient_matrices_closed_form():
     ii, jj = np.array([(i, j) for i in range(n) for j in range(i + 1)]).T
     got = refcpu.ao2mo(X, Ca, Cb)
     assert np.max(np.abs(got - ref[ii[:, None], jj[:, None], ii[None, :], jj[None, :]])) < 1e-11
+
+
+def test_ao2mo_two_bases_closed_form():
+    """The electron-positron block of transform_mo_2_body_integrals (integral.cpp:207-477 with
+    num_ao_a != num_ao_b): for (pq|rs) = sum_k A_k,pq B_k,rs with p, q in basis a (7 functions)
+    and r, s in basis b (4), and rectangular coefficients (5 and 3 of the orbitals, as with frozen
+    core or deleted virtuals), (ij|kl) = sum_k (Ca^T A_k Ca)_ij (Cb^T B_k Cb)_kl; identity
+    coefficients give back the AO block.  A transposed coefficient matrix or a swapped pair of
+    bases changes the shapes or the values."""
+    rng = np.random.default_rng(31)
+    na, nb, K = 7, 4, 9
+    A = rng.standard_normal((K, na, na))
+    A = A + A.transpose(0, 2, 1)
+    B = rng.standard_normal((K, nb, nb))
+    B = B + B.transpose(0, 2, 1)
+    ao = np.einsum("kpq,krs->pqrs", A, B)
+    Ca = rng.standard_normal((na, 5))
+    Cb = rng.standard_normal((nb, 3))
+    got = refcpu.ao2mo(ao, Ca, Cb)
+    assert got.shape == (wl.npair(5), wl.npair(3))
+    Ma = np.einsum("pi,kpq,qj->kij", Ca, A, Ca)
+    Mb = np.einsum("rk,nrs,sl->nkl", Cb, B, Cb)
+    ia, ja = np.tril_indices(5)
+    ib, jb = np.tril_indices(3)
+    pa = wl.pair_index_matrix(5)[ia, ja]
+    pb = wl.pair_index_matrix(3)[ib, jb]
+    exp = np.zeros_like(got)
+    exp[pa[:, None], pb[None, :]] = np.einsum("nx,ny->xy", Ma[:, ia, ja], Mb[:, ib, jb])
+    assert np.max(np.abs(got - exp)) < 1e-10 * np.max(np.abs(exp))
+    ident = refcpu.ao2mo(ao, np.eye(na), np.eye(nb))
+    ia, ja = np.tril_indices(na)
+    ib, jb = np.tril_indices(nb)
+    pa = wl.pair_index_matrix(na)[ia, ja]
+    pb = wl.pair_index_matrix(nb)[ib, jb]
+    assert np.max(np.abs(ident[pa[:, None], pb[None, :]] - ao[ia[:, None], ja[:, None], ib[None, :], jb[None, :]])) < 1e-12
+
+
+def test_electron_positron_ao_blocks_share_coinciding_functions():
+    """molint's electron-positron tensor (the AO input of the two-basis transform) on the PsH bases
+    of the reference's tests: the positron functions (exponents 0.02 and 0.009) are the last two
+    electron functions, so (pq|rs)_ep equals (pq|r+3 s+3)_ee; the one-body integrals of those
+    functions agree likewise."""
+    from polyquant_amd import molint
+
+    a = molint.electron_positron_atom("H", "psh_electron", "psh_positron")
+    assert np.max(np.abs(a["eri_ep"] - a["eri_ee"][:, :, 3:, 3:])) < 1e-14
+    assert np.max(np.abs(a["eri_pp"] - a["eri_ee"][3:, 3:, 3:, 3:])) < 1e-14
+    for key in ("S", "T", "V"):
+        assert np.max(np.abs(a[key][1] - a[key][0][3:, 3:])) < 1e-14
+
+
+def test_two_species_frozen_core_oracle_equals_restricted_full_space():
+    """The restated frozen-core loop for two species (integral.cpp:166-205: the positron's operator
+    gets charge_e * charge_p * J of the electron core density, no exchange) with the restricted
+    frozen-core energy (epci.cpp:118-125) and the two-basis quarter transform over the active
+    orbitals: Li- + e+ (s shells of the reference's Li_basis.g94 for both species), electron 1s
+    frozen and the top orbital of each species deleted, reproduces the lowest eigenvalues of the
+    full-space Hamiltonian restricted to the determinants with the core doubly occupied and the
+    deleted orbitals empty.  A wrong sign of the charge product or a dropped species term fails by
+    more than 1e-3."""
+    from polyquant_amd import molint
+
+    a = molint.electron_positron_atom("Li", "li_s", "li_s")
+    n = a["S"][0].shape[0]
+    w, U = np.linalg.eigh(a["S"][0])
+    X = U @ np.diag(w ** -0.5) @ U.T
+    _, Ue = np.linalg.eigh(X.T @ (a["T"][0] + a["V"][0]) @ X)
+    Ce = X @ Ue
+    w, U = np.linalg.eigh(a["S"][1])
+    Cp = U @ np.diag(w ** -0.5) @ U.T
+    He, Hp = a["T"][0] + a["V"][0], a["T"][1] - a["V"][1]
+    D = Ce[:, :1] @ Ce[:, :1].T
+    Fe = refcpu.frozen_core_ints_block(np.zeros((n, n)), a["eri_ee"], [D], True, 0, 0, True, 1.0) + He
+    Fp = refcpu.frozen_core_ints_block(np.zeros((n, n)), a["eri_ep"].transpose(2, 3, 0, 1), [D], False, 0, 0, True,
+                                       -1.0) + Hp
+    efc = float(np.sum(D * (He + Fe)))
+    se, sp_ = Ce[:, 1:n - 1], Cp[:, :n - 1]
+    na, npo = n - 2, n - 1
+
+    act = wl.CIProblem(species=[wl.Species(na, 1, 1, -1.0, 1, None), wl.Species(npo, 1, 0, 1.0, 1, None)],
+                       h=[[se.T @ Fe @ se], [sp_.T @ Fp @ sp_]],
+                       eri={(0, 0, 0, 0): refcpu.ao2mo(a["eri_ee"], se), (1, 0, 1, 0): refcpu.ao2mo(a["eri_pp"], sp_),
+                            (0, 0, 1, 0): refcpu.ao2mo(a["eri_ep"], se, sp_)}, nroots=3, e_shift=efc)
+    E = np.linalg.eigvalsh(refcpu.OracleCI(act).dense())[:3] + efc
+    full = wl.CIProblem(species=[wl.Species(n, 2, 2, -1.0, 1, None), wl.Species(n, 1, 0, 1.0, 1, None)],
+                        h=[[Ce.T @ He @ Ce], [Cp.T @ Hp @ Cp]],
+                        eri={(0, 0, 0, 0): refcpu.ao2mo(a["eri_ee"], Ce), (1, 0, 1, 0): refcpu.ao2mo(a["eri_pp"], Cp),
+                             (0, 0, 1, 0): refcpu.ao2mo(a["eri_ep"], Ce, Cp)}, nroots=3)
+    oc = refcpu.OracleCI(full)
+    Sa, Sb, Sp, U4 = oc.strings(0, 0), oc.strings(0, 1), oc.strings(1, 0), oc.unfold_all()
+    top = n - 1
+    ok_e = lambda x: (int(x) & 1) and not (int(x) >> top) & 1  # noqa: E731
+    keep = np.array([i for i in range(oc.N) if ok_e(Sa[U4[i, 0]]) and ok_e(Sb[U4[i, 1]])
+                     and not (int(Sp[U4[i, 2]]) >> top) & 1], dtype=np.int64)
+    assert len(keep) == na * na * npo
+    I = np.zeros((oc.N, len(keep)))
+    I[keep, np.arange(len(keep))] = 1.0
+    H = oc.sigma_rows(I, keep)
+    ref = np.linalg.eigvalsh(0.5 * (H + H.T))[:3]
+    assert np.max(np.abs(E - ref)) < 1e-9, (E, ref)
