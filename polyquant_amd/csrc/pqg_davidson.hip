@@ -44,85 +44,114 @@ __global__ __launch_bounds__(256) void k_tn_reduce(const double* __restrict__ pa
   if (threadIdx.x == 0) out[(o / nb) + (o % nb) * ldo] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// Partials of A^T B for tall-skinny A (nrows x na) and B (nrows x nb), column-major: wave g sums
-// rows [g rpw, (g + 1) rpw) into part[g][a][b].  Lane = row; 16 columns of A and 4 of B are held
-// in registers at a time, so a row of A is read once per 4 columns of B and a row of B once per
-// 16 columns of A (each operand once when na <= 16 or nb <= 4).  Every wave sums its rows in the
-// same order and the partials are summed in a fixed order: deterministic.
-__global__ __launch_bounds__(256) void k_tn_waves(const double* __restrict__ A, int64_t lda, int na,
-                                                  const double* __restrict__ B, int64_t ldb, int nb, int64_t nrows,
-                                                  int64_t rpw, double* __restrict__ part) {
+// Partials of A^T B for tall-skinny A (nrows x na) and B (nrows x nb), column-major, every column
+// 16-byte aligned (even leading dimensions, aligned bases).  Block b sums the rows
+// [b rpb, (b + 1) rpb); each of its waves owns a task of 8 columns of A x 4 columns of B
+// (blockIdx.y selects further groups of four tasks), so a lane keeps 32 accumulators and takes
+// two consecutive rows per step with 16-byte loads: many waves per CU, each with twelve 1 KB
+// loads in flight, which keeps HBM busy (a B column read by the several waves of a block comes
+// from L2 after the first).  The wave's sums meet by shuffles; part[b][a][c] is summed over b in
+// a fixed order afterwards: deterministic.
+constexpr int kTnQA = 8;
+__global__ __launch_bounds__(256) void k_tn_part(const double* __restrict__ A, int64_t lda, int na,
+                                                 const double* __restrict__ B, int64_t ldb, int nb, int64_t nrows,
+                                                 int64_t rpb, double* __restrict__ part) {
   const int lane = threadIdx.x & 63;
-  const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t r0 = g * rpw, r1 = min(nrows, r0 + rpw);
-  if (r0 >= nrows) return;  // waves past the last slice (the launch rounds up to whole blocks)
-  for (int b0 = 0; b0 < nb; b0 += 4) {
-    const int nbl = min(4, nb - b0);
-    for (int a0 = 0; a0 < na; a0 += 16) {
-      const int nal = min(16, na - a0);
-      double acc[16][4];
+  const int task = (threadIdx.x >> 6) + blockIdx.y * (blockDim.x >> 6);
+  const int ta = (na + kTnQA - 1) / kTnQA;
+  if (task >= ta * ((nb + 3) / 4)) return;  // idle waves of the last task group
+  const int a0 = (task % ta) * kTnQA, b0 = (task / ta) * 4;
+  const int nal = min(kTnQA, na - a0), nbl = min(4, nb - b0);
+  const int64_t r0 = (int64_t)blockIdx.x * rpb, r1 = min(nrows, r0 + rpb);
+  double acc[kTnQA][4];
 #pragma unroll
-      for (int i = 0; i < 16; i++)
+  for (int i = 0; i < kTnQA; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-      for (int64_t r = r0 + lane; r < r1; r += 64) {
-        double y[4];
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+  for (int64_t r = r0 + 2 * lane; r < r1; r += 128) {
+    const bool full = r + 1 < r1;
+    double2 y[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) y[j] = j < nbl ? B[r + (int64_t)(b0 + j) * ldb] : 0.0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-          if (i < nal) {
-            const double x = A[r + (int64_t)(a0 + i) * lda];
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] += x * y[j];
-          }
-        }
+    for (int j = 0; j < 4; j++) {
+      y[j] = make_double2(0.0, 0.0);
+      if (j < nbl) {
+        const double* q = B + r + (int64_t)(b0 + j) * ldb;
+        y[j] = full ? *reinterpret_cast<const double2*>(q) : make_double2(q[0], 0.0);
       }
+    }
 #pragma unroll
-      for (int i = 0; i < 16; i++)
+    for (int i = 0; i < kTnQA; i++) {
+      if (i < nal) {
+        const double* q = A + r + (int64_t)(a0 + i) * lda;
+        const double2 x = full ? *reinterpret_cast<const double2*>(q) : make_double2(q[0], 0.0);
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          double v = acc[i][j];
-          for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-          if (lane == 0 && i < nal && j < nbl) part[(g * na + a0 + i) * nb + b0 + j] = v;
-        }
+        for (int j = 0; j < 4; j++) acc[i][j] += x.x * y[j].x + x.y * y[j].y;
+      }
     }
   }
+#pragma unroll
+  for (int i = 0; i < kTnQA; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      double v = acc[i][j];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0 && i < nal && j < nbl) part[((int64_t)blockIdx.x * na + a0 + i) * nb + b0 + j] = v;
+    }
 }
 
 // out[r][j] = beta * out[r][j] + alpha * sum_c A[r][c] Bs[c][j]   (Bs small, ka x nb, in LDS).
-// blockIdx.y selects (A0, out0) or (A1, out1), so two products with the same Bs share a launch.
-// A row is read completely before its outputs are written, so out may alias A when nb <= 8.
+// A thread takes two consecutive rows (16-byte loads and stores; columns 16-byte aligned) and
+// up to NB output columns at once; wider products run in column chunks of NB.  blockIdx.y
+// selects (A0, out0) or (A1, out1), so two products with the same Bs share a launch.  A row is
+// read completely before its outputs are written, so out may alias A when nb <= NB.
+template <int NB>
 __global__ __launch_bounds__(kTB) void k_nn(const double* A0, const double* A1, int64_t lda, int ka,
                                             const double* __restrict__ Bs, int ldb, int nb, double* out0,
                                             double* out1, int64_t ldo, int64_t nrows, double alpha, double beta) {
-  __shared__ double bs[64 * 64];
-  for (int i = threadIdx.x; i < ka * nb; i += kTB) {
-    const int c = i % ka, j = i / ka;
-    bs[c + j * 64] = Bs[c + j * ldb];
-  }
-  __syncthreads();
-  const int64_t r = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (r >= nrows) return;
+  __shared__ double bs[64 * NB];
+  const int64_t r = ((int64_t)blockIdx.x * kTB + threadIdx.x) * 2;
   const double* A = blockIdx.y ? A1 : A0;
   double* out = blockIdx.y ? out1 : out0;
-  for (int j0 = 0; j0 < nb; j0 += 8) {
-    double acc[8];
+  const bool full = r + 1 < nrows;
+  for (int j0 = 0; j0 < nb; j0 += NB) {
+    const int nbl = min(NB, nb - j0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < ka * NB; i += kTB) {
+      const int c = i / NB, j = i % NB;
+      bs[i] = j < nbl ? Bs[c + (j0 + j) * ldb] : 0.0;
+    }
+    __syncthreads();
+    if (r >= nrows) continue;
+    double2 acc[NB];
 #pragma unroll
-    for (int j = 0; j < 8; j++) acc[j] = 0.0;
-#pragma unroll 16
+    for (int j = 0; j < NB; j++) acc[j] = make_double2(0.0, 0.0);
+#pragma unroll 8
     for (int c = 0; c < ka; c++) {
-      const double a = A[r + c * lda];
+      const double* q = A + r + c * lda;
+      const double2 a = full ? *reinterpret_cast<const double2*>(q) : make_double2(q[0], 0.0);
 #pragma unroll
-      for (int j = 0; j < 8; j++)
-        if (j0 + j < nb) acc[j] += a * bs[c + (j0 + j) * 64];
+      for (int j = 0; j < NB; j++) {
+        const double b = bs[c * NB + j];
+        acc[j].x += a.x * b;
+        acc[j].y += a.y * b;
+      }
     }
 #pragma unroll
-    for (int j = 0; j < 8; j++)
-      if (j0 + j < nb) {
-        double* o = out + r + (j0 + j) * ldo;
-        *o = (beta == 0.0 ? 0.0 : beta * *o) + alpha * acc[j];
+    for (int j = 0; j < NB; j++) {
+      if (j >= nbl) break;
+      double* o = out + r + (int64_t)(j0 + j) * ldo;
+      if (full) {
+        double2 v = make_double2(alpha * acc[j].x, alpha * acc[j].y);
+        if (beta != 0.0) {
+          const double2 q = *reinterpret_cast<const double2*>(o);
+          v.x += beta * q.x;
+          v.y += beta * q.y;
+        }
+        *reinterpret_cast<double2*>(o) = v;
+      } else {
+        o[0] = (beta == 0.0 ? 0.0 : beta * o[0]) + alpha * acc[j].x;
       }
+    }
   }
 }
 
@@ -130,16 +159,16 @@ __global__ __launch_bounds__(kTB) void k_nn(const double* A0, const double* A1, 
 // give zero corrections.
 __global__ void k_residual(const double* __restrict__ X, const double* __restrict__ AX, const double* __restrict__ theta,
                            const double* __restrict__ Hd, double* __restrict__ R, double* __restrict__ T, int64_t n,
-                           int nev) {
+                           int64_t ld, int nev) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double d = Hd[i];
   for (int k = 0; k < nev; k++) {
-    const double r = AX[i + k * n] - theta[k] * X[i + k * n];
-    R[i + k * n] = r;
+    const double r = AX[i + k * ld] - theta[k] * X[i + k * ld];
+    R[i + k * ld] = r;
     double den = theta[k] - d;
     if (fabs(den) < 1e-8) den = den < 0 ? -1e-8 : 1e-8;
-    T[i + k * n] = (d >= 1e29) ? 0.0 : r / den;
+    T[i + k * ld] = (d >= 1e29) ? 0.0 : r / den;
   }
 }
 
@@ -462,6 +491,7 @@ static void dav_log(const char* tag, int iter, int m, double tol, const std::vec
 
 Davidson::Davidson(pqg_ci* ci, const pqg_dav_params& p) : ci_(ci), p_(p) {
   N_ = ci->n_int;
+  ld_ = (N_ + 31) / 32 * 32;  // every column 256-byte aligned
   nev_ = std::max(1, p.nev);
   PQG_REQUIRE(nev_ <= 64 && nev_ <= ci->n_ref, PQG_ERR_ARG, "nev out of range");
   ninit_ = std::max(p.nvec_init, nev_);
@@ -471,17 +501,17 @@ Davidson::Davidson(pqg_ci* ci, const pqg_dav_params& p) : ci_(ci), p_(p) {
   PQG_REQUIRE(nmax_ >= ninit_, PQG_ERR_ARG, "subspace too small");
   mixed_ = p.precision == PQG_MIXED;
   pqg_ctx* ctx = ci->ctx;
-  V_.reset(ctx, (size_t)N_ * nmax_);
-  W_.reset(ctx, (size_t)N_ * nmax_);
+  V_.reset(ctx, (size_t)ld_ * nmax_);
+  W_.reset(ctx, (size_t)ld_ * nmax_);
   G_.reset(ctx, 64 * 64);
   Y_.reset(ctx, 64 * 64);
   Yprev_.reset(ctx, 64 * 64);
   Ynext_.reset(ctx, 64 * 64);
   theta_.reset(ctx, 64);
-  X_.reset(ctx, (size_t)N_ * nev_);
-  AX_.reset(ctx, (size_t)N_ * nev_);
-  RT_.reset(ctx, (size_t)N_ * nev_ * 2);  // residuals, then corrections (one Gram covers both)
-  tmp_.reset(ctx, (size_t)N_ * ninit_);
+  X_.reset(ctx, (size_t)ld_ * nev_);
+  AX_.reset(ctx, (size_t)ld_ * nev_);
+  RT_.reset(ctx, (size_t)ld_ * nev_ * 2);  // residuals, then corrections (one Gram covers both)
+  tmp_.reset(ctx, (size_t)ld_ * ninit_);
   small_.reset(ctx, 64 * 64);
   gram0_.reset(ctx, 64 * 64);
   rgram_.reset(ctx, 64 * 64);
@@ -494,7 +524,7 @@ Davidson::Davidson(pqg_ci* ci, const pqg_dav_params& p) : ci_(ci), p_(p) {
   nchunk_ = (int)((N_ + 511) / 512);
   part_.reset(ctx, (size_t)std::min<int64_t>(nchunk_, 1024) * 64 * 64);
   if (mixed_) {
-    W64_.reset(ctx, (size_t)N_ * nev_);
+    W64_.reset(ctx, (size_t)ld_ * nev_);
     c32_.reset(ctx, (size_t)N_ * nev_ > (size_t)N_ * ninit_ ? (size_t)N_ * nev_ : (size_t)N_ * ninit_);
     s32_.reset(ctx, c32_.n);
     ensure_mixed(ci);
@@ -508,33 +538,63 @@ uint64_t Davidson::device_bytes() const {
          c32_.bytes() + s32_.bytes() + part_.bytes();
 }
 
+// the tall-skinny kernels read two rows per 16-byte access: every column 16-byte aligned
+static void ts_check(const double* p, int64_t ld) {
+  PQG_REQUIRE(((uintptr_t)p & 15) == 0 && (ld & 1) == 0, PQG_ERR_ARG, "tall-skinny operand not 16-byte aligned");
+}
+
 void Davidson::ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, const double* B, int64_t ldb, int nb,
                           double* out, int ldo) {
-  // one wave per row slice of at least 4096 rows, at most 3072 waves (the 3 per SIMD its
-  // registers allow on 256 CUs) and what the partial buffer holds
-  int64_t waves = std::min<int64_t>(3072, (N_ + 4095) / 4096);
-  waves = std::min<int64_t>(waves, (int64_t)(part_.n / std::max(1, na * nb)));
-  waves = std::max<int64_t>(1, waves / 4 * 4);
-  const int64_t rpw = (N_ + waves - 1) / waves;
-  const int nwave = (int)((N_ + rpw - 1) / rpw);
-  PQG_REQUIRE((size_t)nwave * na * nb <= part_.n, PQG_ERR_ARG, "tall-skinny partial buffer too small");
-  k_tn_waves<<<(nwave + 3) / 4, 256, 0, st>>>(A, lda, na, B, ldb, nb, N_, rpw, part_.get());
+  ts_check(A, lda);
+  ts_check(B, ldb);
+  PQG_REQUIRE(na >= 1 && nb >= 1 && na <= 64 && nb <= 64, PQG_ERR_ARG, "tall-skinny product shape");
+  // rows per block: about 8192 blocks, at least 256 rows, and no more blocks than the partial
+  // buffer holds; a multiple of the 128 rows a wave takes per step
+  const int64_t G = (int64_t)na * nb;
+  int64_t rpb = std::max<int64_t>(256, (N_ + 8191) / 8192);
+  const int64_t maxblk = (int64_t)part_.n / G;
+  rpb = std::max<int64_t>(rpb, (N_ + maxblk - 1) / maxblk);
+  rpb = (rpb + 127) / 128 * 128;
+  const int64_t nblk = (N_ + rpb - 1) / rpb;
+  PQG_REQUIRE((size_t)(nblk * G) <= part_.n, PQG_ERR_ARG, "tall-skinny partial buffer too small");
+  const int tasks = ((na + kTnQA - 1) / kTnQA) * ((nb + 3) / 4);
+  const int per = std::min(tasks, 4);
+  dim3 grid((unsigned)nblk, (unsigned)((tasks + per - 1) / per));
+  k_tn_part<<<grid, 64 * per, 0, st>>>(A, lda, na, B, ldb, nb, N_, rpb, part_.get());
   PQG_HIP(hipGetLastError());
-  k_tn_reduce<<<na * nb, 256, 0, st>>>(part_.get(), nwave, na, nb, out, ldo);
+  k_tn_reduce<<<(unsigned)G, 256, 0, st>>>(part_.get(), (int)nblk, na, nb, out, ldo);
+  PQG_HIP(hipGetLastError());
+}
+
+static void nn_launch(dim3 g, hipStream_t st, const double* A0, const double* A1, int64_t lda, int ka,
+                      const double* Bs, int ldb, int nb, double* out0, double* out1, int64_t ldo, int64_t nrows,
+                      double alpha, double beta) {
+  if (nb <= 4)
+    k_nn<4><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta);
+  else if (nb <= 8)
+    k_nn<8><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta);
+  else
+    k_nn<16><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta);
   PQG_HIP(hipGetLastError());
 }
 
 void Davidson::ts_gemm_nn(hipStream_t st, const double* A, int64_t lda, int ka, const double* Bs, int ldb, int nb,
                           double* out, int64_t ldo, double alpha, double beta) {
-  k_nn<<<g1(N_, kTB), kTB, 0, st>>>(A, A, lda, ka, Bs, ldb, nb, out, out, ldo, N_, alpha, beta);
-  PQG_HIP(hipGetLastError());
+  ts_check(A, lda);
+  ts_check(out, ldo);
+  PQG_REQUIRE(ka >= 1 && ka <= 64 && nb >= 1, PQG_ERR_ARG, "tall-skinny product shape");
+  PQG_REQUIRE(out != A || nb <= 16, PQG_ERR_ARG, "in-place product wider than one column chunk");
+  nn_launch(dim3(g1((N_ + 1) / 2, kTB)), st, A, A, lda, ka, Bs, ldb, nb, out, out, ldo, N_, alpha, beta);
 }
 
 void Davidson::ts_gemm_nn2(hipStream_t st, const double* A0, const double* A1, int64_t lda, int ka, const double* Bs,
                            int ldb, int nb, double* out0, double* out1, int64_t ldo) {
-  dim3 g(g1(N_, kTB), 2);
-  k_nn<<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, N_, 1.0, 0.0);
-  PQG_HIP(hipGetLastError());
+  ts_check(A0, lda);
+  ts_check(A1, lda);
+  ts_check(out0, ldo);
+  ts_check(out1, ldo);
+  PQG_REQUIRE(ka >= 1 && ka <= 64 && nb >= 1, PQG_ERR_ARG, "tall-skinny product shape");
+  nn_launch(dim3(g1((N_ + 1) / 2, kTB), 2), st, A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, N_, 1.0, 0.0);
 }
 
 void Davidson::initial_guess(hipStream_t st) {
@@ -550,25 +610,25 @@ void Davidson::initial_guess(hipStream_t st) {
   PQG_HIP(hipMemcpyAsync(pos.get(), idx.data(), ninit_ * 8, hipMemcpyHostToDevice, st));
   if (p_.guess_perturb > 0.0) {
     // per-entry size guess_perturb * sqrt(3 / N): the admixture has norm ~guess_perturb
-    k_unit_perturbed<<<g1(N_, 256), 256, 0, st>>>(V_.get(), N_, N_, pos.get(), ninit_,
+    k_unit_perturbed<<<g1(N_, 256), 256, 0, st>>>(V_.get(), N_, ld_, pos.get(), ninit_,
                                                   p_.guess_perturb * std::sqrt(3.0 / (double)N_));
     PQG_HIP(hipGetLastError());
-    mask_invalid<double>(ci_, V_.get(), N_, ninit_, st);
+    mask_invalid<double>(ci_, V_.get(), ld_, ninit_, st);
     for (int pass = 0; pass < 2; pass++) {  // CholQR2
-      ts_gemm_tn(st, V_.get(), N_, ninit_, V_.get(), N_, ninit_, small_.get(), 64);
+      ts_gemm_tn(st, V_.get(), ld_, ninit_, V_.get(), ld_, ninit_, small_.get(), 64);
       k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nullptr, ninit_, nullptr, 0.0, mbuf_.get(), nk_dev_.get());
       PQG_HIP(hipGetLastError());
-      ts_gemm_nn(st, V_.get(), N_, ninit_, mbuf_.get(), 64, ninit_, tmp_.get(), N_, 1.0, 0.0);
-      PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)N_ * ninit_ * 8, hipMemcpyDeviceToDevice, st));
+      ts_gemm_nn(st, V_.get(), ld_, ninit_, mbuf_.get(), 64, ninit_, tmp_.get(), ld_, 1.0, 0.0);
+      PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)ld_ * ninit_ * 8, hipMemcpyDeviceToDevice, st));
     }
   } else {
-    k_unit<<<g1(N_, 256), 256, 0, st>>>(V_.get(), N_, N_, pos.get(), ninit_);
+    k_unit<<<g1(N_, 256), 256, 0, st>>>(V_.get(), N_, ld_, pos.get(), ninit_);
     PQG_HIP(hipGetLastError());
   }
   PQG_HIP(hipStreamSynchronize(st));
   m_ = ninit_;
   nW_ = 0;
-  work_ = SigmaWork{m_, mixed_ ? PQG_MIXED : PQG_FP64, V_.get(), W_.get(), N_};
+  work_ = SigmaWork{m_, mixed_ ? PQG_MIXED : PQG_FP64, V_.get(), W_.get(), ld_};
 }
 
 void Davidson::run_sigma(hipStream_t st) {
@@ -600,30 +660,30 @@ void Davidson::append_async(hipStream_t st, double* T, int nt, const double* g0,
   pending_nt_ = nt;
   if (nt <= 0) return;
   if (!g0) {
-    ts_gemm_tn(st, T, N_, nt, T, N_, nt, gram0_.get(), 64);
+    ts_gemm_tn(st, T, ld_, nt, T, ld_, nt, gram0_.get(), 64);
     g0 = gram0_.get();
   }
   for (int pass = 0; pass < 2; pass++) {
-    ts_gemm_tn(st, V_.get(), N_, m_, T, N_, nt, small_.get(), 64);
-    ts_gemm_nn(st, V_.get(), N_, m_, small_.get(), 64, nt, T, N_, -1.0, 1.0);
+    ts_gemm_tn(st, V_.get(), ld_, m_, T, ld_, nt, small_.get(), 64);
+    ts_gemm_nn(st, V_.get(), ld_, m_, small_.get(), 64, nt, T, ld_, -1.0, 1.0);
   }
   // pass 1: drop vanishing columns, orthonormalise the rest straight into V[:, m:]
-  ts_gemm_tn(st, T, N_, nt, T, N_, nt, small_.get(), 64);
+  ts_gemm_tn(st, T, ld_, nt, T, ld_, nt, small_.get(), 64);
   k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nullptr, nt, g0, 1e-10, mbuf_.get(), nk_dev_.get(), rg, p_.tol);
   PQG_HIP(hipGetLastError());
-  double* dst = V_.get() + (int64_t)m_ * N_;
-  ts_gemm_nn(st, T, N_, nt, mbuf_.get(), 64, nt, dst, N_, 1.0, 0.0);
+  double* dst = V_.get() + (int64_t)m_ * ld_;
+  ts_gemm_nn(st, T, ld_, nt, mbuf_.get(), 64, nt, dst, ld_, 1.0, 0.0);
   // pass 2 (CholQR2): re-orthonormalise the new block
-  ts_gemm_tn(st, dst, N_, nt, dst, N_, nt, small_.get(), 64);
+  ts_gemm_tn(st, dst, ld_, nt, dst, ld_, nt, small_.get(), 64);
   k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nk_dev_.get(), 0, nullptr, 0.0, mbuf_.get(), nk_dev_.get() + 1);
   PQG_HIP(hipGetLastError());
   if (nt <= 8) {  // in place: each row is read before it is written
-    ts_gemm_nn(st, dst, N_, nt, mbuf_.get(), 64, nt, dst, N_, 1.0, 0.0);
+    ts_gemm_nn(st, dst, ld_, nt, mbuf_.get(), 64, nt, dst, ld_, 1.0, 0.0);
   } else {
-    ts_gemm_nn(st, dst, N_, nt, mbuf_.get(), 64, nt, tmp_.get(), N_, 1.0, 0.0);
-    PQG_HIP(hipMemcpyAsync(dst, tmp_.get(), (size_t)N_ * nt * 8, hipMemcpyDeviceToDevice, st));
+    ts_gemm_nn(st, dst, ld_, nt, mbuf_.get(), 64, nt, tmp_.get(), ld_, 1.0, 0.0);
+    PQG_HIP(hipMemcpyAsync(dst, tmp_.get(), (size_t)ld_ * nt * 8, hipMemcpyDeviceToDevice, st));
   }
-  mask_invalid<double>(ci_, dst, N_, nt, st);
+  mask_invalid<double>(ci_, dst, ld_, nt, st);
 }
 
 // Commits the block appended by append_async (status_h_ must hold the counts): returns how many
@@ -657,13 +717,13 @@ int Davidson::orthonormalize_append(hipStream_t st, double* T, int nt) {
 // Thick restart on the first `keep` Ritz vectors of the current subspace (Y from the last
 // Rayleigh-Ritz); the projected matrix is recomputed exactly.
 void Davidson::restart_to_ritz(hipStream_t st, int keep) {
-  ts_gemm_nn(st, V_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
-  PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
-  ts_gemm_nn(st, W_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
-  PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
+  ts_gemm_nn(st, V_.get(), ld_, m_, Y_.get(), 64, keep, tmp_.get(), ld_, 1.0, 0.0);
+  PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)ld_ * keep * 8, hipMemcpyDeviceToDevice, st));
+  ts_gemm_nn(st, W_.get(), ld_, m_, Y_.get(), 64, keep, tmp_.get(), ld_, 1.0, 0.0);
+  PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)ld_ * keep * 8, hipMemcpyDeviceToDevice, st));
   m_ = nW_ = keep;
   warm_ = 0;
-  ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, G_.get(), 64);
+  ts_gemm_tn(st, V_.get(), ld_, keep, W_.get(), ld_, keep, G_.get(), 64);
   restarts_++;
 }
 
@@ -676,9 +736,9 @@ void Davidson::rayleigh_ritz(hipStream_t st) {
   const int newc = m_ - nW_;
   const double* gt = nullptr;
   if (newc > 0) {
-    ts_gemm_tn(st, V_.get(), N_, m_, W_.get() + (int64_t)nW_ * N_, N_, newc, G_.get() + (int64_t)nW_ * 64, 64);
+    ts_gemm_tn(st, V_.get(), ld_, m_, W_.get() + (int64_t)nW_ * ld_, ld_, newc, G_.get() + (int64_t)nW_ * 64, 64);
     if (mixed_ && old > 0) {
-      ts_gemm_tn(st, V_.get() + (int64_t)old * N_, N_, newc, W_.get(), N_, old, small_.get(), 64);
+      ts_gemm_tn(st, V_.get() + (int64_t)old * ld_, ld_, newc, W_.get(), ld_, old, small_.get(), 64);
       gt = small_.get();
     }
     nW_ = m_;
@@ -704,7 +764,7 @@ void Davidson::post_sigma(hipStream_t st) {
     checking_ = false;
     since_check_ = 0;
     {  // final energies: Rayleigh quotients x^T H x of the (orthonormal) Ritz vectors in FP64
-      ts_gemm_tn(st, X_.get(), N_, nev_, W64_.get(), N_, nev_, small_.get(), 64);
+      ts_gemm_tn(st, X_.get(), ld_, nev_, W64_.get(), ld_, nev_, small_.get(), 64);
       std::vector<double> q(64 * 64);
       PQG_HIP(hipMemcpyAsync(q.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
       PQG_HIP(hipStreamSynchronize(st));
@@ -712,9 +772,9 @@ void Davidson::post_sigma(hipStream_t st) {
       PQG_HIP(hipMemcpyAsync(theta_.get(), theta_h_.data(), nev_ * 8, hipMemcpyHostToDevice, st));
     }
     k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), W64_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(),
-                                            N_, nev_);
+                                            N_, ld_, nev_);
     PQG_HIP(hipGetLastError());
-    ts_gemm_tn(st, Rp(), N_, nev_, Rp(), N_, nev_, small_.get(), 64);
+    ts_gemm_tn(st, Rp(), ld_, nev_, Rp(), ld_, nev_, small_.get(), 64);
     std::vector<double> g(64 * 64);
     PQG_HIP(hipMemcpyAsync(g.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
     PQG_HIP(hipStreamSynchronize(st));
@@ -741,15 +801,15 @@ void Davidson::post_sigma(hipStream_t st) {
     // defect correction: restart on the Ritz vectors, the first nev with their exact FP64
     // sigma, the next ones (up to nvec_init) with the sigma of the current basis
     const int keep = std::max(nev_, std::min(ninit_, m_));
-    ts_gemm_nn(st, V_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
-    PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
-    ts_gemm_nn(st, W_.get(), N_, m_, Y_.get(), 64, keep, tmp_.get(), N_, 1.0, 0.0);
-    PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)N_ * keep * 8, hipMemcpyDeviceToDevice, st));
-    PQG_HIP(hipMemcpyAsync(W_.get(), W64_.get(), (size_t)N_ * nev_ * 8, hipMemcpyDeviceToDevice, st));
+    ts_gemm_nn(st, V_.get(), ld_, m_, Y_.get(), 64, keep, tmp_.get(), ld_, 1.0, 0.0);
+    PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)ld_ * keep * 8, hipMemcpyDeviceToDevice, st));
+    ts_gemm_nn(st, W_.get(), ld_, m_, Y_.get(), 64, keep, tmp_.get(), ld_, 1.0, 0.0);
+    PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)ld_ * keep * 8, hipMemcpyDeviceToDevice, st));
+    PQG_HIP(hipMemcpyAsync(W_.get(), W64_.get(), (size_t)ld_ * nev_ * 8, hipMemcpyDeviceToDevice, st));
     m_ = nW_ = keep;
     warm_ = 0;
     // projected matrix: couplings to the exact columns are taken as v^T (H x) (FP64)
-    ts_gemm_tn(st, V_.get(), N_, keep, W_.get(), N_, keep, small_.get(), 64);
+    ts_gemm_tn(st, V_.get(), ld_, keep, W_.get(), ld_, keep, small_.get(), 64);
     {
       std::vector<double> q(64 * 64), gk(64 * 64, 0.0);
       PQG_HIP(hipMemcpyAsync(q.data(), small_.get(), 64 * 64 * 8, hipMemcpyDeviceToHost, st));
@@ -770,7 +830,7 @@ void Davidson::post_sigma(hipStream_t st) {
     restarts_++;
     const int before = m_;
     orthonormalize_append(st, Tp(), nev_);
-    work_ = SigmaWork{m_ - before, PQG_MIXED, V_.get() + (int64_t)before * N_, W_.get() + (int64_t)before * N_, N_};
+    work_ = SigmaWork{m_ - before, PQG_MIXED, V_.get() + (int64_t)before * ld_, W_.get() + (int64_t)before * ld_, ld_};
     if (work_.k == 0) {  // nothing new to add: accept
       done_ = true;
       seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
@@ -780,17 +840,17 @@ void Davidson::post_sigma(hipStream_t st) {
   iter_++;
   rayleigh_ritz(st);
   // Ritz vectors and residuals
-  ts_gemm_nn2(st, V_.get(), W_.get(), N_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), N_);
-  k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_,
+  ts_gemm_nn2(st, V_.get(), W_.get(), ld_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), ld_);
+  k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_, ld_,
                                           nev_);
   PQG_HIP(hipGetLastError());
   // one Gram of [R T]: R^T R (residual norms) and T^T T (drop thresholds of the corrections)
   const double* g0 = nullptr;
   if (2 * nev_ <= 64) {
-    ts_gemm_tn(st, Rp(), N_, 2 * nev_, Rp(), N_, 2 * nev_, rgram_.get(), 64);
+    ts_gemm_tn(st, Rp(), ld_, 2 * nev_, Rp(), ld_, 2 * nev_, rgram_.get(), 64);
     g0 = rgram_.get() + nev_ + nev_ * 64;
   } else {
-    ts_gemm_tn(st, Rp(), N_, nev_, Rp(), N_, nev_, rgram_.get(), 64);
+    ts_gemm_tn(st, Rp(), ld_, nev_, Rp(), ld_, nev_, rgram_.get(), 64);
   }
   // The next block is prepared before the convergence test is read back, so an iteration costs
   // one host synchronisation.  A mixed-precision solve may still need the current subspace for
@@ -820,7 +880,7 @@ void Davidson::post_sigma(hipStream_t st) {
     // FP64 check of the Ritz vectors (the speculative block is dropped: m_ is unchanged)
     pending_nt_ = 0;
     checking_ = true;
-    work_ = SigmaWork{nev_, PQG_FP64, X_.get(), W64_.get(), N_};
+    work_ = SigmaWork{nev_, PQG_FP64, X_.get(), W64_.get(), ld_};
     return;
   }
   if (approx_conv || iter_ >= p_.maxit) {
@@ -845,8 +905,8 @@ void Davidson::post_sigma(hipStream_t st) {
     seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
     return;
   }
-  work_ = SigmaWork{added, mixed_ ? PQG_MIXED : PQG_FP64, V_.get() + (int64_t)before * N_,
-                    W_.get() + (int64_t)before * N_, N_};
+  work_ = SigmaWork{added, mixed_ ? PQG_MIXED : PQG_FP64, V_.get() + (int64_t)before * ld_,
+                    W_.get() + (int64_t)before * ld_, ld_};
 }
 
 pqg_dav_report Davidson::report() const {
@@ -890,7 +950,7 @@ void dav_finish(pqg_ci* ci, Davidson& dav, double* E, double* X, int64_t ldx, pq
   if (X) {
     PQG_REQUIRE(ldx >= ci->n_ref, PQG_ERR_ARG, "ldx smaller than N");
     DevBuf<double> xr(ci->ctx, (size_t)ci->n_ref * dav.nev());
-    to_reference(ci, dav.ritz_vectors(), ci->n_int, xr.get(), ci->n_ref, dav.nev(), st);
+    to_reference(ci, dav.ritz_vectors(), dav.ld(), xr.get(), ci->n_ref, dav.nev(), st);
     PQG_HIP(hipMemcpy2DAsync(X, ldx * 8, xr.get(), ci->n_ref * 8, ci->n_ref * 8, dav.nev(), hipMemcpyDeviceToHost, st));
     PQG_HIP(hipStreamSynchronize(st));
   }

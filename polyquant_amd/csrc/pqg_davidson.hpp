@@ -42,7 +42,8 @@ class Davidson {
   int nev() const { return nev_; }
   const std::vector<double>& energies() const { return theta_h_; }
   const std::vector<double>& residuals() const { return rn_h_; }
-  const double* ritz_vectors() const { return X_.get(); }  // internal order, N x nev
+  const double* ritz_vectors() const { return X_.get(); }  // internal order, N x nev, leading dimension ld()
+  int64_t ld() const { return ld_; }
   pqg_dav_report report() const;
   uint64_t device_bytes() const;
 
@@ -62,11 +63,12 @@ class Davidson {
   void ts_gemm_nn2(hipStream_t st, const double* A0, const double* A1, int64_t lda, int ka, const double* Bs, int ldb,
                    int nb, double* out0, double* out1, int64_t ldo);
   double* Rp() const { return RT_.get(); }
-  double* Tp() const { return RT_.get() + (size_t)N_ * nev_; }
+  double* Tp() const { return RT_.get() + (size_t)ld_ * nev_; }
 
   pqg_ci* ci_;
   pqg_dav_params p_;
   int64_t N_;
+  int64_t ld_;  // leading dimension of every N-vector block (N_ rounded up to 32)
   int nev_, ninit_, nmax_;
   bool mixed_;
   DevBuf<double> V_, W_, G_, Y_, Yprev_, Ynext_, theta_, X_, AX_, RT_, tmp_, part_, small_, W64_;
