@@ -13,10 +13,11 @@
 // the host per iteration (convergence decision).
 //
 // Mixed precision (PQG_MIXED): sigma of new search directions runs in FP32.  When the
-// FP32-based residual has dropped below max(tol, 0.05 * last true residual), sigma of the
-// Ritz vectors is recomputed in FP64 ("true" residual).  Converged if that is below tol;
-// otherwise the basis restarts on the Ritz vectors with their exact FP64 sigma (defect
-// correction) and extends with corrections built from the true residuals.
+// FP32-based residual has dropped below max(tol, min(1e-3, 0.05 * last true residual)), or
+// stalls at the level where the FP32 error can hold it, sigma of the Ritz vectors is
+// recomputed in FP64 ("true" residual).  Converged if that is below tol; otherwise the basis
+// restarts on the Ritz vectors with their exact FP64 sigma (defect correction) and extends with
+// corrections built from the true residuals.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -169,6 +170,109 @@ __global__ void k_residual(const double* __restrict__ X, const double* __restric
     double den = theta[k] - d;
     if (fabs(den) < 1e-8) den = den < 0 ? -1e-8 : 1e-8;
     T[i + k * ld] = (d >= 1e29) ? 0.0 : r / den;
+  }
+}
+
+// Ritz vectors, residuals and corrections in one pass over the basis (nev <= NEV):
+//   X = V Y, AX = W Y (Y: the first nev eigenvectors of the projected matrix, in LDS),
+//   r = AX - X theta, t = r / (theta - diag) (safeguarded, zero on invalid entries),
+// writing X and T and summing |r_k|^2, |t_k|^2 per block (part[block][2 NEV], fixed order); AX
+// and R are never stored.  A thread takes two consecutive rows per step (16-byte accesses).
+template <int NEV>
+__global__ __launch_bounds__(kTB) void k_ritz_res(const double* __restrict__ V, const double* __restrict__ W,
+                                                  int64_t ld, int m, const double* __restrict__ Y,
+                                                  const double* __restrict__ theta, const double* __restrict__ Hd,
+                                                  int nev, double* __restrict__ X, double* __restrict__ T,
+                                                  int64_t nrows, double* __restrict__ part) {
+  __shared__ double ys[64 * NEV];
+  __shared__ double th[NEV];
+  __shared__ double red[kTB / 64][2 * NEV];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < m * NEV; i += kTB) {
+    const int c = i / NEV, k = i % NEV;
+    ys[i] = k < nev ? Y[c + k * 64] : 0.0;
+  }
+  if (tid < NEV) th[tid] = tid < nev ? theta[tid] : 0.0;
+  __syncthreads();
+  double ss[2 * NEV];
+#pragma unroll
+  for (int j = 0; j < 2 * NEV; j++) ss[j] = 0.0;
+  for (int64_t r = ((int64_t)blockIdx.x * kTB + tid) * 2; r < nrows; r += (int64_t)gridDim.x * kTB * 2) {
+    const bool full = r + 1 < nrows;
+    double2 x[NEV], ax[NEV];
+#pragma unroll
+    for (int k = 0; k < NEV; k++) {
+      x[k] = make_double2(0.0, 0.0);
+      ax[k] = make_double2(0.0, 0.0);
+    }
+#pragma unroll 4
+    for (int c = 0; c < m; c++) {
+      const double* pv = V + r + c * ld;
+      const double* pw = W + r + c * ld;
+      const double2 a = full ? *reinterpret_cast<const double2*>(pv) : make_double2(pv[0], 0.0);
+      const double2 b = full ? *reinterpret_cast<const double2*>(pw) : make_double2(pw[0], 0.0);
+#pragma unroll
+      for (int k = 0; k < NEV; k++) {
+        const double y = ys[c * NEV + k];
+        x[k].x += a.x * y;
+        x[k].y += a.y * y;
+        ax[k].x += b.x * y;
+        ax[k].y += b.y * y;
+      }
+    }
+    const double d0 = Hd[r], d1 = full ? Hd[r + 1] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NEV; k++) {
+      if (k >= nev) break;
+      const double r0 = ax[k].x - th[k] * x[k].x, r1 = full ? ax[k].y - th[k] * x[k].y : 0.0;
+      double e0 = th[k] - d0, e1 = th[k] - d1;
+      if (fabs(e0) < 1e-8) e0 = e0 < 0 ? -1e-8 : 1e-8;
+      if (fabs(e1) < 1e-8) e1 = e1 < 0 ? -1e-8 : 1e-8;
+      const double t0 = d0 >= 1e29 ? 0.0 : r0 / e0, t1 = (!full || d1 >= 1e29) ? 0.0 : r1 / e1;
+      ss[k] += r0 * r0 + r1 * r1;
+      ss[NEV + k] += t0 * t0 + t1 * t1;
+      double* px = X + r + k * ld;
+      double* pt = T + r + k * ld;
+      if (full) {
+        *reinterpret_cast<double2*>(px) = x[k];
+        *reinterpret_cast<double2*>(pt) = make_double2(t0, t1);
+      } else {
+        px[0] = x[k].x;
+        pt[0] = t0;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 2 * NEV; j++) {
+    double v = ss[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((tid & 63) == 0) red[tid >> 6][j] = v;
+  }
+  __syncthreads();
+  if (tid < 2 * NEV) {
+    double v = 0.0;
+    for (int w = 0; w < kTB / 64; w++) v += red[w][tid];
+    part[(int64_t)blockIdx.x * 2 * NEV + tid] = v;
+  }
+}
+
+// |r_k|^2 and |t_k|^2 summed over the blocks in a fixed order, into the diagonal of the [R T]
+// Gram (rows k and nev + k, leading dimension 64)
+__global__ __launch_bounds__(256) void k_ritz_res_reduce(const double* __restrict__ part, int nblk, int nev, int nevt,
+                                                         double* __restrict__ gram) {
+  const int j = blockIdx.x;  // 0 .. 2 nevt - 1
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 256) v += part[(int64_t)b * 2 * nevt + j];
+  __shared__ double red[4];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int k = j % nevt;
+    if (k < nev) {
+      const int row = j < nevt ? k : nev + k;
+      gram[row + row * 64] = (red[0] + red[1]) + (red[2] + red[3]);
+    }
   }
 }
 
@@ -727,6 +831,26 @@ void Davidson::restart_to_ritz(hipStream_t st, int keep) {
   restarts_++;
 }
 
+// X = V Y, T = (W Y - X theta) / (theta - diag) and the diagonal of the [R T] Gram in one pass
+// (nev <= 8)
+void Davidson::ritz_residual(hipStream_t st) {
+  ts_check(V_.get(), ld_);
+  ts_check(W_.get(), ld_);
+  const int64_t pairs = (N_ + 1) / 2;
+  const int nevt = nev_ <= 4 ? 4 : 8;
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(g1(pairs, kTB), (int64_t)part_.n / (2 * nevt)));
+  const int nblk2 = std::min(nblk, 8192);
+  if (nevt == 4)
+    k_ritz_res<4><<<nblk2, kTB, 0, st>>>(V_.get(), W_.get(), ld_, m_, Y_.get(), theta_.get(), operator_diag(ci_),
+                                         nev_, X_.get(), Tp(), N_, part_.get());
+  else
+    k_ritz_res<8><<<nblk2, kTB, 0, st>>>(V_.get(), W_.get(), ld_, m_, Y_.get(), theta_.get(), operator_diag(ci_),
+                                         nev_, X_.get(), Tp(), N_, part_.get());
+  PQG_HIP(hipGetLastError());
+  k_ritz_res_reduce<<<2 * nevt, 256, 0, st>>>(part_.get(), nblk2, nev_, nevt, rgram_.get());
+  PQG_HIP(hipGetLastError());
+}
+
 void Davidson::rayleigh_ritz(hipStream_t st) {
   // G[:, old:m] = V^T W[:, old:m]; the rows of the new columns against older columns follow by
   // symmetry.  Mixed precision: those couplings are taken as v_new^T (W v_old) instead, so that
@@ -839,17 +963,24 @@ void Davidson::post_sigma(hipStream_t st) {
   }
   iter_++;
   rayleigh_ritz(st);
-  // Ritz vectors and residuals
-  ts_gemm_nn2(st, V_.get(), W_.get(), ld_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), ld_);
-  k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_, ld_,
-                                          nev_);
-  PQG_HIP(hipGetLastError());
-  // one Gram of [R T]: R^T R (residual norms) and T^T T (drop thresholds of the corrections)
+  // Ritz vectors and residuals; what the next steps read of the Gram of [R T] is its diagonal:
+  // |r_k|^2 (residual norms) and |t_k|^2 (drop thresholds of the corrections)
   const double* g0 = nullptr;
-  if (2 * nev_ <= 64) {
+  if (nev_ <= 8) {
+    ritz_residual(st);
+    g0 = rgram_.get() + nev_ + nev_ * 64;
+  } else if (2 * nev_ <= 64) {
+    ts_gemm_nn2(st, V_.get(), W_.get(), ld_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), ld_);
+    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_,
+                                            ld_, nev_);
+    PQG_HIP(hipGetLastError());
     ts_gemm_tn(st, Rp(), ld_, 2 * nev_, Rp(), ld_, 2 * nev_, rgram_.get(), 64);
     g0 = rgram_.get() + nev_ + nev_ * 64;
   } else {
+    ts_gemm_nn2(st, V_.get(), W_.get(), ld_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), ld_);
+    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_,
+                                            ld_, nev_);
+    PQG_HIP(hipGetLastError());
     ts_gemm_tn(st, Rp(), ld_, nev_, Rp(), ld_, nev_, rgram_.get(), 64);
   }
   // The next block is prepared before the convergence test is read back, so an iteration costs
@@ -874,8 +1005,20 @@ void Davidson::post_sigma(hipStream_t st) {
   for (int k = 0; k < nev_; k++) theta_abs_max_ = std::max(theta_abs_max_, std::fabs(theta_h_[k]));
   dav_log(mixed_ ? "fp32" : "fp64", iter_, m_, p_.tol, theta_h_, rn_h_);
   since_check_++;
-  const double thr = std::max(p_.tol, std::min(1e-3, 0.05 * last_true_));
-  const bool approx_conv = mixed_ ? (worst < thr || since_check_ >= 12) : (worst < p_.tol);
+  // Mixed precision: the FP32-based residual carries the error of the FP32 sigma of the
+  // directions added since the last FP64 check, a few 1e-6 of |theta| at most, and that error
+  // shrinks with their weight in the Ritz vectors.  So an FP64 check is spent when the residual
+  // has dropped 20x below the last true one (first at 1e-3), or when it no longer improves
+  // (10% over 6 iterations) below 1e-4 |theta| -- the level where the FP32 error can stall it --
+  // and at the latest every 50 iterations.
+  if (mixed_) {
+    worst_hist_[since_check_ % 8] = worst;
+    const double thr = std::max(p_.tol, std::min(1e-3, 0.05 * last_true_));
+    const bool stalled = since_check_ > 6 && worst < 1e-4 * theta_abs_max_ &&
+                         worst > 0.9 * worst_hist_[(since_check_ - 6) % 8];
+    mixed_check_ = worst < thr || stalled || since_check_ >= 50;
+  }
+  const bool approx_conv = mixed_ ? mixed_check_ : (worst < p_.tol);
   if (approx_conv && mixed_) {
     // FP64 check of the Ritz vectors (the speculative block is dropped: m_ is unchanged)
     pending_nt_ = 0;

@@ -56,6 +56,7 @@ class Davidson {
   void read_status(hipStream_t st, const double* rgram);
   void restart_to_ritz(hipStream_t st, int keep);
   void rayleigh_ritz(hipStream_t st);
+  void ritz_residual(hipStream_t st);
   void ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, const double* B, int64_t ldb, int nb,
                   double* out, int ldo);
   void ts_gemm_nn(hipStream_t st, const double* A, int64_t lda, int ka, const double* Bs, int ldb, int nb,
@@ -80,7 +81,8 @@ class Davidson {
   DevBuf<float> c32_, s32_;
   int m_ = 0, nW_ = 0, iter_ = 0, restarts_ = 0, since_check_ = 0, warm_ = 0;
   int sigma_calls_ = 0, sigma_calls64_ = 0;
-  bool done_ = false, checking_ = false;
+  bool done_ = false, checking_ = false, mixed_check_ = false;
+  double worst_hist_[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // FP32-based worst residuals since the last check
   pqg_status status_ = PQG_OK;
   double last_true_ = 1e300;
   double last_worst_ = 1e300, theta_abs_max_ = 0.0;  // from the last residual check
