@@ -272,8 +272,13 @@ static int64_t lg_rank(uint64_t s, const std::vector<std::vector<uint64_t>>& bn)
 
 // U scratch per chunk: about eight vectors' worth (a chunk re-stages each row of C once),
 // at least 1 GiB (one chunk for small spaces), at most 16 GiB
+// Bytes of the link-GEMM output block U (the columns are processed in chunks that fit it).
+// Fewer, wider chunks amortise each workgroup's set-up (staged row, A): at 16 orbitals 8a8b a
+// 4-vector sigma took 1022 / 910 / 860 ms (FP64) and 327 / 320 / 295 ms (BF16x3) with 5, 10.6
+// and 24 GB.  PQG_LG_UBUDGET_GB overrides the size.
 size_t lg_budget_bytes(int64_t N, size_t w) {
-  return std::min<size_t>(16ull << 30, std::max<size_t>(1ull << 30, 8ull * (size_t)N * w));
+  if (const char* e = std::getenv("PQG_LG_UBUDGET_GB")) return (size_t)(std::atof(e) * (1ull << 30));
+  return std::min<size_t>(24ull << 30, std::max<size_t>(1ull << 30, 32ull * (size_t)N * w));
 }
 
 // K steps per wave group: a compile-time bound so the MFMA loop has no branches.  With the

@@ -305,12 +305,19 @@ __global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
   const int uoffb = ((4 * (lane >> 5)) * a.ldu + (lane & 31)) * 4;
   __bf16* Tg = Bt + 2 * grp * plane;  // this group's buffer
   float* P = Pt + grp * (KS - 1) * MB * 1024;
-  // Tile i belongs to group i % 2.  Phase p: the owner of tile p builds it (and stores the output
+  // Tile i belongs to group i % 2.  Phase p: the owner of tile p builds it (then stores the output
   // of its tile p - 2); the other group runs the MFMAs of tile p - 1.  One barrier per phase.
   if (grp < ntiles) prefetch(gbeg + (int64_t)NT * grp);
   lg_f16v acc = {};
   for (int p = 0; p <= ntiles + 1; p++) {
     if ((p & 1) == grp) {
+      // the tile build first, then the stores of tile p - 2: the loads a build waits for were
+      // issued in this group's previous build phase, before that phase's stores, so the wait
+      // never covers stores issued just before it
+      if (p < ntiles) {
+        store(gbeg + (int64_t)NT * p, Tg);
+        if (p + 2 < ntiles) prefetch(gbeg + (int64_t)NT * (p + 2));
+      }
       if (p >= 2 && active && kh == 0) {
         const int64_t m0 = gbeg + (int64_t)NT * (p - 2);
         const int cofs = (int)(m0 - a.c0) * 4;
@@ -338,10 +345,6 @@ __global__ __launch_bounds__(512, 1) void k_lg_b3(const LgArgs<float> a) {
                                                     ro * a.ldu * 4 + cofs, 0);
           }
         }
-      }
-      if (p < ntiles) {
-        store(gbeg + (int64_t)NT * p, Tg);
-        if (p + 2 < ntiles) prefetch(gbeg + (int64_t)NT * (p + 2));
       }
     } else if (p >= 1 && p <= ntiles) {
       acc = lg_f16v{};
