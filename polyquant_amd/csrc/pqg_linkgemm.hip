@@ -566,7 +566,12 @@ static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w, int nt
   const size_t want = std::min<size_t>(lg_budget_bytes(ci->n_int, w), per_col * (size_t)n_c);
   const size_t budget = std::max<size_t>(per_col * nt, want);
   int cw = (int)std::min<size_t>((size_t)n_c, budget / per_col);
-  if (cw < n_c) cw = std::max(nt, cw / nt * nt);
+  if (cw < n_c) {
+    // as many chunks as the budget needs, of balanced width (no narrow last chunk)
+    cw = std::max(nt, cw / nt * nt);
+    const int nch = (n_c + cw - 1) / cw;
+    cw = std::min(cw, ((n_c + nch - 1) / nch + nt - 1) / nt * nt);
+  }
   return cw;
 }
 
