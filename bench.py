@@ -415,7 +415,8 @@ def _serve_loop(args, reqs, ctx, nvec):
         for r in srv.poll():
             got[ids[r["id"]]] = r
             log(f"[serve] served {len(got)}/{n}: request {ids[r['id']]} status {r['status']}, "
-                f"latency {r['t_done'] - r['t_submit']:.3f}s ({time.perf_counter() - t_start:.1f}s)")
+                f"latency {r['t_done'] - r['t_submit']:.3f}s ({time.perf_counter() - t_start:.1f}s)"
+                + (f" {r['message']}" if r["status"] else ""))
         time.sleep(0.0005)
     makespan = time.perf_counter() - t_start
     srv.close()

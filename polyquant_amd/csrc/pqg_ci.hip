@@ -1226,17 +1226,17 @@ static void build_slot(pqg_ci* ci, Slot& sl, int n, int nocc, int max_level) {
       }
     }
   sl.rtab.reset(ctx, tab.size());
-  PQG_HIP(hipMemcpy(sl.rtab.get(), tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+  upload(ci->stream, sl.rtab.get(), tab.data(), tab.size() * 4);
   sl.bits.reset(ctx, sl.nstr);
-  PQG_HIP(hipMemcpy(sl.bits.get(), sl.sorted.data(), sl.nstr * 8, hipMemcpyHostToDevice));
+  upload(ci->stream, sl.bits.get(), sl.sorted.data(), sl.nstr * 8);
   if (!sl.complete) {
     std::vector<int32_t> r2i(total, -1);
     for (int i = 0; i < sl.nstr; i++) r2i[colex_rank(sl.sorted[i])] = i;
     sl.r2i.reset(ctx, total);
-    PQG_HIP(hipMemcpy(sl.r2i.get(), r2i.data(), total * 4, hipMemcpyHostToDevice));
+    upload(ci->stream, sl.r2i.get(), r2i.data(), total * 4);
   }
   sl.d_ref2int.reset(ctx, sl.nstr);
-  PQG_HIP(hipMemcpy(sl.d_ref2int.get(), sl.ref2int.data(), sl.nstr * 4, hipMemcpyHostToDevice));
+  upload(ci->stream, sl.d_ref2int.get(), sl.ref2int.data(), sl.nstr * 4);
   sl.nsingle = (int64_t)sl.nocc * sl.nvirt;
   sl.ndouble = (int64_t)(sl.nocc * (sl.nocc - 1) / 2) * (sl.nvirt * (sl.nvirt - 1) / 2);
 }
@@ -1332,7 +1332,7 @@ static void upload_blocks(pqg_ci* ci, const pqg_ci_desc* d) {
         for (int Q = 0; Q < np_u; Q++)
           h[(size_t)P * np_u + Q] = f * (transpose ? src[(size_t)Q + (size_t)P * na] : src[(size_t)P + (size_t)Q * na]);
       ci->V64[t][u].reset(ctx, h.size());
-      PQG_HIP(hipMemcpy(ci->V64[t][u].get(), h.data(), h.size() * 8, hipMemcpyHostToDevice));
+      upload(ci->stream, ci->V64[t][u].get(), h.data(), h.size() * 8);
     }
 }
 
@@ -1454,9 +1454,9 @@ pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* d, hipStream_t stream) {
       }
     ci->n_ref = (int64_t)addr.size();
     ci->ref_addr.reset(ctx, addr.size());
-    PQG_HIP(hipMemcpy(ci->ref_addr.get(), addr.data(), addr.size() * 8, hipMemcpyHostToDevice));
+    upload(ci->stream, ci->ref_addr.get(), addr.data(), addr.size() * 8);
     ci->valid.reset(ctx, valid.size());
-    PQG_HIP(hipMemcpy(ci->valid.get(), valid.data(), valid.size(), hipMemcpyHostToDevice));
+    upload(ci->stream, ci->valid.get(), valid.data(), valid.size());
   }
 
   // ---- integrals and per-string tables ---------------------------------------------------
@@ -1471,7 +1471,7 @@ pqg_ci* create_ci(pqg_ctx* ctx, const pqg_ci_desc* d, hipStream_t stream) {
     for (int i = 0; i < n; i++)
       for (int j = 0; j < n; j++) hr[(size_t)i * n + j] = hs[i + (size_t)j * n];
     hdev[t].reset(ctx, hr.size());
-    PQG_HIP(hipMemcpy(hdev[t].get(), hr.data(), hr.size() * 8, hipMemcpyHostToDevice));
+    upload(ci->stream, hdev[t].get(), hr.data(), hr.size() * 8);
   }
   hipStream_t st = ci->stream;
   for (int t = 0; t < ci->nslot; t++) {
@@ -2269,7 +2269,8 @@ pqg_status pqg_ci_dump_strings(pqg_ci* ci, int part, int spin, uint64_t* out) {
     PQG_REQUIRE(ci && out && part >= 0 && part < ci->nsp && (spin == 0 || spin == 1), PQG_ERR_ARG, "invalid argument");
     const Slot& sl = ci->slot[2 * part + spin];
     std::vector<uint64_t> dev(sl.nstr);
-    PQG_HIP(hipMemcpy(dev.data(), sl.bits.get(), sl.nstr * 8, hipMemcpyDeviceToHost));
+    PQG_HIP(hipMemcpyAsync(dev.data(), sl.bits.get(), sl.nstr * 8, hipMemcpyDeviceToHost, ci->stream));
+    PQG_HIP(hipStreamSynchronize(ci->stream));
     for (int r = 0; r < sl.nstr; r++) out[r] = dev[sl.ref2int[r]];
   });
 }

@@ -137,6 +137,16 @@ struct pqg_ci {
   // diagonal of the stored operator (stored H plus the penalties): the Davidson's
   // preconditioner and start selection use it while the explicit H is stored
   pqg::DevBuf<double> xdiag;
+
+  pqg_ci() = default;
+  pqg_ci(const pqg_ci&) = delete;
+  pqg_ci& operator=(const pqg_ci&) = delete;
+  // Work queued on the handle's stream finishes before any of its buffers go back to the arena
+  // (which may hand them to another stream, or return a chunk to the device), also when a
+  // setup or a solve is abandoned on an error.
+  ~pqg_ci() {
+    if (stream) (void)hipStreamSynchronize(stream);
+  }
 };
 
 namespace pqg {

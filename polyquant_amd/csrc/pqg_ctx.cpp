@@ -15,6 +15,11 @@ namespace pqg {
 static thread_local std::string t_err;
 void set_error(const std::string& msg) { t_err = msg; }
 
+std::recursive_mutex& device_op_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 static thread_local FixedScratch* t_fixed = nullptr;
 FixedScratch*& fixed_scratch() { return t_fixed; }
 
@@ -36,6 +41,7 @@ static constexpr uint64_t kDedicated = 1ull << 30;
 static uint64_t round_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 Arena::~Arena() {
+  std::lock_guard<std::recursive_mutex> dl(device_op_mutex());
   for (auto& c : chunks_)
     if (c.base) (void)hipFree(c.base);
 }
@@ -65,6 +71,7 @@ bool Arena::release_empty_locked() {
 }
 
 void* Arena::alloc(uint64_t bytes) {
+  std::lock_guard<std::recursive_mutex> dl(device_op_mutex());  // always before mu_
   std::lock_guard<std::mutex> lk(mu_);
   bytes = round_up(std::max<uint64_t>(bytes, 1), kAlign);
   if (bytes < kDedicated) {
@@ -131,6 +138,7 @@ void* Arena::alloc(uint64_t bytes) {
 
 void Arena::free(void* vp) {
   if (!vp) return;
+  std::lock_guard<std::recursive_mutex> dl(device_op_mutex());  // always before mu_
   std::lock_guard<std::mutex> lk(mu_);
   char* p = static_cast<char*>(vp);
   auto it = live_.find(p);

@@ -594,6 +594,18 @@ static void dav_log(const char* tag, int iter, int m, double tol, const std::vec
 }
 
 Davidson::Davidson(pqg_ci* ci, const pqg_dav_params& p) : ci_(ci), p_(p) {
+  try {
+    setup();
+  } catch (...) {
+    // members built so far are released by the unwinding: let queued work finish first
+    (void)hipStreamSynchronize(ci_->stream);
+    throw;
+  }
+}
+
+void Davidson::setup() {
+  pqg_ci* ci = ci_;
+  const pqg_dav_params& p = p_;
   N_ = ci->n_int;
   ld_ = (N_ + 31) / 32 * 32;  // every column 256-byte aligned
   nev_ = std::max(1, p.nev);
@@ -622,7 +634,10 @@ Davidson::Davidson(pqg_ci* ci, const pqg_dav_params& p) : ci_(ci), p_(p) {
   mbuf_.reset(ctx, 64 * 64);
   nk_dev_.reset(ctx, 2);
   status_dev_.reset(ctx, 130);
-  PQG_HIP(hipHostMalloc((void**)&status_pin_.p, 130 * 8, hipHostMallocDefault));
+  {
+    std::lock_guard<std::recursive_mutex> dl(device_op_mutex());
+    PQG_HIP(hipHostMalloc((void**)&status_pin_.p, 130 * 8, hipHostMallocDefault));
+  }
   status_h_ = status_pin_.p;
   // partials of the tall-skinny products: at most ceil(N / 512) row chunks
   nchunk_ = (int)((N_ + 511) / 512);
@@ -735,12 +750,10 @@ void Davidson::initial_guess(hipStream_t st) {
   work_ = SigmaWork{m_, mixed_ ? PQG_MIXED : PQG_FP64, V_.get(), W_.get(), ld_};
 }
 
-void Davidson::run_sigma(hipStream_t st) {
-  if (work_.k == 0) return;
+void Davidson::sigma_body(hipStream_t st) {
   const int k = work_.k;
   if (work_.precision == PQG_FP64) {
     sigma_internal(ci_, work_.C, work_.S, k, work_.ld, st);
-    sigma_calls64_ += k;
   } else {
     const int64_t n = (int64_t)N_ * k;
     grow(c32_, ci_->ctx, n, st);
@@ -752,6 +765,90 @@ void Davidson::run_sigma(hipStream_t st) {
       k_f2d_blk<<<g1(N_, 256), 256, 0, st>>>(s32_.get() + (int64_t)j * N_, work_.S + j * work_.ld, N_);
     PQG_HIP(hipGetLastError());
   }
+}
+
+// Device buffers a sigma may grow (and so move): a captured sigma is valid while they stay put.
+uint64_t Davidson::sigma_buffers() const {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+  mix((uintptr_t)ci_->scratch.get());
+  mix(ci_->scratch.n);
+  mix((uintptr_t)ci_->lg.U.get());
+  mix(ci_->lg.U.n);
+  mix((uintptr_t)ci_->lg.P.get());
+  mix(ci_->lg.P.n);
+  mix((uintptr_t)c32_.get());
+  mix(c32_.n);
+  mix((uintptr_t)s32_.get());
+  mix(s32_.n);
+  return h;
+}
+
+// Captures sigma_body into a graph (nothing is executed); on any failure the capture is ended,
+// errors are cleared, graphs are switched off for the solve and nullptr comes back.  The capture
+// holds device_op_mutex, so no allocation or free of this library runs meanwhile.
+hipGraphExec_t Davidson::capture_sigma(hipStream_t st) {
+  std::lock_guard<std::recursive_mutex> dl(device_op_mutex());
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess;
+  if (ok) {
+    try {
+      sigma_body(st);
+    } catch (...) {
+      ok = false;
+    }
+    ok = (hipStreamEndCapture(st, &g) == hipSuccess) && ok && g;
+    if (ok) ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    if (ex) (void)hipGraphExecDestroy(ex);
+    ex = nullptr;
+    graphs_.on = false;
+  }
+  return ex;
+}
+
+SigmaGraphs::~SigmaGraphs() {
+  for (auto& e : entries)
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+}
+
+// With graphs on, a sigma block seen before (same vectors, count, precision and buffers) is
+// captured into a hipGraph on its second run and replayed from then on: one launch instead of
+// the few tens of kernels of a small space's sigma.  The first run of a block is a plain one,
+// so every buffer the sigma grows is in place before anything is captured.
+void Davidson::run_sigma(hipStream_t st) {
+  if (work_.k == 0) return;
+  const int k = work_.k;
+  if (graphs_.on && !ci_->ctx->profiling.load()) {
+    SigmaGraphs::Entry* e = nullptr;
+    for (auto& x : graphs_.entries)
+      if (x.C == work_.C && x.S == work_.S && x.k == k && x.precision == work_.precision && x.ld == work_.ld) e = &x;
+    const uint64_t bufs = sigma_buffers();
+    if (e && e->bufs == bufs && !e->exec) e->exec = capture_sigma(st);
+    if (e && e->bufs == bufs && e->exec) {
+      PQG_HIP(hipGraphLaunch(e->exec, st));
+    } else if (e && e->bufs == bufs) {
+      sigma_body(st);  // the capture failed: graphs are off for this solve from now on
+    } else {
+      sigma_body(st);
+      // the block as run (after any growth) becomes capturable on its next run
+      if (!e) {
+        graphs_.entries.push_back(SigmaGraphs::Entry{work_.C, work_.S, k, work_.precision, work_.ld, 0, nullptr});
+        e = &graphs_.entries.back();
+      } else if (e->exec) {
+        (void)hipGraphExecDestroy(e->exec);
+        e->exec = nullptr;
+      }
+      e->bufs = sigma_buffers();
+    }
+  } else {
+    sigma_body(st);
+  }
+  if (work_.precision == PQG_FP64) sigma_calls64_ += k;
   sigma_calls_ += k;
 }
 
@@ -802,11 +899,10 @@ int Davidson::finish_append() {
 }
 
 // Reads residual norms, Ritz values and the appended count in one transfer (one sync).
-void Davidson::read_status(hipStream_t st, const double* rgram) {
+void Davidson::read_status_async(hipStream_t st, const double* rgram) {
   k_pack_status<<<1, 64, 0, st>>>(rgram, theta_.get(), nk_dev_.get(), nev_, status_dev_.get());
   PQG_HIP(hipGetLastError());
   PQG_HIP(hipMemcpyAsync(status_h_, status_dev_.get(), 130 * 8, hipMemcpyDeviceToHost, st));
-  PQG_HIP(hipStreamSynchronize(st));
 }
 
 int Davidson::orthonormalize_append(hipStream_t st, double* T, int nt) {
@@ -882,6 +978,56 @@ void Davidson::rayleigh_ritz(hipStream_t st) {
 }
 
 void Davidson::post_sigma(hipStream_t st) {
+  post_sigma_begin(st);
+  post_sigma_end(st);
+}
+
+// The device half of an iteration, up to the asynchronous copy of the status block: nothing here
+// waits for the device, so many solves can queue their iterations back to back before one
+// synchronisation (post_sigma_end completes each of them).
+void Davidson::post_sigma_begin(hipStream_t st) {
+  began_ = false;
+  if (done_ || checking_) return;
+  iter_++;
+  rayleigh_ritz(st);
+  // Ritz vectors and residuals; what the next steps read of the Gram of [R T] is its diagonal:
+  // |r_k|^2 (residual norms) and |t_k|^2 (drop thresholds of the corrections)
+  const double* g0 = nullptr;
+  if (nev_ <= 8) {
+    ritz_residual(st);
+    g0 = rgram_.get() + nev_ + nev_ * 64;
+  } else if (2 * nev_ <= 64) {
+    ts_gemm_nn2(st, V_.get(), W_.get(), ld_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), ld_);
+    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_,
+                                            ld_, nev_);
+    PQG_HIP(hipGetLastError());
+    ts_gemm_tn(st, Rp(), ld_, 2 * nev_, Rp(), ld_, 2 * nev_, rgram_.get(), 64);
+    g0 = rgram_.get() + nev_ + nev_ * 64;
+  } else {
+    ts_gemm_nn2(st, V_.get(), W_.get(), ld_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), ld_);
+    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_,
+                                            ld_, nev_);
+    PQG_HIP(hipGetLastError());
+    ts_gemm_tn(st, Rp(), ld_, nev_, Rp(), ld_, nev_, rgram_.get(), 64);
+  }
+  // The next block is prepared before the convergence test is read back, so an iteration costs
+  // one host synchronisation.  A mixed-precision solve may still need the current subspace for
+  // its FP64 check, so it does not restart ahead of the test.
+  need_restart_ = m_ + nev_ > nmax_;
+  speculate_ = !(mixed_ && need_restart_);
+  if (speculate_) {
+    if (need_restart_) restart_to_ritz(st, ninit_);
+    // with the perturbed start every eigenvector has weight in the basis, so converged roots
+    // need not be expanded; with unit vectors their corrections can still bring in missing roots
+    append_async(st, Tp(), nev_, g0, (p_.guess_perturb > 0.0 && 2 * nev_ <= 64) ? rgram_.get() : nullptr);
+  }
+  read_status_async(st, rgram_.get());
+  began_ = true;
+}
+
+// The host half: waits for the status block of post_sigma_begin (or runs an FP64 check of the
+// mixed path), decides convergence and queues the next sigma block.
+void Davidson::post_sigma_end(hipStream_t st) {
   if (done_) return;
   if (checking_) {
     // FP64 sigma of the Ritz vectors is in W64_: true residuals
@@ -961,40 +1107,10 @@ void Davidson::post_sigma(hipStream_t st) {
     }
     return;
   }
-  iter_++;
-  rayleigh_ritz(st);
-  // Ritz vectors and residuals; what the next steps read of the Gram of [R T] is its diagonal:
-  // |r_k|^2 (residual norms) and |t_k|^2 (drop thresholds of the corrections)
-  const double* g0 = nullptr;
-  if (nev_ <= 8) {
-    ritz_residual(st);
-    g0 = rgram_.get() + nev_ + nev_ * 64;
-  } else if (2 * nev_ <= 64) {
-    ts_gemm_nn2(st, V_.get(), W_.get(), ld_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), ld_);
-    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_,
-                                            ld_, nev_);
-    PQG_HIP(hipGetLastError());
-    ts_gemm_tn(st, Rp(), ld_, 2 * nev_, Rp(), ld_, 2 * nev_, rgram_.get(), 64);
-    g0 = rgram_.get() + nev_ + nev_ * 64;
-  } else {
-    ts_gemm_nn2(st, V_.get(), W_.get(), ld_, m_, Y_.get(), 64, nev_, X_.get(), AX_.get(), ld_);
-    k_residual<<<g1(N_, 256), 256, 0, st>>>(X_.get(), AX_.get(), theta_.get(), operator_diag(ci_), Rp(), Tp(), N_,
-                                            ld_, nev_);
-    PQG_HIP(hipGetLastError());
-    ts_gemm_tn(st, Rp(), ld_, nev_, Rp(), ld_, nev_, rgram_.get(), 64);
-  }
-  // The next block is prepared before the convergence test is read back, so an iteration costs
-  // one host synchronisation.  A mixed-precision solve may still need the current subspace for
-  // its FP64 check, so it does not restart ahead of the test.
-  const bool need_restart = m_ + nev_ > nmax_;
-  const bool speculate = !(mixed_ && need_restart);
-  if (speculate) {
-    if (need_restart) restart_to_ritz(st, ninit_);
-    // with the perturbed start every eigenvector has weight in the basis, so converged roots
-    // need not be expanded; with unit vectors their corrections can still bring in missing roots
-    append_async(st, Tp(), nev_, g0, (p_.guess_perturb > 0.0 && 2 * nev_ <= 64) ? rgram_.get() : nullptr);
-  }
-  read_status(st, rgram_.get());
+  if (!began_) return;
+  began_ = false;
+  PQG_HIP(hipStreamSynchronize(st));
+  const bool need_restart = need_restart_, speculate = speculate_;
   theta_h_.assign(status_h_ + 64, status_h_ + 64 + nev_);
   rn_h_.assign(status_h_, status_h_ + nev_);
   double worst = 0.0;

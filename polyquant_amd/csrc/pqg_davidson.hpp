@@ -15,7 +15,9 @@ struct PinnedHost {
   PinnedHost(const PinnedHost&) = delete;
   PinnedHost& operator=(const PinnedHost&) = delete;
   ~PinnedHost() {
-    if (p) (void)hipHostFree(p);
+    if (!p) return;
+    std::lock_guard<std::recursive_mutex> dl(device_op_mutex());
+    (void)hipHostFree(p);
   }
 };
 
@@ -28,13 +30,41 @@ struct SigmaWork {
   int64_t ld = 0;
 };
 
+// Captured sigma launches of one solve, keyed by the block they apply to.
+struct SigmaGraphs {
+  struct Entry {
+    const double* C;
+    double* S;
+    int k, precision;
+    int64_t ld;
+    uint64_t bufs;          // the sigma buffers the capture (or the first run) used
+    hipGraphExec_t exec;    // nullptr until captured
+  };
+  bool on = false;
+  std::vector<Entry> entries;
+  SigmaGraphs() = default;
+  SigmaGraphs(const SigmaGraphs&) = delete;
+  SigmaGraphs& operator=(const SigmaGraphs&) = delete;
+  ~SigmaGraphs();
+};
+
 class Davidson {
  public:
   Davidson(pqg_ci* ci, const pqg_dav_params& p);
+  Davidson(const Davidson&) = delete;
+  Davidson& operator=(const Davidson&) = delete;
+  // the solve's queued work finishes before its buffers go back to the arena
+  ~Davidson() { (void)hipStreamSynchronize(ci_->stream); }
   // Work the solver needs before post_sigma() can run (k == 0 when finished).
   SigmaWork pending() const { return work_; }
   // Runs the rest of one iteration after the pending sigma has been written.
   void post_sigma(hipStream_t st);
+  // post_sigma in two halves: begin queues the device work of the iteration (no wait), end
+  // synchronises the stream and takes the host decisions
+  void post_sigma_begin(hipStream_t st);
+  void post_sigma_end(hipStream_t st);
+  // replay the sigma of a block seen before from a captured hipGraph (small spaces)
+  void use_graphs(bool on) { graphs_.on = on; }
   // Run the pending sigma itself (FP32 conversion included for mixed precision).
   void run_sigma(hipStream_t st);
   bool done() const { return done_; }
@@ -48,12 +78,16 @@ class Davidson {
   uint64_t device_bytes() const;
 
  private:
+  void setup();
   void initial_guess(hipStream_t st);
+  hipGraphExec_t capture_sigma(hipStream_t st);
   int orthonormalize_append(hipStream_t st, double* T, int nt);
   // rg: residual Gram of the roots the columns correct; columns of converged roots are dropped
   void append_async(hipStream_t st, double* T, int nt, const double* g0, const double* rg = nullptr);
   int finish_append();
-  void read_status(hipStream_t st, const double* rgram);
+  void read_status_async(hipStream_t st, const double* rgram);
+  void sigma_body(hipStream_t st);
+  uint64_t sigma_buffers() const;
   void restart_to_ritz(hipStream_t st, int keep);
   void rayleigh_ritz(hipStream_t st);
   void ritz_residual(hipStream_t st);
@@ -82,6 +116,8 @@ class Davidson {
   int m_ = 0, nW_ = 0, iter_ = 0, restarts_ = 0, since_check_ = 0, warm_ = 0;
   int sigma_calls_ = 0, sigma_calls64_ = 0;
   bool done_ = false, checking_ = false, mixed_check_ = false;
+  bool began_ = false, need_restart_ = false, speculate_ = false;  // state between the halves
+  SigmaGraphs graphs_;
   double worst_hist_[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // FP32-based worst residuals since the last check
   pqg_status status_ = PQG_OK;
   double last_true_ = 1e300;

@@ -37,6 +37,14 @@ struct Error {
     if (!(cond)) throw ::pqg::Error{(code), (msg)};      \
   } while (0)
 
+// Host -> device copy ordered on `st` and complete on return.  The library keeps every copy on
+// its own non-blocking streams: a blocking copy on the legacy stream would depend on (and so
+// break) a stream capture running in another thread.
+inline void upload(hipStream_t st, void* dst, const void* src, size_t bytes) {
+  PQG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+  PQG_HIP(hipStreamSynchronize(st));
+}
+
 // Runs f, converting exceptions into a status + thread-local message.
 template <class F>
 pqg_status guard(F&& f) {
@@ -58,6 +66,11 @@ pqg_status guard(F&& f) {
 // ---------------------------------------------------------------------------------------
 // device memory manager: first-fit arena inside an HBM budget
 // ---------------------------------------------------------------------------------------
+// Serialises the HIP calls that may synchronise the device implicitly (hipMalloc, hipFree,
+// hipHostMalloc, hipHostFree) with stream captures: a capture in one thread is invalidated by
+// such a call in another.  Recursive: a capture may reach the arena.
+std::recursive_mutex& device_op_mutex();
+
 class Arena {
  public:
   Arena(int device, uint64_t budget) : device_(device), budget_(budget) {}

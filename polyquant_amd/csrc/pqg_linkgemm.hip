@@ -417,7 +417,7 @@ void link_gemm_setup(pqg_ci* ci) {
           }
       }
       lt.links.reset(ci->ctx, links.size());
-      PQG_HIP(hipMemcpy(lt.links.get(), links.data(), links.size() * sizeof(int2), hipMemcpyHostToDevice));
+      upload(ci->stream, lt.links.get(), links.data(), links.size() * sizeof(int2));
     }
     if (t < 2) {
       // column-set map: e-beta (pass A) with its off-diagonal pairs, e-alpha (pass B) without;
@@ -445,7 +445,7 @@ void link_gemm_setup(pqg_ci* ci) {
         map[(size_t)(nin - 1) * s.nstr + m] = m << 1;
       }
       lt.tbm.reset(ci->ctx, map.size());
-      PQG_HIP(hipMemcpy(lt.tbm.get(), map.data(), map.size() * 4, hipMemcpyHostToDevice));
+      upload(ci->stream, lt.tbm.get(), map.data(), map.size() * 4);
       if (nbo > 0) {
         // rows padded to a multiple of 4 pairs, and one tile of -1 rows past the last string
         // words: byte offset of the target pair in the kernel's staged row | odd << 31, and for
@@ -459,7 +459,7 @@ void link_gemm_setup(pqg_ci* ci) {
             if (x >= 0) mt[(size_t)m * nbo4 + kb] = ((uint32_t)(x >> 1) * 4u) | ((uint32_t)(x & 1) << 31);
           }
         lt.tbmT.reset(ci->ctx, mt.size());
-        PQG_HIP(hipMemcpy(lt.tbmT.get(), mt.data(), mt.size() * 4, hipMemcpyHostToDevice));
+        upload(ci->stream, lt.tbmT.get(), mt.data(), mt.size() * 4);
       }
     }
   }

@@ -321,9 +321,9 @@ static bool build_groups(pqg_ci* ci, Slot& sl) {
   }
   sl.ngroup = (int64_t)kb.size();
   sl.gbits.reset(ci->ctx, kb.size());
-  PQG_HIP(hipMemcpy(sl.gbits.get(), kb.data(), kb.size() * 8, hipMemcpyHostToDevice));
+  upload(ci->stream, sl.gbits.get(), kb.data(), kb.size() * 8);
   sl.gmem.reset(ci->ctx, mem.size());
-  PQG_HIP(hipMemcpy(sl.gmem.get(), mem.data(), mem.size() * 4, hipMemcpyHostToDevice));
+  upload(ci->stream, sl.gmem.get(), mem.data(), mem.size() * 4);
   sl.gm = m;
   return true;
 }

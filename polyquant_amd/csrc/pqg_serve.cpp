@@ -64,6 +64,7 @@ struct Request {
 
 // Requests waiting longer than this are admitted oldest first; below it, smallest work first.
 constexpr double kMaxWaitS = 30.0;
+constexpr int64_t kGraphMaxDet = 131072;  // determinants up to which a solve replays captured sigmas
 
 }  // namespace
 
@@ -153,6 +154,8 @@ struct pqg_server {
       pqg_status status = guard([&] {
         std::unique_ptr<pqg_ci> ci(create_ci(ctx, &r->desc.d, st));
         Davidson dav(ci.get(), r->params);
+        // small spaces: the sigma is a few tens of short kernels, replayed from a captured graph
+        if (ci->n_int <= kGraphMaxDet && !ci->xh) dav.use_graphs(true);
         while (!dav.done()) {
           dav.run_sigma(st);
           dav.post_sigma(st);
