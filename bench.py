@@ -772,8 +772,8 @@ def main():
     ap.add_argument("--serve-save", default=None, help="write the leg's per-request results (JSON)")
     ap.add_argument("--serve-combine", nargs=2, default=None, metavar=("SEQ", "SRV"),
                     help="print the serving report from saved legs (SEQ: comma-separated sequential parts)")
-    ap.add_argument("--dav-nvec", type=int, default=16, help="num_subspace_vec of the default workload")
-    ap.add_argument("--dav-nmax", type=int, default=64, help="largest search space of the default workload")
+    ap.add_argument("--dav-nvec", type=int, default=16, help="num_subspace_vec of the 48+48 workload")
+    ap.add_argument("--dav-nmax", type=int, default=40, help="largest search space of the 48+48 workload")
     ap.add_argument("--fci-nvec", type=int, default=16)
     ap.add_argument("--fci-nmax", type=int, default=48)
     ap.add_argument("--fci-maxit", type=int, default=400)
@@ -844,8 +844,11 @@ def bench_ep48(args):
     nev = prob.nroots
     log(f"[bench] workload ep_fci_48e_48p: N={N} roots={nev} setup {setup_s:.3f}s")
 
-    # num_subspace_vec = 16 and a 64-vector search space (the reference's knobs,
-    # epci.cpp:589-608; its defaults 5 / max(10, 10*nstates) need ~2.5x more iterations here)
+    # num_subspace_vec = 16 and a 40-vector search space (the reference's knobs,
+    # epci.cpp:589-608; its defaults 5 / max(10, 10*nstates) need ~2.5x more iterations here).
+    # Measured on one MI355X, whole FP64 solves: 84.4 ms at 16/40, 92.0 ms at 16/64, 95.4 ms at
+    # 16/32 (the reference's max(2*16, 10*3)), 91.2 ms at 8/32 -- fewer iterations with a larger
+    # space stop paying once the subspace eigensolve grows with m^3.
     nvec_init, nvec_max = args.dav_nvec, args.dav_nmax
 
     def solve(precision):

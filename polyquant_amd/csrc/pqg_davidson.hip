@@ -310,31 +310,6 @@ __global__ void k_scale_cols(double* __restrict__ v, int64_t n, int64_t ld, cons
   for (int j = 0; j < k; j++) v[i + j * ld] *= s[j];
 }
 
-// Parallel-order (round-robin) cyclic Jacobi for a symmetric m x m matrix, m <= 64, in one
-// workgroup, warm-started: with P = blockdiag(Yp (warm x warm), I), it diagonalises P^T G P
-// and accumulates the rotations onto P, so a projected matrix that grew by a few rows since the
-// previous iteration needs fewer sweeps.  In each step the indices form m/2 disjoint pairs
-// (circle method); thread (bi, bj) owns the 2x2 block of A at the rows of pair bi and the
-// columns of pair bj, and two rows of V at the columns of pair bj, so the rotation of a step
-// is applied in place with no exchange between threads.  Outputs ascending eigenvalues and the
-// matching eigenvectors (columns of Y).
-__device__ __forceinline__ void rr_pair(int mm, int step, int k, int& p, int& q) {
-  // circle method: index mm-1 stays, the others rotate
-  const int n1 = mm - 1;
-  if (k == 0) {
-    p = n1;
-    q = step;
-  } else {
-    p = (step + k) % n1;
-    q = (step - k + n1) % n1;
-  }
-  if (p > q) {
-    const int t = p;
-    p = q;
-    q = t;
-  }
-}
-
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   const int tid = threadIdx.x;
@@ -345,28 +320,96 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
   return t;
 }
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+__device__ __forceinline__ double rsq_nr(double y) {  // y >= 1
+  double r = __builtin_amdgcn_rsq(y);
+  r = r * fma(-0.5 * y * r, r, 1.5);
+  r = r * fma(-0.5 * y * r, r, 1.5);
+  return r;
+}
+__device__ __forceinline__ void jac_rot(double app, double aqq, double apq, double& c, double& s) {
+  const double th = (aqq - app) * rcp_nr(2.0 * apq);
+  const double ath = fabs(th);
+  double t;
+  if (ath > 1e100) {
+    t = 0.5 * rcp_nr(th);
+  } else {
+    const double y = fma(th, th, 1.0);
+    const double sq = y * rsq_nr(y);
+    t = rcp_nr(ath + sq);
+    if (th < 0) t = -t;
+  }
+  c = rsq_nr(fma(t, t, 1.0));
+  s = t * c;
+}
+// where position j of the round robin moves between steps (slot k = positions 2k, 2k+1; the
+// first of slot 0 stays, the other firsts move one slot down, the seconds one slot up)
+__device__ __forceinline__ int rr_next(int mm, int j) {
+  const int half = mm >> 1, k = j >> 1;
+  if (mm == 2 || j == 0) return j;
+  if (j == 1) return 3;
+  if ((j & 1) == 0) return k == 1 ? 1 : j - 2;
+  return k == half - 1 ? j - 1 : j + 2;
+}
 
-__global__ __launch_bounds__(1024) void k_jacobi(double* __restrict__ G, int m, int old, const double* __restrict__ gt,
-                                                 const double* __restrict__ Yp, int warm, double* __restrict__ theta,
-                                                 double* __restrict__ Y, double* __restrict__ Ynext, double rel_tol2) {
-  __shared__ double A[64][65];
-  __shared__ double Vv[64][65];
-  __shared__ double Tm[64][65];
+// 64-bit DPP lane shifts across the wave: dn(x)[l] = x[l + 1], up(x)[l] = x[l - 1] (the lanes at
+// the ends keep their own value; callers never use those)
+__device__ __forceinline__ double dpp_dn(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const int lo = (int)(u & 0xffffffffu), hi = (int)(u >> 32);
+  const int l2 = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
+  const int h2 = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)h2 << 32) | (unsigned)l2);
+}
+__device__ __forceinline__ double dpp_up(double x) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+  const int lo = (int)(u & 0xffffffffu), hi = (int)(u >> 32);
+  const int l2 = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false);
+  const int h2 = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)h2 << 32) | (unsigned)l2);
+}
+
+// Subspace eigensolve of the Rayleigh-Ritz step: parallel-order (round-robin) cyclic Jacobi for a
+// symmetric m x m matrix, m <= 64, in one workgroup, warm-started: with P = blockdiag(Yp (warm x
+// warm), I) it diagonalises P^T G P and accumulates the rotations onto P, so a projected matrix
+// that grew by a few rows since the previous iteration is diagonal but for those rows.
+//
+// The matrix is kept by round-robin position, not by index: slot k is positions (2k, 2k + 1),
+// the indices at a slot's two positions form one of the step's m/2 disjoint pairs, and between
+// steps every position moves by one fixed permutation (rr_next), which walks each index past
+// every other once per mm - 1 steps.  Thread (bi, bj) owns the 2x2 block of rows of slot bi and
+// columns of slot bj: it reads the block as two 16-byte loads from the current buffer, rotates
+// it from both sides and writes its entries straight to their next-step positions in the other
+// buffer.  V's rows stay with their threads in registers (rows 2 bi, 2 bi + 1 at the columns of
+// slot bj): a step rotates them and moves them one slot along the wave by DPP lane shifts, so V
+// never goes through LDS during the sweeps.  A pair whose coupling is below the sweep's
+// threshold keeps c = 1, s = 0.  Outputs ascending eigenvalues and the matching eigenvectors
+// (columns of Y).
+constexpr int kJacT = 1024;
+constexpr int kJacLd = 66;  // row stride (doubles): 16-byte aligned rows
+__global__ __launch_bounds__(kJacT) void k_jacobi(double* __restrict__ G, int m, int old,
+                                                  const double* __restrict__ gt, const double* __restrict__ Yp, int warm,
+                                                  double* __restrict__ theta, double* __restrict__ Y,
+                                                  double* __restrict__ Ynext, double rel_tol2) {
+  __shared__ __attribute__((aligned(16))) double Ab[2][64][kJacLd];
+  __shared__ __attribute__((aligned(16))) double Vb[1][64][kJacLd];
   __shared__ double cs[32], sn[32];
-  __shared__ int pp[32], qq[32];
-  __shared__ double red[16];
+  __shared__ int rot[32];
+  __shared__ int pidx[2][64];
+  __shared__ double red[kJacT / 64];
   __shared__ int order[64];
   const int tid = threadIdx.x;
   const int mm = m + (m & 1), half = mm / 2;
-  // G (leading dimension 64) holds V^T W, whose columns [old, m) belong to the new basis
-  // vectors; the couplings of new and old vectors are completed by symmetry (from
-  // gt[(c - old) + r * 64] = v_c^T W v_r when given), the rest is symmetrised, and G is written
-  // back symmetric.
-  for (int i = tid; i < 64 * 64; i += 1024) {
+  for (int i = tid; i < 64 * 64; i += kJacT) {
     const int r = i / 64, c = i % 64;
     double v = (r == c) ? 1.0 : 0.0;
     if (r < warm && c < warm) v = Yp[r + c * 64];
-    Vv[r][c] = v;
+    Vb[0][r][c] = v;
     double g = 0.0;
     if (r < m && c < m) {
       if (r < old && c < old)
@@ -378,37 +421,81 @@ __global__ __launch_bounds__(1024) void k_jacobi(double* __restrict__ G, int m, 
       else
         g = 0.5 * (G[r + c * 64] + G[c + r * 64]);
     }
-    A[r][c] = g;
+    Ab[0][r][c] = g;
+  }
+  if (tid < 64) pidx[0][tid] = tid;
+  if (tid < 32) {
+    cs[tid] = 1.0;
+    sn[tid] = 0.0;
+    rot[tid] = 0;
   }
   __syncthreads();
-  for (int i = tid; i < m * m; i += 1024) {
+  for (int i = tid; i < m * m; i += kJacT) {
     const int r = i / m, c = i % m;
-    G[r + c * 64] = A[r][c];
+    G[r + c * 64] = Ab[0][r][c];
   }
-  if (warm > 0) {  // A = P^T Gs P
-    for (int i = tid; i < m * m; i += 1024) {
-      const int r = i / m, c = i % m;
-      double v = 0.0;
-      for (int k = 0; k < m; k++) v += A[r][k] * Vv[k][c];
-      Tm[r][c] = v;
+  if (warm > 0) {
+    // A = P^T A P in 2x2 register tiles (Ab[1] holds A P); sums over k < m keep A zero outside m
+    const int r0 = 2 * (tid >> 5), c0 = 2 * (tid & 31);
+    double a00 = 0.0, a01 = 0.0, a10 = 0.0, a11 = 0.0;
+    for (int k = 0; k < m; k++) {
+      const double x0 = Ab[0][r0][k], x1 = Ab[0][r0 + 1][k], y0 = Vb[0][k][c0], y1 = Vb[0][k][c0 + 1];
+      a00 = fma(x0, y0, a00);
+      a01 = fma(x0, y1, a01);
+      a10 = fma(x1, y0, a10);
+      a11 = fma(x1, y1, a11);
     }
-    __syncthreads();
-    for (int i = tid; i < m * m; i += 1024) {
-      const int r = i / m, c = i % m;
-      double v = 0.0;
-      for (int k = 0; k < m; k++) v += Vv[k][r] * Tm[k][c];
-      A[r][c] = v;
+    Ab[1][r0][c0] = a00;
+    Ab[1][r0][c0 + 1] = a01;
+    Ab[1][r0 + 1][c0] = a10;
+    Ab[1][r0 + 1][c0 + 1] = a11;
+    __syncthreads();  // A P complete; every read of Ab[0] is done
+    a00 = a01 = a10 = a11 = 0.0;
+    for (int k = 0; k < m; k++) {
+      const double x0 = Vb[0][k][r0], x1 = Vb[0][k][r0 + 1], y0 = Ab[1][k][c0], y1 = Ab[1][k][c0 + 1];
+      a00 = fma(x0, y0, a00);
+      a01 = fma(x0, y1, a01);
+      a10 = fma(x1, y0, a10);
+      a11 = fma(x1, y1, a11);
     }
-    __syncthreads();
+    Ab[0][r0][c0] = a00;
+    Ab[0][r0][c0 + 1] = a01;
+    Ab[0][r0 + 1][c0] = a10;
+    Ab[0][r0 + 1][c0 + 1] = a11;
   }
+  __syncthreads();
+  int ca = 0;
   const int bi = tid >> 5, bj = tid & 31;
-  const bool active = bi < half && bj < half;
-  const int vr0 = 2 * bi;  // rows of V owned by this thread
+  const bool blk = bi < half && bj < half;
+  const bool dgb = bi == bj;
+  // fixed per thread: where this block's rows and columns go after a step, as offsets (doubles)
+  // into a buffer; buffer cb is at + kBuf * cb
+  constexpr int kBuf = 64 * kJacLd;
+  const int rd0 = (2 * bi) * kJacLd + 2 * bj, rd1 = rd0 + kJacLd;
+  int wA00 = 0, wA01 = 0, wA10 = 0, wA11 = 0, wV00 = 0, wV01 = 0, wV10 = 0, wV11 = 0, Q0 = 0, Q1 = 0;
+  if (blk) {
+    const int P0 = rr_next(mm, 2 * bi), P1 = rr_next(mm, 2 * bi + 1);
+    Q0 = rr_next(mm, 2 * bj);
+    Q1 = rr_next(mm, 2 * bj + 1);
+    wA00 = P0 * kJacLd + Q0;
+    wA01 = P0 * kJacLd + Q1;
+    wA10 = P1 * kJacLd + Q0;
+    wA11 = P1 * kJacLd + Q1;
+    wV00 = (2 * bi) * kJacLd + Q0;
+    wV01 = (2 * bi) * kJacLd + Q1;
+    wV10 = (2 * bi + 1) * kJacLd + Q0;
+    wV11 = (2 * bi + 1) * kJacLd + Q1;
+  }
+  double* const Abase = &Ab[0][0][0];
+  // this thread's V entries: rows 2 bi, 2 bi + 1 at the positions of slot bj, in registers; a
+  // step rotates them and moves them one slot along the wave
+  double vx0 = Vb[0][2 * bi][2 * bj], vy0 = Vb[0][2 * bi][2 * bj + 1];
+  double vx1 = Vb[0][2 * bi + 1][2 * bj], vy1 = Vb[0][2 * bi + 1][2 * bj + 1];
   for (int sweep = 0; sweep < 40; sweep++) {
     double off = 0.0, dia = 0.0;
-    for (int i = tid; i < m * m; i += 1024) {
-      const int r = i / m, c = i % m;
-      const double a = A[r][c];
+    for (int i = tid; i < mm * mm; i += kJacT) {
+      const int r = i / mm, c = i % mm;
+      const double a = Ab[ca][r][c];
       if (r != c)
         off += a * a;
       else
@@ -418,75 +505,89 @@ __global__ __launch_bounds__(1024) void k_jacobi(double* __restrict__ G, int m, 
     __syncthreads();
     const double dias = block_sum(dia, red);
     if (offs <= rel_tol2 * dias || offs < 1e-300) break;
+    const double thr2 = fmax(rel_tol2 * dias / ((double)m * (double)m), 1e-300);
     for (int step = 0; step < mm - 1; step++) {
-      __syncthreads();  // previous step's updates are visible
+      __syncthreads();  // the previous step's buffers are complete
       if (tid < half) {
-        int p, q;
-        rr_pair(mm, step, tid, p, q);
+        const int p = 2 * tid, q = p + 1;
+        const double apq = Ab[ca][p][q], app = Ab[ca][p][p], aqq = Ab[ca][q][q];
         double c = 1.0, s = 0.0;
-        if (q < m) {
-          const double apq = A[p][q];
-          if (fabs(apq) > 1e-300) {
-            const double th = (A[q][q] - A[p][p]) / (2.0 * apq);
-            const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            c = 1.0 / sqrt(t * t + 1.0);
-            s = t * c;
-          }
+        int r = 0;
+        if (pidx[ca][p] < m && pidx[ca][q] < m && apq * apq > thr2) {
+          jac_rot(app, aqq, apq, c, s);
+          r = 1;
         }
         cs[tid] = c;
         sn[tid] = s;
-        pp[tid] = p;
-        qq[tid] = q;
+        rot[tid] = r;
       }
       __syncthreads();
-      if (active) {
-        // new column p = c*col p - s*col q, new column q = s*col p + c*col q (same for rows)
-        const int pi = pp[bi], qi = qq[bi], pj = pp[bj], qj = qq[bj];
-        const double ci = cs[bi], si = sn[bi], cj = cs[bj], sj = sn[bj];
-        const bool qiok = qi < m, qjok = qj < m;
-        const double a00 = A[pi][pj];
-        const double a01 = qjok ? A[pi][qj] : 0.0;
-        const double a10 = qiok ? A[qi][pj] : 0.0;
-        const double a11 = (qiok && qjok) ? A[qi][qj] : 0.0;
-        // rows
-        const double r00 = ci * a00 - si * a10, r01 = ci * a01 - si * a11;
-        const double r10 = si * a00 + ci * a10, r11 = si * a01 + ci * a11;
-        // columns
-        const double n00 = cj * r00 - sj * r01, n01 = sj * r00 + cj * r01;
-        const double n10 = cj * r10 - sj * r11, n11 = sj * r10 + cj * r11;
-        const bool diag = bi == bj;  // the rotated pair's coupling is annihilated exactly
-        A[pi][pj] = n00;
-        if (qjok) A[pi][qj] = diag ? 0.0 : n01;
-        if (qiok) A[qi][pj] = diag ? 0.0 : n10;
-        if (qiok && qjok) A[qi][qj] = n11;
-#pragma unroll
-        for (int e = 0; e < 2; e++) {
-          const int r = vr0 + e;
-          if (r < m) {
-            const double vp = Vv[r][pj], vq = qjok ? Vv[r][qj] : 0.0;
-            Vv[r][pj] = cj * vp - sj * vq;
-            if (qjok) Vv[r][qj] = sj * vp + cj * vq;
-          }
+      const double ci = cs[bi & 31], si = sn[bi & 31], cj = cs[bj], sj = sn[bj];
+      if (blk) {
+        const double* A0 = Abase + ca * kBuf;
+        double* A1 = Abase + (ca ^ 1) * kBuf;
+        const double2 a0 = *reinterpret_cast<const double2*>(A0 + rd0);
+        const double2 a1 = *reinterpret_cast<const double2*>(A0 + rd1);
+        // new column p = c*col p - s*col q, new column q = s*col p + c*col q (same for rows);
+        // pairs left alone have c = 1, s = 0, which reproduces their entries exactly
+        const double r00 = ci * a0.x - si * a1.x, r01 = ci * a0.y - si * a1.y;
+        const double r10 = si * a0.x + ci * a1.x, r11 = si * a0.y + ci * a1.y;
+        double n01 = sj * r00 + cj * r01, n10 = cj * r10 - sj * r11;
+        if (dgb && rot[bi]) n01 = n10 = 0.0;  // the rotated pair's coupling is annihilated exactly
+        A1[wA00] = cj * r00 - sj * r01;
+        A1[wA01] = n01;
+        A1[wA10] = n10;
+        A1[wA11] = sj * r10 + cj * r11;
+        if (bi == 0) {
+          pidx[ca ^ 1][Q0] = pidx[ca][2 * bj];
+          pidx[ca ^ 1][Q1] = pidx[ca][2 * bj + 1];
         }
       }
+      {
+        // V: rotate the columns of slot bj, then move to the next step's positions: the first
+        // of slot bj comes from slot bj + 1's first (slot 0 keeps its own, the last slot takes
+        // its own second), the second from slot bj - 1's second (slot 0 takes slot 1's first)
+        const double x0 = cj * vx0 - sj * vy0, y0 = sj * vx0 + cj * vy0;
+        const double x1 = cj * vx1 - sj * vy1, y1 = sj * vx1 + cj * vy1;
+        const double dx0 = dpp_dn(x0), dx1 = dpp_dn(x1), uy0 = dpp_up(y0), uy1 = dpp_up(y1);
+        if (half == 1) {
+          vx0 = x0; vy0 = y0; vx1 = x1; vy1 = y1;
+        } else {
+          vx0 = bj == 0 ? x0 : (bj == half - 1 ? y0 : dx0);
+          vx1 = bj == 0 ? x1 : (bj == half - 1 ? y1 : dx1);
+          vy0 = bj == 0 ? dx0 : uy0;
+          vy1 = bj == 0 ? dx1 : uy1;
+        }
+      }
+      ca ^= 1;
     }
     __syncthreads();
   }
   __syncthreads();
-  if (tid < m) {  // rank of each eigenvalue (ties by index)
-    const double d = A[tid][tid];
-    int rk = 0;
-    for (int j = 0; j < m; j++) {
-      const double e = A[j][j];
-      rk += (e < d || (e == d && j < tid)) ? 1 : 0;
-    }
-    order[rk] = tid;
+  if (blk) {
+    Vb[0][2 * bi][2 * bj] = vx0;
+    Vb[0][2 * bi][2 * bj + 1] = vy0;
+    Vb[0][2 * bi + 1][2 * bj] = vx1;
+    Vb[0][2 * bi + 1][2 * bj + 1] = vy1;
   }
   __syncthreads();
-  for (int i = tid; i < m; i += 1024) theta[i] = A[order[i]][order[i]];
-  for (int i = tid; i < m * m; i += 1024) {
+  // positions holding one of the m indices, ranked by eigenvalue (ties by position)
+  if (tid < mm) {
+    const bool ok = pidx[ca][tid] < m;
+    const double d = Ab[ca][tid][tid];
+    int rk = 0;
+    for (int j = 0; j < mm; j++) {
+      if (pidx[ca][j] >= m) continue;
+      const double e = Ab[ca][j][j];
+      rk += (e < d || (e == d && j < tid)) ? 1 : 0;
+    }
+    if (ok) order[rk] = tid;
+  }
+  __syncthreads();
+  for (int i = tid; i < m; i += kJacT) theta[i] = Ab[ca][order[i]][order[i]];
+  for (int i = tid; i < m * m; i += kJacT) {
     const int r = i % m, c = i / m;
-    const double v = Vv[r][order[c]];
+    const double v = Vb[0][r][order[c]];
     Y[r + c * 64] = v;
     Ynext[r + c * 64] = v;
   }
@@ -970,7 +1071,7 @@ void Davidson::rayleigh_ritz(hipStream_t st) {
   double rel = 1e-15;
   if (have_worst_ && theta_abs_max_ > 0.0)
     rel = std::max(1e-15, std::min(1e-6, 1e-3 * std::max(last_worst_, p_.tol) / theta_abs_max_));
-  k_jacobi<<<1, 1024, 0, st>>>(G_.get(), m_, old, gt, Yprev_.get(), std::min(warm_, m_), theta_.get(), Y_.get(),
+  k_jacobi<<<1, kJacT, 0, st>>>(G_.get(), m_, old, gt, Yprev_.get(), std::min(warm_, m_), theta_.get(), Y_.get(),
                                Ynext_.get(), rel * rel);
   PQG_HIP(hipGetLastError());
   std::swap(Yprev_, Ynext_);
