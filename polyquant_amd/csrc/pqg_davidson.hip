@@ -310,6 +310,11 @@ __global__ void k_scale_cols(double* __restrict__ v, int64_t n, int64_t ld, cons
   for (int j = 0; j < k; j++) v[i + j * ld] *= s[j];
 }
 
+__device__ __forceinline__ double wave_sum64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   const int tid = threadIdx.x;
@@ -593,6 +598,38 @@ __global__ __launch_bounds__(kJacT) void k_jacobi(double* __restrict__ G, int m,
   }
 }
 
+// Coefficients of a thick restart that keeps the previous iteration's Ritz vectors (GD+k):
+// columns 0..keep-nz-1 of Q are the current Ritz vectors Y[:, c]; column keep-nz+j is the
+// previous Ritz vector j, Z_j = [Yk[:, j] (mp rows, the earlier basis is a prefix of the
+// current one); 0], orthogonalised twice against the columns before it and normalised; when
+// less than 1e-6 of it is left (the root has stopped moving) the next current Ritz vector
+// Y[:, keep-nz+j] takes its place.  One wave, lane r holds row r of every column (m <= 64).
+__global__ __launch_bounds__(64) void k_gdk_basis(const double* __restrict__ Y, const double* __restrict__ Yk,
+                                                  int m, int mp, int keep, int nz, double* __restrict__ Q) {
+  __shared__ double q[64][65];
+  const int r = threadIdx.x;
+  const int nc = keep - nz;
+  for (int c = 0; c < nc; c++) q[c][r] = r < m ? Y[r + c * 64] : 0.0;
+  for (int j = 0; j < nz; j++) {
+    const int c = nc + j;
+    for (int attempt = 0; attempt < 2; attempt++) {
+      double v = attempt == 0 ? (r < mp ? Yk[r + j * 64] : 0.0) : (r < m ? Y[r + c * 64] : 0.0);
+      const double v0 = sqrt(wave_sum64(v * v));
+      for (int pass = 0; pass < 2; pass++)
+        for (int b = 0; b < c; b++) {
+          const double d = wave_sum64(q[b][r] * v);
+          v -= d * q[b][r];
+        }
+      const double nrm = sqrt(wave_sum64(v * v));
+      if (attempt == 1 || nrm > 1e-6 * v0) {
+        q[c][r] = nrm > 0.0 ? v / nrm : 0.0;
+        break;
+      }
+    }
+  }
+  for (int c = 0; c < keep; c++) Q[r + c * 64] = r < m ? q[c][r] : 0.0;
+}
+
 // Cholesky factor of a small Gram matrix B (n x n, leading dimension 64) with column dropping,
 // on the device: column j is kept when its residual norm after the kept columns before it
 // exceeds thr_j = max(rel * sqrt(g0[j][j]), 1e-30) (thr_j = 0 without g0), and, when rg is given,
@@ -717,6 +754,9 @@ void Davidson::setup() {
   nmax_ = (int)std::min<int64_t>(std::max(nmax_, ninit_ + nev_), std::min<int64_t>(ci->n_ref, 64));
   PQG_REQUIRE(nmax_ >= ninit_, PQG_ERR_ARG, "subspace too small");
   mixed_ = p.precision == PQG_MIXED;
+  // thick restarts keep the previous iteration's Ritz vectors unless PQG_DAV_GDK=0
+  gdk_ = true;
+  if (const char* e = std::getenv("PQG_DAV_GDK")) gdk_ = std::atoi(e) != 0;
   pqg_ctx* ctx = ci->ctx;
   V_.reset(ctx, (size_t)ld_ * nmax_);
   W_.reset(ctx, (size_t)ld_ * nmax_);
@@ -730,6 +770,8 @@ void Davidson::setup() {
   RT_.reset(ctx, (size_t)ld_ * nev_ * 2);  // residuals, then corrections (one Gram covers both)
   tmp_.reset(ctx, (size_t)ld_ * ninit_);
   small_.reset(ctx, 64 * 64);
+  Yk_.reset(ctx, 64 * 64);
+  Qr_.reset(ctx, 64 * 64);
   gram0_.reset(ctx, 64 * 64);
   rgram_.reset(ctx, 64 * 64);
   mbuf_.reset(ctx, 64 * 64);
@@ -1017,10 +1059,18 @@ int Davidson::orthonormalize_append(hipStream_t st, double* T, int nt) {
 
 // Thick restart on the first `keep` Ritz vectors of the current subspace (Y from the last
 // Rayleigh-Ritz); the projected matrix is recomputed exactly.
+// Thick restart to `keep` vectors: the lowest Ritz vectors, or (GD+k) the lowest keep - nev
+// of them and the previous iteration's nev Ritz vectors
 void Davidson::restart_to_ritz(hipStream_t st, int keep) {
-  ts_gemm_nn(st, V_.get(), ld_, m_, Y_.get(), 64, keep, tmp_.get(), ld_, 1.0, 0.0);
+  const double* Yr = Y_.get();
+  if (gdk_ && yk_rows_ > 0 && keep > nev_ && keep <= m_) {
+    k_gdk_basis<<<1, 64, 0, st>>>(Y_.get(), Yk_.get(), m_, yk_rows_, keep, nev_, Qr_.get());
+    PQG_HIP(hipGetLastError());
+    Yr = Qr_.get();
+  }
+  ts_gemm_nn(st, V_.get(), ld_, m_, Yr, 64, keep, tmp_.get(), ld_, 1.0, 0.0);
   PQG_HIP(hipMemcpyAsync(V_.get(), tmp_.get(), (size_t)ld_ * keep * 8, hipMemcpyDeviceToDevice, st));
-  ts_gemm_nn(st, W_.get(), ld_, m_, Y_.get(), 64, keep, tmp_.get(), ld_, 1.0, 0.0);
+  ts_gemm_nn(st, W_.get(), ld_, m_, Yr, 64, keep, tmp_.get(), ld_, 1.0, 0.0);
   PQG_HIP(hipMemcpyAsync(W_.get(), tmp_.get(), (size_t)ld_ * keep * 8, hipMemcpyDeviceToDevice, st));
   m_ = nW_ = keep;
   warm_ = 0;
@@ -1068,6 +1118,14 @@ void Davidson::rayleigh_ritz(hipStream_t st) {
   // Off-diagonal tolerance of the subspace eigensolve: the eigenvectors only need to be accurate
   // well below the current residual level (1e-3 of the worst residual relative to the largest
   // Ritz value), and to full precision once the residuals approach the convergence threshold.
+  // the previous iteration's Ritz vectors in this basis (its first warm_ columns are the
+  // previous basis), for a restart that keeps them
+  if (gdk_ && warm_ > 0) {
+    PQG_HIP(hipMemcpyAsync(Yk_.get(), Y_.get(), (size_t)64 * nev_ * 8, hipMemcpyDeviceToDevice, st));
+    yk_rows_ = warm_;
+  } else {
+    yk_rows_ = 0;
+  }
   double rel = 1e-15;
   if (have_worst_ && theta_abs_max_ > 0.0)
     rel = std::max(1e-15, std::min(1e-6, 1e-3 * std::max(last_worst_, p_.tol) / theta_abs_max_));

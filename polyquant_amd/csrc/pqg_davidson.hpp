@@ -108,6 +108,9 @@ class Davidson {
   bool mixed_;
   DevBuf<double> V_, W_, G_, Y_, Yprev_, Ynext_, theta_, X_, AX_, RT_, tmp_, part_, small_, W64_;
   DevBuf<double> gram0_, rgram_, mbuf_, status_dev_;
+  DevBuf<double> Yk_, Qr_;  // previous Ritz coefficients and the restart coefficients (GD+k)
+  bool gdk_ = true;         // thick restart keeps the previous Ritz vectors (GD+k)
+  int yk_rows_ = 0;         // rows of Yk_ (0: none saved for this basis)
   DevBuf<int> nk_dev_;
   PinnedHost status_pin_;        // pinned host copy of status_dev_
   double* status_h_ = nullptr;
