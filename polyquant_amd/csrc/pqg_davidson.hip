@@ -325,6 +325,12 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += red[w];
   return t;
 }
+// Rotation (c, s) that annihilates the coupling apq of a pair with diagonal entries app, aqq
+// (Rutishauser's form): t = sgn(th) / (|th| + sqrt(th^2 + 1)), th = (aqq - app) / (2 apq),
+// c = 1 / sqrt(1 + t^2), s = t c.  The FP64 division and square root are long instruction
+// sequences on the eigensolve's critical path, so the hardware reciprocal and reciprocal square
+// root are refined by two Newton steps each instead; c is then accurate to rounding and the
+// rotation orthogonal.
 __device__ __forceinline__ double rcp_nr(double x) {
   double r = __builtin_amdgcn_rcp(x);
   r = fma(r, fma(-x, r, 1.0), r);
