@@ -150,7 +150,7 @@ def test_water_sto3g_cisd_on_gpu_matches_reference(ctx):
     assert rep.converged == 1
     E = E + prob.e_shift
     assert abs(E[0] - float(fx["e0"])) < 1e-8
-    assert abs(E[1] - float(fx["e1"])) < 1e-6
+    assert abs(E[1] - float(fx["e1"])) < 1e-7  # 6e-8: from the input elements (test_oracle)
 
 
 def test_water_631g_fci_sigma_matches_oracle_rows(ctx):
@@ -305,7 +305,7 @@ def test_exact_diag_water_sto3g_cisd(ctx):
     ci = EPCI(prob, ctx, num_states=2, exact_diag=True)
     E = ci.run()
     assert abs(E[0] - float(fx["e0"])) < 1e-8
-    assert abs(E[1] - float(fx["e1"])) < 1e-6
+    assert abs(E[1] - float(fx["e1"])) < 1e-7  # 6e-8: from the input elements (test_oracle)
     assert ci.C_ci.shape == (141, 2)
     dv = EPCI(prob, ctx, num_states=2, num_subspace_vec=20, convergence_E=1e-10)
     Ed = dv.run()
