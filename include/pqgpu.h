@@ -61,9 +61,9 @@ typedef struct {
  * (epci.cpp:589-608): num_states, num_subspace_vec, max(2*nsv, 10*nstates), iteration_max,
  * convergence_E.  precision selects PQG_FP64 or PQG_MIXED.                              */
 typedef struct {
-  int nev;
+  int nev;                         /* 1..64                                              */
   int nvec_init;
-  int nvec_max;                    /* <= 0: max(2*nvec_init, 10*nev)                     */
+  int nvec_max;                    /* <= 0: max(2*nvec_init, 10*nev); clamped to 64      */
   int maxit;
   double tol;                      /* residual-norm tolerance                            */
   int precision;
