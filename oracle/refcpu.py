@@ -360,3 +360,71 @@ def ao2mo(ao_full, Ca, Cb=None):
     out = np.zeros((nma * (nma + 1) // 2, nmb * (nmb + 1) // 2), order="F")
     lib().rc_ao2mo2(na, nb, nma, nmb, _ptr(ao), _ptr(Ca), _ptr(Cb), _ptr(out))
     return out
+
+
+# --- QMCPACK multideterminant dump (reference's word loops restated, 64-bit words) ------------
+_M64 = (1 << 64) - 1
+
+
+def get_det_withfcorbs(det, nfc, max_orb, frozen_core):
+    """POLYQUANT_DETSET::get_det_withfcorbs (src/ci/detset/determinant_creation.cpp:391-454),
+    step for step: det is a string as a list of 64-bit words, most significant first (orbital o
+    in word W-1-(o>>6)); num_int from the largest max_orb + frozen_core over the species
+    (:396-403); with no frozen core the string is returned, or prefixed by a zero word
+    (:405-413); otherwise the overflow of the top word opens a new word (:418-426), each word
+    is shifted left by nfc and takes the top nfc bits of the next word (:427-442), and the
+    lowest word gets the nfc core orbitals set (:446-448)."""
+    one = 1
+    max_orb_with_fc = [m + f for m, f in zip(max_orb, frozen_core)]
+    if nfc > 64:
+        raise ValueError("Can't handle >=64 core orbitals")
+    num_int = (max(max_orb_with_fc) >> 6) + 1
+    if nfc == 0:
+        if num_int == len(det):
+            return list(det)
+        return [0] + list(det)
+    new_det = []
+    mask = ((one << nfc) - one) & _M64
+    if num_int != len(det):
+        new_det.append((det[0] >> (64 - nfc)) & mask)
+    for count, i in enumerate(det):
+        j = (i << nfc) & _M64
+        if count + 1 != len(det):
+            j |= (det[count + 1] >> (64 - nfc)) & mask
+        if count == len(det) - 1:
+            j |= ~(~0 & (~(one << nfc) + one)) & _M64
+        new_det.append(j)
+    return new_det
+
+
+def qmcpack_multidet(oc, C, max_orb, frozen_core, n_states):
+    """The /MultiDet datasets of dump_post_mf_for_qmcpack (src/calculation/calculation.cpp:622-657)
+    and dump_post_mf_to_hdf5_for_QMCPACK (src/io/hdf5_utilities.cpp:399-468): per (part, spin)
+    the padded string of every determinant (address order, det_idx_unfold) with its words
+    reversed (CI_{2 part + spin}, N_dets x N_int), one coefficient column per state (Coeff,
+    Coeff_1, ...), NbDet, Nbits (words of part 0's strings), nexcitedstate and nstate (the
+    largest max_orb + frozen_core)."""
+    u = oc.unfold_all()
+    nsp = len(max_orb)
+    W = (max(max_orb) >> 6) + 1  # words of the unpadded strings (determinant_creation.cpp:12-13)
+    out = {}
+    nbits = None
+    for p in range(nsp):
+        for s in range(2):
+            strs = oc.strings(p, s)
+            rows = []
+            for i in range(oc.N):
+                x = int(strs[u[i, 2 * p + s]])
+                det = [(x >> (64 * (W - 1 - w))) & _M64 for w in range(W)]
+                rows.append(get_det_withfcorbs(det, frozen_core[p], max_orb, frozen_core))
+            if nbits is None:
+                nbits = len(rows[0])
+            out[f"CI_{2 * p + s}"] = np.array([[r[nbits - j - 1] for j in range(nbits)] for r in rows],
+                                             dtype=np.uint64)
+    for i in range(n_states):
+        out["Coeff" if i == 0 else f"Coeff_{i}"] = np.array([C[j, i] for j in range(oc.N)], dtype=np.float64)
+    out["NbDet"] = oc.N
+    out["Nbits"] = nbits
+    out["nexcitedstate"] = n_states
+    out["nstate"] = max(m + f for m, f in zip(max_orb, frozen_core))
+    return out

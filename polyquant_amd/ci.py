@@ -456,6 +456,20 @@ class EPCI:
         self.S_squared = self.detset.evaluate_s2(self.C_ci)
         return self.S_squared
 
+    def dump_post_mf_for_qmcpack(self, path: Optional[str] = None, frozen_core: Optional[list] = None) -> dict:
+        """POLYQUANT_CALCULATION::dump_post_mf_for_qmcpack (src/calculation/calculation.cpp:622-657):
+        the /MultiDet datasets of every state (strings padded with frozen_core orbitals per species,
+        see polyquant_amd.qmcpack); with path, also saved as an .npz archive."""
+        from . import qmcpack
+
+        if self.C_ci is None:
+            raise RuntimeError("run() first")
+        arrays = qmcpack.multidet_arrays(self.detset, self.C_ci, [sp.n_orb for sp in self.prob.species], frozen_core,
+                                         self.num_states)
+        if path is not None:
+            qmcpack.save_multidet(path, arrays)
+        return arrays
+
     def calculate_NOs(self, states=None, orbitals=None):
         """POLYQUANT_EPCI::calculate_NOs (src/ci/epci.cpp:200-315) without frozen core: the
         1-RDM of every (species, spin) for each listed state on the GPU, then diag_dm_helper
