@@ -18,7 +18,10 @@ the sigma.  Its launch durations come from HIP events on the CI stream during on
 sigma of 4 vectors; achieved = the algorithmic flops of that sigma, F = 2 * k * N * (conn + 1)
 with conn = 2 s + 2 d + s^2 = 5792 connected determinants per determinant (s = 64 singles,
 d = 784 same-spin doubles per string) and k = 4, divided by the summed k_lg_u time; the GEMM's
-own flop count (its M x K x N as posed) is reported beside it.
+own flop count (its M x K x N as posed) is reported beside it.  The sigma's HBM-bound kernel, the
+gather of U into sigma (k_lg_gather), is reported in sigma_kernels against the 8 TB/s HBM peak on
+its algorithmic bytes, k * N * w * (2 L + 3): U read once per pass (L = 64 links per string),
+sigma written in both passes and read back in the accumulating one (w = 8 B FP64, 4 B mixed).
 cpu_baseline: the oracle's restated reference sigma on strided determinant samples of the same
 space and vectors, per-row cost scaled to the full sigma and multiplied by the number of sigma
 vectors of the GPU FP64 solve (an extrapolation, labelled so in "sample").
@@ -299,7 +302,12 @@ def bench_fci16(args):
     kern = {}
     # the mixed-precision sigma runs its link GEMM as BF16x3 (three bf16 products per FP32 product on
     # the bf16 matrix cores: 2.5 PFLOP/s dense / 3 of FP32-equivalent work)
+    # the gather (k_lg_gather, HBM-bound) reads U once per pass (N x links x vectors elements) and
+    # writes sigma in both passes, reading it back in the accumulating one
+    links = s1
     for tag, prec, suf, peak in (("fp64", PQG_FP64, "_f64", 78.6), ("bf16x3", PQG_MIXED, "_f32", 2500.0 / 3)):
+        w = 8 if prec == PQG_FP64 else 4
+        gather_bytes = k * N * w * (2 * links + 3)
         ds.sigma_device(C.data_ptr(), N, k, S.data_ptr(), N, prec)  # warm-up
         torch.cuda.synchronize()
         ctx.profile(True)
@@ -317,7 +325,10 @@ def bench_fci16(args):
                      "alg_tflops": round(f_alg / (gemm_ms * 1e-3) / 1e12, 3),
                      "alg_frac": round(f_alg / (gemm_ms * 1e-3) / 1e12 / peak, 4),
                      "gemm_tflops": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 3),
-                     "gemm_frac": round(gemm_flops / (gemm_ms * 1e-3) / 1e12 / peak, 4), "peak_tflops": peak}
+                     "gemm_frac": round(gemm_flops / (gemm_ms * 1e-3) / 1e12 / peak, 4), "peak_tflops": peak,
+                     "gather_alg_bytes": gather_bytes,
+                     "gather_hbm_tbs": round(gather_bytes / (gather_ms * 1e-3) / 1e12, 3),
+                     "gather_hbm_frac": round(gather_bytes / (gather_ms * 1e-3) / 8e12, 4)}
         log(f"[fci16] sigma {tag}: {kern[tag]}")
     del C, S
     r64 = kern["fp64"]
