@@ -66,8 +66,15 @@ struct LinkGemm {
   bool enabled = false;
   LinkTab tab[kSlots];
   DevBuf<char> U;             // GEMM output for one column chunk
+  DevBuf<char> U2;            // second output block: the gather of a chunk overlaps the next GEMM
   DevBuf<uint32_t> P;         // BF16x3 kernel: the vector of a pass as bf16 (hi, lo) pairs
   DevBuf<float> V32s[kSlots]; // FP32 copies of the same-set coupling blocks
+  hipStream_t side = nullptr; // the overlapped gathers (default priority), created on first use
+  hipEvent_t ev[4] = {};      // GEMM done into U / U2, gather done from U / U2
+  LinkGemm() = default;
+  LinkGemm(const LinkGemm&) = delete;
+  LinkGemm& operator=(const LinkGemm&) = delete;
+  ~LinkGemm();
 };
 
 }  // namespace pqg
@@ -146,6 +153,7 @@ struct pqg_ci {
   // setup or a solve is abandoned on an error.
   ~pqg_ci() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (lg.side) (void)hipStreamSynchronize(lg.side);
   }
 };
 

@@ -561,9 +561,11 @@ static const float* xblock(pqg_ci* ci, int t, int u, float) { return ci->X32[t][
 
 // U bytes per column of a pass, and the chunk width that keeps U within its budget
 static size_t lg_per_col(const LgPass& d, size_t w) { return (size_t)d.n_r * d.L * w; }
-static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w, int nt) {
+// share: the part of the budget one output block takes (1, or 2/3 with two blocks)
+static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w, int nt, double share = 1.0) {
   const size_t per_col = lg_per_col(d, w);
-  const size_t want = std::min<size_t>(lg_budget_bytes(ci->n_int, w), per_col * (size_t)n_c);
+  const size_t want = std::min<size_t>((size_t)(share * (double)lg_budget_bytes(ci->n_int, w)),
+                                       per_col * (size_t)n_c);
   const size_t budget = std::max<size_t>(per_col * nt, want);
   int cw = (int)std::min<size_t>((size_t)n_c, budget / per_col);
   if (cw < n_c) {
@@ -583,6 +585,39 @@ static int lg_nt(const LgPass& d, int n_c, size_t w, int ksplit, int kqh) {
   if (w != 4 || ksplit != 2) return LG_NT;
   if (const char* e = std::getenv("PQG_LG_NT")) return std::atoi(e) == 64 ? LG_NT32 : LG_NT;
   return lg_lds_bytes(d, n_c, kqh, ksplit, w, LG_NT32) <= 160 * 1024 ? LG_NT32 : LG_NT;
+}
+
+static bool lg_overlap_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("PQG_LG_OVERLAP");
+    return !e || std::atoi(e) != 0;
+  }();
+  return on;
+}
+
+static bool lg_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  PQG_HIP(hipStreamIsCapturing(st, &cs));
+  return cs != hipStreamCaptureStatusNone;
+}
+
+// the side stream of the overlapped gathers and its events, once per handle
+static void lg_side_init(LinkGemm& lg) {
+  if (lg.side) return;
+  std::lock_guard<std::recursive_mutex> dl(device_op_mutex());
+  for (auto& e : lg.ev)
+    if (!e) PQG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  PQG_HIP(hipStreamCreateWithFlags(&lg.side, hipStreamNonBlocking));
+}
+
+LinkGemm::~LinkGemm() {
+  std::lock_guard<std::recursive_mutex> dl(device_op_mutex());
+  if (side) {
+    (void)hipStreamSynchronize(side);
+    (void)hipStreamDestroy(side);
+  }
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
 }
 
 // One pass of the link GEMM for one vector; accumulate: add to S or overwrite it.
@@ -654,18 +689,32 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
     a.part_off = sh.part_off;
     a.bstride = sh.stride;
   }
+  // FP32: two output blocks when the columns take more than one chunk; the gather of chunk i
+  // (HBM-bound) runs on a side stream while the GEMM of chunk i + 1 writes the other block, each
+  // block taking 2/3 of the budget.  Not under graph capture or in the allocation-free call;
+  // PQG_LG_OVERLAP=0 turns it off.  Measured at 16 orbitals 8a8b, sigma of 4 vectors: BF16x3
+  // 280.9 ms against 296.6 ms; FP64 858 ms against 864 ms, the FP64 GEMM slowing by what the
+  // hidden gathers save (five chunks instead of four, shared bandwidth), so FP64 keeps one stream.
+  bool overlap = sizeof(T) == 4 && !fixed_scratch() && lg_overlap_enabled() &&
+                 lg_chunk(ci, d, a.n_c, sizeof(T), nt) < a.n_c && !lg_capturing(st);
+  const double share = overlap ? 2.0 / 3.0 : 1.0;
   // U holds n_r * L * ldu values (ldu = cw rounded up to 4); sized for the larger of the two
   // passes so it grows once
-  int cw = lg_chunk(ci, d, a.n_c, sizeof(T), nt);
+  int cw = lg_chunk(ci, d, a.n_c, sizeof(T), nt, share);
   size_t need = lg_per_col(d, sizeof(T)) * (size_t)(cw + 3);
   {
     const LgPass o = lg_pass_desc(ci, 1 - pass);
     const int n_co = ci->slot[o.col].nstr;
     const int kqo = lg_kqh_bound((o.nin + o.L + 3) / 4, ksplit);
     need = std::max(need, lg_per_col(o, sizeof(T)) * (size_t)(lg_chunk(ci, o, n_co, sizeof(T),
-                                                                        lg_nt(o, n_co, sizeof(T), ksplit, kqo)) + 3));
+                                                                        lg_nt(o, n_co, sizeof(T), ksplit, kqo),
+                                                                        share) + 3));
   }
-  if (FixedScratch* fs = fixed_scratch()) {
+  if (overlap) {
+    grow(lg.U, ci->ctx, need, st);
+    grow(lg.U2, ci->ctx, need, st);
+    lg_side_init(lg);
+  } else if (FixedScratch* fs = fixed_scratch()) {
     // allocation-free call: U is a view of what the fixed workspace has left, and the column
     // chunk shrinks to fit it (the per-element sums do not depend on the chunking)
     if (!lg.U.get()) lg.U.view(fs, std::min(need, fs->left() > 256 ? fs->left() - 256 : 0));
@@ -680,7 +729,19 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
   const int mb = (a.L + 15) / 16;
   // algorithmic flops of the GEMM as posed (M = L links, K = in-row rows + links, no padding)
   const int64_t flops_per_col = 2ll * a.n_r * a.L * a.K;
-  for (int c0 = 0; c0 < a.n_c; c0 += cw) {
+  // with two blocks the handle's stream waits for the side stream's last gather when the pass
+  // ends (work that follows on it sees every gather); on an error the side stream is drained
+  int chunk = 0;
+  try {
+  for (int c0 = 0; c0 < a.n_c; c0 += cw, chunk++) {
+    const int blk = chunk & 1;
+    hipStream_t gst = st;
+    if (overlap) {
+      a.U = reinterpret_cast<T*>(blk ? lg.U2.get() : lg.U.get());
+      // the block is free once the gather of chunk - 2 has read it
+      if (chunk >= 2) PQG_HIP(hipStreamWaitEvent(st, lg.ev[2 + blk], 0));
+      gst = lg.side;
+    }
     a.c0 = c0;
     a.cw = std::min(cw, a.n_c - c0);
     a.ldu = (a.cw + 3) / 4 * 4;
@@ -706,14 +767,24 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
       }
       PQG_HIP(hipGetLastError());
     }
+    if (overlap) {
+      PQG_HIP(hipEventRecord(lg.ev[blk], st));
+      PQG_HIP(hipStreamWaitEvent(gst, lg.ev[blk], 0));
+    }
     {
-      ProfScope prof(ci->ctx, st, sizeof(T) == 8 ? "lg_gather_f64" : "lg_gather_f32",
+      ProfScope prof(ci->ctx, gst, sizeof(T) == 8 ? "lg_gather_f64" : "lg_gather_f32",
                      (int64_t)a.n_r * a.cw * (a.L + 2) * (int64_t)sizeof(T));
       constexpr int CPT = 4;
       dim3 g2((unsigned)a.n_r, (unsigned)lg_grid(a.cw, 256 * CPT * (16 / (int)sizeof(T))));
-      k_lg_gather<T, CPT><<<g2, 256, 0, st>>>(a, S);
+      k_lg_gather<T, CPT><<<g2, 256, 0, gst>>>(a, S);
       PQG_HIP(hipGetLastError());
     }
+    if (overlap) PQG_HIP(hipEventRecord(lg.ev[2 + blk], gst));
+  }
+  if (overlap) PQG_HIP(hipStreamWaitEvent(st, lg.ev[2 + ((chunk - 1) & 1)], 0));
+  } catch (...) {
+    if (overlap) (void)hipStreamSynchronize(lg.side);
+    throw;
   }
 }
 
