@@ -577,14 +577,17 @@ static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w, int nt
   return cw;
 }
 
-// tile width of a pass: 64 columns for the FP32 kernel with the K split over two wave groups
-// when the tile and the staged row fit LDS, else 32 (measured, FP32 sigma of 4 vectors: 16
-// orbitals 8a8b 675 ms with 64 columns, 726 ms with 32; water 6-31G, which takes the 256-thread
-// kernel, 8.3 ms with 64 and 5.1 ms with 32)
+// tile width of a pass: 64 columns with the K split over two wave groups when the tile and the
+// staged row fit LDS (FP32 always, FP64 on spaces of 2e7 determinants and more), else 32.
+// Measured, FP32 sigma of 4 vectors: 16 orbitals 8a8b 675 ms with 64 columns, 726 ms with 32;
+// water 6-31G, which takes the 256-thread kernel, 8.3 ms with 64 and 5.1 ms with 32.  FP64: at
+// 16 orbitals 8a8b only pass B fits (K = 81), 831 ms against 862 ms; water 6-31G 8.2 ms against
+// 8.0 ms.  PQG_LG_NT=32|64 overrides (64 only where it fits).
 static int lg_nt(const LgPass& d, int n_c, size_t w, int ksplit, int kqh) {
-  if (w != 4 || ksplit != 2) return LG_NT;
-  if (const char* e = std::getenv("PQG_LG_NT")) return std::atoi(e) == 64 ? LG_NT32 : LG_NT;
-  return lg_lds_bytes(d, n_c, kqh, ksplit, w, LG_NT32) <= 160 * 1024 ? LG_NT32 : LG_NT;
+  if (ksplit != 2) return LG_NT;
+  const bool fits = lg_lds_bytes(d, n_c, kqh, ksplit, w, LG_NT32) <= 160 * 1024;
+  if (const char* e = std::getenv("PQG_LG_NT")) return std::atoi(e) == 64 && fits ? LG_NT32 : LG_NT;
+  return fits && (w == 4 || d.n_r * n_c >= 20000000) ? LG_NT32 : LG_NT;
 }
 
 static bool lg_overlap_enabled() {
@@ -761,6 +764,9 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
           if (mb <= 2) lg_go_kq<T, 2, LG_NT>(a, ksplit, kqh, grid, lds, st);
           else lg_go_kq<T, 1, LG_NT>(a, ksplit, kqh, grid, lds, st);
         }
+      } else if (nt == LG_NT32) {
+        if (mb <= 2) lg_go_kq<T, 2, LG_NT32>(a, ksplit, kqh, grid, lds, st);
+        else lg_go_kq<T, 1, LG_NT32>(a, ksplit, kqh, grid, lds, st);
       } else {
         if (mb <= 2) lg_go_kq<T, 2, LG_NT>(a, ksplit, kqh, grid, lds, st);
         else lg_go_kq<T, 1, LG_NT>(a, ksplit, kqh, grid, lds, st);
