@@ -109,6 +109,16 @@ pqg_status pqg_ci_destroy(pqg_ci* ci);
 /* rows()/cols() (determinant_set.hpp:127-134) */
 int64_t pqg_ci_ndets(const pqg_ci* ci);
 int64_t pqg_ci_nstrings(const pqg_ci* ci, int part, int spin);
+/* Which sigma kernel family the handle runs (no counterpart in the reference, whose sigma has one
+ * form): chosen at pqg_ci_create from the shape of the space and the limits of the fast paths
+ * (the link GEMM needs full CI with a column set of at most 16 orbitals whose row fits LDS and at
+ * most 64 single excitations per row tuple; the pair GEMM one particle per string), or the stored
+ * explicit H after pqg_ci_set_explicit.  Returns a PQG_SIGMA_* value, -1 for a null handle.  */
+#define PQG_SIGMA_ENUMERATION 0 /* general pull-form enumeration kernel                   */
+#define PQG_SIGMA_LINK_GEMM 1   /* full CI, >= 2 particles per string set: link GEMM      */
+#define PQG_SIGMA_PAIR_GEMM 2   /* one particle per string: pair GEMM                     */
+#define PQG_SIGMA_STORED_H 3    /* stored explicit H (FP64)                               */
+int pqg_ci_sigma_path(const pqg_ci* ci);
 /* diagonal_Hii (determinant_set.hpp:259), reference address order */
 pqg_status pqg_ci_diag(pqg_ci* ci, double* hdiag);
 /* operator()(i, j) -> Slater_Condon (determinant_set.hpp:139, slater_condon.cpp:4-100) for

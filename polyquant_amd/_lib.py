@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libpqgpu.so")
 PQG_OK, PQG_ERR_ARG, PQG_ERR_HIP, PQG_ERR_OOM, PQG_ERR_NOCONV, PQG_ERR_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 PQG_FP64, PQG_MIXED = 0, 1
 PQG_AO2MO_FP64, PQG_AO2MO_BF16X3 = 0, 1
+PQG_SIGMA_ENUMERATION, PQG_SIGMA_LINK_GEMM, PQG_SIGMA_PAIR_GEMM, PQG_SIGMA_STORED_H = 0, 1, 2, 3
 
 _dp = ct.POINTER(ct.c_double)
 _vp = ct.c_void_p
@@ -86,6 +87,7 @@ SIGNATURES = [
     ("pqg_ci_destroy", ct.c_int, [_vp]),
     ("pqg_ci_ndets", ct.c_int64, [_vp]),
     ("pqg_ci_nstrings", ct.c_int64, [_vp, ct.c_int, ct.c_int]),
+    ("pqg_ci_sigma_path", ct.c_int, [_vp]),
     ("pqg_ci_diag", ct.c_int, [_vp, _vp]),
     ("pqg_ci_elements", ct.c_int, [_vp, _vp, _vp, ct.c_int64, _vp]),
     ("pqg_ci_dense_h", ct.c_int, [_vp, _vp, ct.c_int64, ct.c_int]),

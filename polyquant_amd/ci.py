@@ -254,6 +254,15 @@ class DetSet:
 
     __mul__ = __matmul__
 
+    @property
+    def sigma_path(self) -> str:
+        """The sigma kernel family of this determinant set: "link_gemm" (full CI on MFMA),
+        "pair_gemm" (one particle per string), "enumeration" (every other space, including full
+        CI beyond the link GEMM's limits) or "stored_h" (after set_explicit)."""
+        names = {_lib.PQG_SIGMA_ENUMERATION: "enumeration", _lib.PQG_SIGMA_LINK_GEMM: "link_gemm",
+                 _lib.PQG_SIGMA_PAIR_GEMM: "pair_gemm", _lib.PQG_SIGMA_STORED_H: "stored_h"}
+        return names[load().pqg_ci_sigma_path(self.h)]
+
     # --- parity dumps -------------------------------------------------------------------
     def nstrings(self, part: int, spin: int) -> int:
         return load().pqg_ci_nstrings(self.h, part, spin)

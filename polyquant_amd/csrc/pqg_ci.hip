@@ -1920,6 +1920,14 @@ pqg_status pqg_ci_destroy(pqg_ci* ci) {
 
 int64_t pqg_ci_ndets(const pqg_ci* ci) { return ci ? ci->n_ref : -1; }
 
+int pqg_ci_sigma_path(const pqg_ci* ci) {
+  if (!ci) return -1;
+  if (ci->xh) return PQG_SIGMA_STORED_H;  // the order sigma_impl tests them in
+  if (ci->tensor.enabled) return PQG_SIGMA_PAIR_GEMM;
+  if (ci->lg.enabled) return PQG_SIGMA_LINK_GEMM;
+  return PQG_SIGMA_ENUMERATION;
+}
+
 int64_t pqg_ci_nstrings(const pqg_ci* ci, int part, int spin) {
   if (!ci || part < 0 || part >= ci->nsp || spin < 0 || spin > 1) return -1;
   return ci->slot[2 * part + spin].nstr;

@@ -106,3 +106,16 @@ def test_struct_layouts_match_the_ctypes_mirror(tmp_path):
     for name, cls in mirror.items():
         for f in cls._fields_:
             assert (name, f[0]) in seen, (name, f[0])
+
+
+def test_header_constants_match_the_binding():
+    """Every #define PQG_<name> <integer> of include/pqgpu.h has the same value in _lib.py."""
+    from polyquant_amd import _lib
+
+    text = open(os.path.join(ROOT, "include", "pqgpu.h")).read()
+    consts = dict(re.findall(r"#define\s+(PQG_[A-Z0-9_]+)\s+(\d+)\b", text))
+    assert "PQG_SIGMA_LINK_GEMM" in consts and "PQG_MIXED" in consts
+    for name, value in consts.items():
+        if name.startswith("PQG_NOALLOC"):
+            continue
+        assert getattr(_lib, name) == int(value), name

@@ -89,6 +89,33 @@ def test_sigma_fp64_and_mixed(name, ctx):
     assert np.max(np.abs(g32 - ref)) < 2e-5 * scale
 
 
+@pytest.mark.parametrize("name,path", [("fci_restricted_7o_3a3b", "link_gemm"), ("cisd_h2o_layout_7o_5a5b", "enumeration"),
+                                       ("fci_1p_strings_11o_1a1b", "pair_gemm"), ("fci_beyond_link_gemm_17o_2a2b", "enumeration")])
+def test_sigma_path_reported(name, path, ctx):
+    """pqg_ci_sigma_path names the kernel family each space runs; full CI past the link GEMM's
+    limits (17 orbitals: 136 column pairs) takes the enumeration kernel and still equals the
+    oracle's sigma; the stored explicit H is reported while it is set."""
+    problems = dict(PROBLEMS)
+    problems["fci_beyond_link_gemm_17o_2a2b"] = lambda: wl.small_problem(113, 17, 2, 2)
+    prob = problems[name]()
+    print(f"[gpu] case sigma path {name}", flush=True)
+    from oracle import refcpu
+    from polyquant_amd.ci import DetSet
+
+    ds = DetSet(prob, ctx)
+    assert ds.sigma_path == path
+    if name.startswith("fci_beyond"):
+        oc = refcpu.OracleCI(prob)
+        C = np.random.default_rng(9).standard_normal((oc.N, 2))
+        ref = oc.sigma(C)
+        assert np.max(np.abs(ds.sigma(C) - ref)) < 1e-12 * np.max(np.abs(ref))
+    if ds.N_dets <= 2000:
+        ds.set_explicit(True)
+        assert ds.sigma_path == "stored_h"
+        ds.set_explicit(False)
+        assert ds.sigma_path == path
+
+
 @pytest.mark.parametrize("name", list(PROBLEMS))
 def test_davidson_fp64_matches_dense(name, ctx):
     prob, ds, oc = _pair(name, ctx)
