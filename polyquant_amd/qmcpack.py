@@ -9,10 +9,10 @@ writes the group /MultiDet:
   CI_{2 part + spin}   uint64 (N_dets, N_int): row i = the padded string of determinant i
                         (address order), its 64-bit words least significant first
   Coeff, Coeff_1, ...   float64 (N_dets,): the CI vector of each state
-  NbDet                 N_dets
-  Nbits                 N_int (words per padded string)
-  nexcitedstate         the number of states
-  nstate                the largest active + frozen-core orbital count over the species
+  NbDet                 N_dets (int32)
+  Nbits                 N_int, words per padded string (uint64)
+  nexcitedstate         the number of states (int32)
+  nstate                the largest active + frozen-core orbital count over the species (int32)
 
 ``multidet_arrays`` builds these datasets from a determinant set (the strings and the address
 unfold come from the GPU determinant set through the C-ABI); ``save_multidet`` stores them as
@@ -73,10 +73,11 @@ def multidet_arrays(dets, C: np.ndarray, n_orb: Sequence[int], frozen_core: Opti
             out[f"MultiDet/CI_{2 * p + s}"] = np.ascontiguousarray(table[unfold[:, 2 * p + s]])
     for i in range(n_states):
         out["MultiDet/Coeff" if i == 0 else f"MultiDet/Coeff_{i}"] = np.ascontiguousarray(C[:, i])
-    out["MultiDet/NbDet"] = np.int64(N)
+    # scalar types as the reference passes them (int N_dets, N_states, N_mo; size_t words)
+    out["MultiDet/NbDet"] = np.int32(N)
     out["MultiDet/Nbits"] = np.uint64(num_int)
-    out["MultiDet/nexcitedstate"] = np.int64(n_states)
-    out["MultiDet/nstate"] = np.int64(max(n + f for n, f in zip(n_orb, fc)))
+    out["MultiDet/nexcitedstate"] = np.int32(n_states)
+    out["MultiDet/nstate"] = np.int32(max(n + f for n, f in zip(n_orb, fc)))
     return out
 
 

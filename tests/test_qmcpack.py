@@ -82,6 +82,9 @@ def test_multidet_assembly_matches_oracle(fc, tmp_path):
         else:
             assert int(g) == v, k
     assert got["MultiDet/CI_0"].shape == (oc.N, 2 if fc[0] == 60 else 1)
+    # dataset types of the reference's arguments (hdf5_utilities.cpp:399-400, 454-467)
+    assert got["MultiDet/NbDet"].dtype == np.int32 and got["MultiDet/nstate"].dtype == np.int32
+    assert got["MultiDet/nexcitedstate"].dtype == np.int32 and got["MultiDet/Nbits"].dtype == np.uint64
     path = tmp_path / "multidet.npz"
     qmcpack.save_multidet(str(path), got)
     back = np.load(path)
