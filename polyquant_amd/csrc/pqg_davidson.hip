@@ -34,7 +34,8 @@ constexpr int kTB = 256;
 
 // out[a][b] = sum over the chunks of the partials in a fixed order (one block per output)
 __global__ __launch_bounds__(256) void k_tn_reduce(const double* __restrict__ part, int nchunk, int na, int nb,
-                                                   double* __restrict__ out, int ldo) {
+                                                   double* __restrict__ out, int ldo, const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;  // a gated product that is not needed (uniform over the grid)
   const int o = blockIdx.x, G = na * nb;
   double v = 0.0;
   for (int c = threadIdx.x; c < nchunk; c += 256) v += part[(size_t)c * G + o];
@@ -56,7 +57,9 @@ __global__ __launch_bounds__(256) void k_tn_reduce(const double* __restrict__ pa
 constexpr int kTnQA = 8;
 __global__ __launch_bounds__(256) void k_tn_part(const double* __restrict__ A, int64_t lda, int na,
                                                  const double* __restrict__ B, int64_t ldb, int nb, int64_t nrows,
-                                                 int64_t rpb, double* __restrict__ part) {
+                                                 int64_t rpb, double* __restrict__ part,
+                                                 const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const int lane = threadIdx.x & 63;
   const int task = (threadIdx.x >> 6) + blockIdx.y * (blockDim.x >> 6);
   const int ta = (na + kTnQA - 1) / kTnQA;
@@ -108,7 +111,9 @@ __global__ __launch_bounds__(256) void k_tn_part(const double* __restrict__ A, i
 template <int NB>
 __global__ __launch_bounds__(kTB) void k_nn(const double* A0, const double* A1, int64_t lda, int ka,
                                             const double* __restrict__ Bs, int ldb, int nb, double* out0,
-                                            double* out1, int64_t ldo, int64_t nrows, double alpha, double beta) {
+                                            double* out1, int64_t ldo, int64_t nrows, double alpha, double beta,
+                                            const int* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   __shared__ double bs[64 * NB];
   const int64_t r = ((int64_t)blockIdx.x * kTB + threadIdx.x) * 2;
   const double* A = blockIdx.y ? A1 : A0;
@@ -782,6 +787,8 @@ void Davidson::setup() {
   rgram_.reset(ctx, 64 * 64);
   mbuf_.reset(ctx, 64 * 64);
   nk_dev_.reset(ctx, 2);
+  reorth_dev_.reset(ctx, 1);
+  small2_.reset(ctx, 64 * 64);
   status_dev_.reset(ctx, 130);
   {
     std::lock_guard<std::recursive_mutex> dl(device_op_mutex());
@@ -812,7 +819,7 @@ static void ts_check(const double* p, int64_t ld) {
 }
 
 void Davidson::ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, const double* B, int64_t ldb, int nb,
-                          double* out, int ldo) {
+                          double* out, int ldo, const int* gate) {
   ts_check(A, lda);
   ts_check(B, ldb);
   PQG_REQUIRE(na >= 1 && nb >= 1 && na <= 64 && nb <= 64, PQG_ERR_ARG, "tall-skinny product shape");
@@ -828,31 +835,31 @@ void Davidson::ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, 
   const int tasks = ((na + kTnQA - 1) / kTnQA) * ((nb + 3) / 4);
   const int per = std::min(tasks, 4);
   dim3 grid((unsigned)nblk, (unsigned)((tasks + per - 1) / per));
-  k_tn_part<<<grid, 64 * per, 0, st>>>(A, lda, na, B, ldb, nb, N_, rpb, part_.get());
+  k_tn_part<<<grid, 64 * per, 0, st>>>(A, lda, na, B, ldb, nb, N_, rpb, part_.get(), gate);
   PQG_HIP(hipGetLastError());
-  k_tn_reduce<<<(unsigned)G, 256, 0, st>>>(part_.get(), (int)nblk, na, nb, out, ldo);
+  k_tn_reduce<<<(unsigned)G, 256, 0, st>>>(part_.get(), (int)nblk, na, nb, out, ldo, gate);
   PQG_HIP(hipGetLastError());
 }
 
 static void nn_launch(dim3 g, hipStream_t st, const double* A0, const double* A1, int64_t lda, int ka,
                       const double* Bs, int ldb, int nb, double* out0, double* out1, int64_t ldo, int64_t nrows,
-                      double alpha, double beta) {
+                      double alpha, double beta, const int* gate = nullptr) {
   if (nb <= 4)
-    k_nn<4><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta);
+    k_nn<4><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta, gate);
   else if (nb <= 8)
-    k_nn<8><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta);
+    k_nn<8><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta, gate);
   else
-    k_nn<16><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta);
+    k_nn<16><<<g, kTB, 0, st>>>(A0, A1, lda, ka, Bs, ldb, nb, out0, out1, ldo, nrows, alpha, beta, gate);
   PQG_HIP(hipGetLastError());
 }
 
 void Davidson::ts_gemm_nn(hipStream_t st, const double* A, int64_t lda, int ka, const double* Bs, int ldb, int nb,
-                          double* out, int64_t ldo, double alpha, double beta) {
+                          double* out, int64_t ldo, double alpha, double beta, const int* gate) {
   ts_check(A, lda);
   ts_check(out, ldo);
   PQG_REQUIRE(ka >= 1 && ka <= 64 && nb >= 1, PQG_ERR_ARG, "tall-skinny product shape");
   PQG_REQUIRE(out != A || nb <= 16, PQG_ERR_ARG, "in-place product wider than one column chunk");
-  nn_launch(dim3(g1((N_ + 1) / 2, kTB)), st, A, A, lda, ka, Bs, ldb, nb, out, out, ldo, N_, alpha, beta);
+  nn_launch(dim3(g1((N_ + 1) / 2, kTB)), st, A, A, lda, ka, Bs, ldb, nb, out, out, ldo, N_, alpha, beta, gate);
 }
 
 void Davidson::ts_gemm_nn2(hipStream_t st, const double* A0, const double* A1, int64_t lda, int ka, const double* Bs,
@@ -1001,10 +1008,35 @@ void Davidson::run_sigma(hipStream_t st) {
   sigma_calls_ += k;
 }
 
-// Orthogonalise T (nt columns) against V (m columns) with two block Gram-Schmidt sweeps, then
-// orthonormalise the block by two Cholesky-QR passes (columns that vanish are dropped) into
-// V[:, m:m+nt], all on the stream; the kept count (and the second pass's count, which must agree)
-// lands in nk_dev_[0..1].  Nothing is read back here; finish_append() commits the columns.
+// PQG_DAV_REORTH=1: the second Gram-Schmidt sweep always runs (the gate below is then always open)
+static int reorth_always() {
+  static const int on = [] {
+    const char* e = std::getenv("PQG_DAV_REORTH");
+    return e && std::atoi(e) != 0 ? 1 : 0;
+  }();
+  return on;
+}
+
+// Second Gram-Schmidt sweep needed?  DGKS criterion: yes when a column kept less than 1/sqrt(2) of
+// its norm in the first sweep (g0: Gram before, g1: after; columns of converged roots, which the
+// Cholesky-QR drops anyway, do not count).  One thread; the flag gates the second sweep.
+__global__ void k_reorth_gate(const double* __restrict__ g0, const double* __restrict__ g1, int n,
+                              const double* __restrict__ rg, double skip, int always, int* __restrict__ gate) {
+  if (threadIdx.x != 0) return;
+  int need = always;
+  for (int j = 0; j < n; j++) {
+    if (rg && rg[j + j * 64] < skip * skip) continue;
+    const double before = g0[j + j * 64], after = g1[j + j * 64];
+    if (before > 0.0 && after < 0.5 * before) need = 1;
+  }
+  *gate = need;
+}
+
+// Orthogonalise T (nt columns) against V (m columns) by block Gram-Schmidt, a second sweep only
+// where the first lost most of a column (k_reorth_gate), then orthonormalise the block by two
+// Cholesky-QR passes (columns that vanish are dropped) into V[:, m:m+nt], all on the stream; the
+// kept count (and the second pass's count, which must agree) lands in nk_dev_[0..1].  Nothing is
+// read back here; finish_append() commits the columns.
 void Davidson::append_async(hipStream_t st, double* T, int nt, const double* g0, const double* rg) {
   nt = std::min(nt, nmax_ - m_);
   pending_nt_ = nt;
@@ -1013,12 +1045,16 @@ void Davidson::append_async(hipStream_t st, double* T, int nt, const double* g0,
     ts_gemm_tn(st, T, ld_, nt, T, ld_, nt, gram0_.get(), 64);
     g0 = gram0_.get();
   }
-  for (int pass = 0; pass < 2; pass++) {
-    ts_gemm_tn(st, V_.get(), ld_, m_, T, ld_, nt, small_.get(), 64);
-    ts_gemm_nn(st, V_.get(), ld_, m_, small_.get(), 64, nt, T, ld_, -1.0, 1.0);
-  }
-  // pass 1: drop vanishing columns, orthonormalise the rest straight into V[:, m:]
+  ts_gemm_tn(st, V_.get(), ld_, m_, T, ld_, nt, small_.get(), 64);
+  ts_gemm_nn(st, V_.get(), ld_, m_, small_.get(), 64, nt, T, ld_, -1.0, 1.0);
   ts_gemm_tn(st, T, ld_, nt, T, ld_, nt, small_.get(), 64);
+  k_reorth_gate<<<1, 64, 0, st>>>(g0, small_.get(), nt, rg, p_.tol, reorth_always(), reorth_dev_.get());
+  PQG_HIP(hipGetLastError());
+  const int* gate = reorth_dev_.get();
+  ts_gemm_tn(st, V_.get(), ld_, m_, T, ld_, nt, small2_.get(), 64, gate);
+  ts_gemm_nn(st, V_.get(), ld_, m_, small2_.get(), 64, nt, T, ld_, -1.0, 1.0, gate);
+  ts_gemm_tn(st, T, ld_, nt, T, ld_, nt, small_.get(), 64, gate);
+  // Cholesky-QR pass 1: drop vanishing columns, orthonormalise the rest straight into V[:, m:]
   k_cholqr_small<<<1, 64, 0, st>>>(small_.get(), nullptr, nt, g0, 1e-10, mbuf_.get(), nk_dev_.get(), rg, p_.tol);
   PQG_HIP(hipGetLastError());
   double* dst = V_.get() + (int64_t)m_ * ld_;

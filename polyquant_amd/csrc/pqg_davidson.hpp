@@ -91,10 +91,11 @@ class Davidson {
   void restart_to_ritz(hipStream_t st, int keep);
   void rayleigh_ritz(hipStream_t st);
   void ritz_residual(hipStream_t st);
+  // gate: a device flag; when given, the product runs only if the flag is nonzero
   void ts_gemm_tn(hipStream_t st, const double* A, int64_t lda, int na, const double* B, int64_t ldb, int nb,
-                  double* out, int ldo);
+                  double* out, int ldo, const int* gate = nullptr);
   void ts_gemm_nn(hipStream_t st, const double* A, int64_t lda, int ka, const double* Bs, int ldb, int nb,
-                  double* out, int64_t ldo, double alpha, double beta);
+                  double* out, int64_t ldo, double alpha, double beta, const int* gate = nullptr);
   void ts_gemm_nn2(hipStream_t st, const double* A0, const double* A1, int64_t lda, int ka, const double* Bs, int ldb,
                    int nb, double* out0, double* out1, int64_t ldo);
   double* Rp() const { return RT_.get(); }
@@ -112,6 +113,8 @@ class Davidson {
   bool gdk_ = true;         // thick restart keeps the previous Ritz vectors (GD+k)
   int yk_rows_ = 0;         // rows of Yk_ (0: none saved for this basis)
   DevBuf<int> nk_dev_;
+  DevBuf<int> reorth_dev_;       // second Gram-Schmidt pass needed (1) or not (0)
+  DevBuf<double> small2_;        // coefficients of the second Gram-Schmidt pass
   PinnedHost status_pin_;        // pinned host copy of status_dev_
   double* status_h_ = nullptr;
   int pending_nt_ = 0;
