@@ -65,12 +65,11 @@ struct LinkTab {
 struct LinkGemm {
   bool enabled = false;
   LinkTab tab[kSlots];
-  DevBuf<char> U;             // GEMM output for one column chunk
-  DevBuf<char> U2;            // second output block: the gather of a chunk overlaps the next GEMM
+  DevBuf<char> U;             // GEMM output for one column chunk (FP32: two half blocks)
   DevBuf<uint32_t> P;         // BF16x3 kernel: the vector of a pass as bf16 (hi, lo) pairs
   DevBuf<float> V32s[kSlots]; // FP32 copies of the same-set coupling blocks
   hipStream_t side = nullptr; // the overlapped gathers (default priority), created on first use
-  hipEvent_t ev[4] = {};      // GEMM done into U / U2, gather done from U / U2
+  hipEvent_t ev[4] = {};      // GEMM done into block 0 / 1, gather done from block 0 / 1
   LinkGemm() = default;
   LinkGemm(const LinkGemm&) = delete;
   LinkGemm& operator=(const LinkGemm&) = delete;

@@ -561,7 +561,7 @@ static const float* xblock(pqg_ci* ci, int t, int u, float) { return ci->X32[t][
 
 // U bytes per column of a pass, and the chunk width that keeps U within its budget
 static size_t lg_per_col(const LgPass& d, size_t w) { return (size_t)d.n_r * d.L * w; }
-// share: the part of the budget one output block takes (1, or 2/3 with two blocks)
+// share: the part of the budget one output block takes (1, or 1/2 with two blocks)
 static int lg_chunk(const pqg_ci* ci, const LgPass& d, int n_c, size_t w, int nt, double share = 1.0) {
   const size_t per_col = lg_per_col(d, w);
   const size_t want = std::min<size_t>((size_t)(share * (double)lg_budget_bytes(ci->n_int, w)),
@@ -693,14 +693,16 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
     a.bstride = sh.stride;
   }
   // FP32: two output blocks when the columns take more than one chunk; the gather of chunk i
-  // (HBM-bound) runs on a side stream while the GEMM of chunk i + 1 writes the other block, each
-  // block taking 2/3 of the budget.  Not under graph capture or in the allocation-free call;
-  // PQG_LG_OVERLAP=0 turns it off.  Measured at 16 orbitals 8a8b, sigma of 4 vectors: BF16x3
-  // 280.9 ms against 296.6 ms; FP64 858 ms against 864 ms, the FP64 GEMM slowing by what the
-  // hidden gathers save (five chunks instead of four, shared bandwidth), so FP64 keeps one stream.
+  // (HBM-bound) runs on a side stream while the GEMM of chunk i + 1 writes the other block.  Both
+  // blocks are halves of the one U buffer (half the budget each), so the overlap needs no memory
+  // beyond what one block took (the serving loop's admission estimate counts one).  Not under
+  // graph capture or in the allocation-free call; PQG_LG_OVERLAP=0 turns it off.  Measured at 16
+  // orbitals 8a8b, sigma of 4 vectors: BF16x3 288.9 ms against 296.6 ms; FP64 858 ms against
+  // 864 ms, the FP64 GEMM slowing by what the hidden gathers save (five chunks instead of four,
+  // shared bandwidth), so FP64 keeps one stream.
   bool overlap = sizeof(T) == 4 && !fixed_scratch() && lg_overlap_enabled() &&
                  lg_chunk(ci, d, a.n_c, sizeof(T), nt) < a.n_c && !lg_capturing(st);
-  const double share = overlap ? 2.0 / 3.0 : 1.0;
+  const double share = overlap ? 0.5 : 1.0;
   // U holds n_r * L * ldu values (ldu = cw rounded up to 4); sized for the larger of the two
   // passes so it grows once
   int cw = lg_chunk(ci, d, a.n_c, sizeof(T), nt, share);
@@ -713,9 +715,21 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
                                                                         lg_nt(o, n_co, sizeof(T), ksplit, kqo),
                                                                         share) + 3));
   }
+  size_t half = 0;  // byte offset of the second block in U
   if (overlap) {
-    grow(lg.U, ci->ctx, need, st);
-    grow(lg.U2, ci->ctx, need, st);
+    need = (need + 255) / 256 * 256;
+    grow(lg.U, ci->ctx, 2 * need, st);
+    half = lg.U.n / 2 / 256 * 256;  // >= need: U holds at least two blocks
+    // when U is larger (an FP64 pass grew it), wider chunks: as wide as half of U holds, at most
+    // the one-block width, balanced as lg_chunk balances them
+    const int64_t fit = (int64_t)(half / lg_per_col(d, sizeof(T))) - 3;
+    const int cw1 = lg_chunk(ci, d, a.n_c, sizeof(T), nt);
+    if (fit > cw) {
+      int w = (int)std::min<int64_t>(fit, cw1);
+      w = std::max(nt, w / nt * nt);
+      const int nch = (a.n_c + w - 1) / w;
+      cw = std::max(cw, std::min(w, ((a.n_c + nch - 1) / nch + nt - 1) / nt * nt));
+    }
     lg_side_init(lg);
   } else if (FixedScratch* fs = fixed_scratch()) {
     // allocation-free call: U is a view of what the fixed workspace has left, and the column
@@ -740,7 +754,7 @@ static void lg_pass(pqg_ci* ci, int pass, const T* C, T* S, bool accumulate, hip
     const int blk = chunk & 1;
     hipStream_t gst = st;
     if (overlap) {
-      a.U = reinterpret_cast<T*>(blk ? lg.U2.get() : lg.U.get());
+      a.U = reinterpret_cast<T*>(lg.U.get() + (blk ? half : 0));
       // the block is free once the gather of chunk - 2 has read it
       if (chunk >= 2) PQG_HIP(hipStreamWaitEvent(st, lg.ev[2 + blk], 0));
       gst = lg.side;
